@@ -1,0 +1,230 @@
+"""Device state of one pressure simulation over a batch of shots, and its launches.
+
+A `Propagator` binds a simulation and its material to the library: it prepares
+the inverse inertia and the harmonic cell stiffness once (`cw_scalar_prepare`),
+uploads the PML profiles, buckets source and sensor points by the kernels' tiles,
+and launches the fused forward and adjoint steps on torch's current stream. It
+plays the part of the reference's closure factories (`define_step`,
+`define_excitation`, `define_get_signal`, `define_adjoint_gradient`,
+`define_adjoint_pml`, cuwave/scalar.py:279-811, cuwave/wave.py:956-1033), minus
+the per-step host work: a call advances a whole range of steps.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _abi
+from .boundary import pml_profiles, pml_widths
+
+
+@dataclass
+class PointList:
+    """Points bucketed by tile on the device, and the struct that names them."""
+
+    points: _abi.Points
+    keep: tuple  # the tensors the struct points into
+
+
+class Propagator:
+    """The kernels of one simulation bound to its material, for `nbatch` shots at once."""
+
+    def __init__(self, sim, mat: dict, nbatch: int):
+        self.device = _abi.require_device()
+        self.sim, self.nbatch = sim, int(nbatch)
+        self.grid = sim.abi_grid(self.nbatch)
+        dtype = sim.dtype
+        self.stiff = mat["stiff"]
+        self.minv = mat.get("minv")
+        self.cell = torch.empty((sim.ndim, *sim.Nx_padded), dtype=dtype, device=self.device)
+        self.inv = torch.empty(sim.Nx_padded, dtype=dtype, device=self.device)
+        self.mat = _abi.Materials(
+            _abi.ptr(self.stiff), _abi.ptr(self.minv), _abi.ptr(self.cell), _abi.ptr(self.inv)
+        )
+        _abi.call("cw_scalar_prepare", ctypes.byref(self.grid), ctypes.byref(self.mat), _abi.stream())
+        tiles = _abi.I32()
+        _abi.call("cw_tile_count", ctypes.byref(self.grid), ctypes.byref(tiles))
+        self.ntiles = tiles.value
+        self.field = int(np.prod(sim.Nx_padded))
+        # the layer: widths, profiles and slab sizes per lined axis
+        self.pml = _abi.Pml()
+        self.slabs = []  # (axis, slab nodes)
+        self.profiles = []
+        if sim.pml is not None:
+            for d, ((lo, hi), prof) in enumerate(zip(pml_widths(sim), pml_profiles(sim))):
+                if prof is None:
+                    continue
+                profile = torch.as_tensor(prof, device=self.device)
+                self.profiles.append(profile)
+                self.pml.lo[d], self.pml.hi[d] = lo, hi
+                self.pml.profile[d] = _abi.ptr(profile)
+                plane = self.field // sim.Nx_padded[d]
+                self.slabs.append((d, (lo + hi) * plane))
+
+    # ------------------------------------------------------------ memory
+    def aux_memory(self) -> list[torch.Tensor]:
+        """Zeroed forward layer memory, (nbatch, 3, slab) per lined axis."""
+        return [torch.zeros((self.nbatch, 3, size), dtype=self.sim.dtype, device=self.device)
+                for _, size in self.slabs]
+
+    def eta_memory(self) -> list[torch.Tensor]:
+        """Zeroed adjoint layer memory, two alternating buffers per lined axis."""
+        return [torch.zeros((2, self.nbatch, 3, size), dtype=self.sim.dtype, device=self.device)
+                for _, size in self.slabs]
+
+    def _layer(self, aux=None, eta=None) -> _abi.Pml:
+        layer = _abi.Pml()
+        ctypes.memmove(ctypes.byref(layer), ctypes.byref(self.pml), ctypes.sizeof(layer))
+        for k, (d, _) in enumerate(self.slabs):
+            if aux is not None:
+                layer.aux[d] = _abi.ptr(aux[k])
+            if eta is not None:
+                layer.eta[d][0] = _abi.ptr(eta[k][0])
+                layer.eta[d][1] = _abi.ptr(eta[k][1])
+        return layer
+
+    @staticmethod
+    def levels(tensor: torch.Tensor, offset: int = 0) -> _abi.Levels:
+        """Levels over a (slots, nbatch, *field) tensor, level k in slot (k + offset) % slots."""
+        slots, nbatch = tensor.shape[:2]
+        field = int(np.prod(tensor.shape[2:]))
+        return _abi.Levels(_abi.ptr(tensor), nbatch * field, field, slots, offset)
+
+    # ------------------------------------------------------------ points
+    def point_list(self, lin, shot, col, columns: int, weight=None, shot_stride: int = 0) -> PointList:
+        """Bucket points by the kernels' tiles and upload them (cw_point_lists)."""
+        lin = np.ascontiguousarray(lin, dtype=np.int32)
+        count = len(lin)
+        starts = np.zeros(self.ntiles + 1, dtype=np.int32)
+        order = np.zeros(max(count, 1), dtype=np.int32)
+        _abi.call("cw_point_lists", ctypes.byref(self.grid), lin.ctypes.data, count,
+                  starts.ctypes.data, order.ctypes.data)
+        order = order[:count]
+        dev = self.device
+
+        def upload(a, dtype):
+            return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=dev)
+
+        t_starts = upload(starts, torch.int32)
+        t_lin = upload(lin[order], torch.int32)
+        t_shot = upload(np.broadcast_to(np.asarray(shot, dtype=np.int32), (count,))[order], torch.int32)
+        t_col = upload(np.asarray(col, dtype=np.int32)[order], torch.int32)
+        keep = [t_starts, t_lin, t_shot, t_col]
+        pts = _abi.Points(_abi.ptr(t_starts), _abi.ptr(t_lin), _abi.ptr(t_shot), _abi.ptr(t_col),
+                          None, int(columns), int(shot_stride))
+        if weight is not None:
+            t_w = upload(np.asarray(weight, dtype=self.sim.np_dtype)[order], self.sim.dtype)
+            keep.append(t_w)
+            pts.weight = _abi.ptr(t_w)
+        return PointList(pts, tuple(keep))
+
+    # ------------------------------------------------------------ launches
+    def forward(self, u: torch.Tensor, t0: int, t1: int, *, offset: int = 0, aux=None,
+                sources: PointList | None = None, signal: torch.Tensor | None = None,
+                sensors: PointList | None = None, record: torch.Tensor | None = None,
+                strip: dict | None = None) -> None:
+        """Steps t0 .. t1 - 1 into the levels of `u` (cw_scalar_forward)."""
+        a = _abi.ForwardArgs()
+        a.grid = self.grid
+        a.mat = self.mat
+        a.pml = self._layer(aux=aux)
+        a.u = self.levels(u, offset)
+        if sources is not None:
+            a.sources = sources.points
+            a.signal = _abi.ptr(signal)
+        if sensors is not None:
+            a.sensors = sensors.points
+            a.record = _abi.ptr(record)
+        if strip is not None:
+            a.strip = _abi.Strip(_abi.ptr(strip["map"]), _abi.ptr(strip["record"]), strip["count"],
+                                 strip["first"], strip["rows"], int(strip.get("bf16", 0)))
+        a.steps = self.sim.N
+        _abi.call("cw_scalar_forward", ctypes.byref(a), int(t0), int(t1), _abi.stream())
+
+    def backward(self, lam: torch.Tensor, u: torch.Tensor, n0: int, n1: int, *, eta,
+                 g_mass: torch.Tensor, g_stiff: torch.Tensor | None = None, u_offset: int = 0,
+                 reverse: bool = False, sensors: PointList | None = None,
+                 adjoint: torch.Tensor | None = None, sources: PointList | None = None,
+                 signal: torch.Tensor | None = None, strip: dict | None = None,
+                 per_shot: bool = False, combine: tuple[float, float] | None = None,
+                 skip_layer: bool = False) -> None:
+        """Adjoint levels n1 - 1 down to n0 (cw_scalar_backward)."""
+        a = _abi.BackwardArgs()
+        a.grid = self.grid
+        a.mat = self.mat
+        a.pml = self._layer(eta=eta)
+        a.lam = self.levels(lam, 0)
+        a.u = self.levels(u, u_offset)
+        a.reverse = int(reverse)
+        if sensors is not None:
+            a.sensors = sensors.points
+            a.adjoint_signal = _abi.ptr(adjoint)
+        if sources is not None:
+            a.sources = sources.points
+            a.signal = _abi.ptr(signal)
+        if strip is not None:
+            a.strip = _abi.Strip(_abi.ptr(strip["map"]), _abi.ptr(strip["record"]), strip["count"],
+                                 strip["first"], strip["rows"], int(strip.get("bf16", 0)))
+        a.g_mass = _abi.ptr(g_mass)
+        a.g_stiff = _abi.ptr(g_stiff)
+        a.grad_batch_stride = self.field if per_shot else 0
+        if combine is not None:
+            a.combine = 1
+            a.jac_mass, a.jac_stiff = float(combine[0]), float(combine[1])
+        a.skip_layer = int(skip_layer)
+        a.mass_factor = float(self.sim.np_dtype(self.sim.adjoint_mass_factor()))
+        a.steps = self.sim.N
+        _abi.call("cw_scalar_backward", ctypes.byref(a), int(n0), int(n1), _abi.stream())
+
+    def superposed(self, u0, u1, u2, acc_mass, acc_stiff, ft: float, fs: float, step: bool) -> None:
+        """One superposition step, or the Frechet densities of a triplet (cw_scalar_superposed)."""
+        if self.nbatch != 1:
+            raise ValueError("the superposition step takes one shot")
+        a = _abi.SuperposedArgs()
+        a.grid = self.grid
+        a.mat = self.mat
+        a.u0, a.u1, a.u2 = _abi.ptr(u0), _abi.ptr(u1), _abi.ptr(u2)
+        a.acc_mass, a.acc_stiff = _abi.ptr(acc_mass), _abi.ptr(acc_stiff)
+        a.ft, a.fs, a.step = float(ft), float(fs), int(step)
+        _abi.call("cw_scalar_superposed", ctypes.byref(a), _abi.stream())
+
+
+# ------------------------------------------------------------------ shot points
+def source_points(prop: Propagator, mat: dict, sources) -> tuple[PointList, torch.Tensor]:
+    """Every shot's source points and one (N, columns) signal table, shot b's columns its own.
+
+    The points of all shots share one list: each entry names its shot, so a
+    launch adds shot b's sources to shot b's field only (cuwave/wave.py:985-993).
+    """
+    from .wave import as_numpy
+
+    sim = prop.sim
+    lins, shots, cols, weights, signals = [], [], [], [], []
+    column = 0
+    for b, source in enumerate(sources):
+        pos = np.asarray(as_numpy(source.position))
+        lin = sim.flatten_indices(pos)
+        sig = as_numpy(source.signal)
+        if sig.ndim != 2 or sig.shape[0] < sim.N or sig.shape[1] != len(lin):
+            raise ValueError(f"shot {b}: signal needs ({sim.N}, {len(lin)}), not {sig.shape}")
+        lins.append(lin)
+        shots.append(np.full(len(lin), b, dtype=np.int32))
+        cols.append(column + np.arange(len(lin), dtype=np.int32))
+        weights.append(sim.excitation_weights(mat, lin))
+        signals.append(np.asarray(sig[: sim.N], dtype=sim.np_dtype))
+        column += len(lin)
+    table = np.ascontiguousarray(np.concatenate(signals, axis=1), dtype=sim.np_dtype)
+    pts = prop.point_list(np.concatenate(lins), np.concatenate(shots), np.concatenate(cols),
+                          column, weight=np.concatenate(weights))
+    return pts, torch.as_tensor(table, device=prop.device)
+
+
+def sensor_points(prop: Propagator, sensors, weight=None, shot_stride: int = 0) -> PointList:
+    """Sensor nodes every shot shares, column k the k-th sensor."""
+    lin = prop.sim.flatten_indices(sensors)
+    return prop.point_list(lin, -1, np.arange(len(lin), dtype=np.int32), len(lin),
+                           weight=weight, shot_stride=shot_stride)

@@ -1,0 +1,467 @@
+"""Adjoint sensitivities, same arguments and same return as the reference's.
+
+`sensitivity` stores the forward field (N + 2 grids) and steps the exact
+transpose of the discretisation backwards (cuwave/sensitivity.py:384-497).
+`reconstruction_sensitivity` stores only the strip of nodes the PML makes
+irreversible and rebuilds the forward field by a reverse march fused into the
+adjoint step itself (cuwave/sensitivity.py:500-750); `checkpoints` cuts the run
+into segments replayed from snapshots in treeverse order, so the memory no longer
+grows with N. `superposition_sensitivity` reads the cross term off the diagonal of
+B(u + k lambda) in three slots, for a lossless operator (cuwave/sensitivity.py:
+753-893).
+
+All return `(cost, {"mass": ..., "stiff": ...}, traces, info)`: the gradients
+with respect to the inertia and stiffness coefficients over the padded grid,
+as torch tensors on the GPU.
+"""
+
+from __future__ import annotations
+
+import math
+import warnings
+
+import numpy as np
+import scipy.ndimage as ndi
+import torch
+
+from .boundary import pml_nodes, require_off_held_walls
+from .engine import Propagator, sensor_points, source_points
+from .wave import Simulation, Source, as_numpy
+
+# cancellation past which B(w, w) - B(u, u) has eaten too much of the mantissa
+CANCELLATION_LIMIT = {"float32": 1e5, "float64": 1e12}
+ADJOINT_DELAY = 1  # lines the superposition adjoint up with the forward triplet
+
+
+# ------------------------------------------------------------------ objectives
+def l2_misfit(observed):
+    """Objective factory: J = 1/2 sum (traces - observed)^2, and its derivative."""
+
+    def objective(traces):
+        data = torch.as_tensor(as_numpy(observed) if not isinstance(observed, torch.Tensor)
+                               else observed, device=traces.device, dtype=traces.dtype)
+        residual = traces - data
+        return 0.5 * float(torch.sum(residual**2)), residual
+
+    return objective
+
+
+def windowed_misfit(observed, window):
+    """`l2_misfit` restricted to a time window, its derivative carrying the window twice."""
+
+    def objective(traces):
+        data = torch.as_tensor(as_numpy(observed), device=traces.device, dtype=traces.dtype)
+        w = torch.as_tensor(as_numpy(window), device=traces.device, dtype=traces.dtype)
+        residual = w * (traces - data)
+        return 0.5 * float(torch.sum(residual**2)), w * residual
+
+    return objective
+
+
+def adjoint_signal(sim: Simulation, dphi, sensors, scale: float = 1.0, delay: int = 0) -> torch.Tensor:
+    """The adjoint excitation: `dphi` reversed in time, over W and the source factor.
+
+    (cuwave/sensitivity.py:272-292). `delay` drops entries off the front and pads
+    zeros at the back; `scale` is the superposition k.
+    """
+    dphi = torch.as_tensor(dphi, dtype=sim.dtype)
+    w = torch.as_tensor(sim.adjoint_weights(sensors), device=dphi.device)
+    signal = torch.flip(dphi, (0,)) / w
+    if delay:
+        signal = torch.cat((signal[delay:], torch.zeros_like(signal[:delay])))
+    if scale != 1.0:
+        signal = signal * torch.tensor(scale, dtype=sim.dtype, device=signal.device)
+    return signal.contiguous()
+
+
+# --------------------------------------------------------------------- helpers
+def _require(sim: Simulation, source: Source, sensors) -> None:
+    sim.require_interior(sensors, "sensor")
+    sim.require_interior(source.position, "source")
+    require_off_held_walls(sim, np.asarray(as_numpy(source.position)))
+
+
+def _finalize(sim: Simulation, grads: dict, device) -> dict:
+    """Weight the accumulators by the cell weights W (cuwave/wave.py:336-344)."""
+    w = torch.as_tensor(sim.cell_weights(), device=device)
+    for f in grads.values():
+        f.mul_(w)
+    return grads
+
+
+def _zero_outside(grads: dict, mask: np.ndarray, device) -> dict:
+    """Assign zero where `mask` holds, so a NaN there cannot survive as 0 * NaN."""
+    m = torch.as_tensor(mask, device=device)
+    for f in grads.values():
+        f[m] = 0.0
+    return grads
+
+
+def reconstruction_nodes(sim: Simulation, indicator=None, span: int | None = None):
+    """Nodes a reverse march must replay, and where its gradient stays exact.
+
+    A PML node is irreversible; the strip is every irreversible interior node
+    within the reach of a reversible one, diagonals included
+    (cuwave/sensitivity.py:160-195). Returns the (ndim, num) strip indices in row-major
+    order and the mask of reversible interior nodes.
+    """
+    interior = np.zeros(sim.Nx_padded, dtype=bool)
+    interior[tuple(slice(1, n - 1) for n in sim.Nx)] = True
+    if sim.pml is None:
+        return np.zeros((sim.ndim, 0), dtype=np.int32), interior
+    reversible = interior & ~pml_nodes(sim)
+    square = ndi.generate_binary_structure(sim.ndim, sim.ndim)
+    reach = ndi.binary_dilation(reversible, structure=square, iterations=sim.reach,
+                                brute_force=True)
+    strip = np.stack(np.nonzero(interior & ~reversible & reach)).astype(np.int32)
+    return strip, reversible
+
+
+def segment_split(lo: int, hi: int, free: int) -> int:
+    """The binomial treeverse split of segments lo .. hi with `free` snapshot slots.
+
+    `free` slots and r replays reverse at most C(free + r, free) segments
+    (Griewank & Walther 2000, https://doi.org/10.1145/347837.347846).
+    """
+    r = 0
+    while math.comb(free + r, free) < hi - lo:
+        r += 1
+    return max(lo + 1, hi - math.comb(free - 1 + r, free - 1))
+
+
+def checkpoint_schedule(segments: int, slots: int) -> list[tuple]:
+    """Treeverse over `segments` with `slots` snapshots (cuwave/sensitivity.py:243-269).
+
+    Actions in order: ("restore", k), ("store", k), ("drop", k), ("advance", a, b)
+    and ("leaf", k), which replays segment k and reverses it.
+    """
+    actions = []
+
+    def reverse(lo, hi, free):
+        while hi - lo > 1:
+            if free == 0:
+                for k in range(hi - 1, lo, -1):
+                    actions.extend((("restore", lo), ("advance", lo, k), ("leaf", k)))
+                break
+            mid = segment_split(lo, hi, free)
+            actions.extend((("restore", lo), ("advance", lo, mid), ("store", mid)))
+            reverse(mid, hi, free - 1)
+            actions.append(("drop", mid))
+            hi = mid
+        actions.extend((("restore", lo), ("leaf", lo)))
+
+    reverse(0, segments, slots)
+    return actions
+
+
+# ------------------------------------------------------------------ sensitivity
+def sensitivity(sim: Simulation, source: Source, indicator, sensors, objective):
+    """Cost and d(cost)/d(mass, stiff) by the exact adjoint over a stored history.
+
+    The forward field is kept for all N + 2 levels; the adjoint steps the exact
+    transpose with both gradient densities fused in and the transposed PML, the
+    layer then frozen out of the gradient (cuwave/sensitivity.py:384-497).
+    """
+    _require(sim, source, sensors)
+    mat = sim.build_materials(indicator)
+    prop = Propagator(sim, mat, 1)
+    dev = prop.device
+    src, signal = source_points(prop, mat, [source])
+    rec = sensor_points(prop, sensors)
+    nrec = rec.points.columns
+    V = torch.zeros((sim.N + 2, 1, *sim.Nx_padded), dtype=sim.dtype, device=dev)
+    record = torch.zeros((1, sim.N, nrec), dtype=sim.dtype, device=dev)
+    prop.forward(V, 0, sim.N, offset=2, aux=prop.aux_memory(), sources=src, signal=signal,
+                 sensors=rec, record=record)
+    um = record[0]
+    cost, dphi = objective(um)
+    adjoint = adjoint_signal(sim, dphi, sensors)
+    lin = sim.flatten_indices(sensors)
+    adj = sensor_points(prop, sensors, weight=sim.excitation_weights(mat, lin),
+                        shot_stride=sim.N * nrec)
+    P = torch.zeros((2, 1, *sim.Nx_padded), dtype=sim.dtype, device=dev)
+    grads = {k: torch.zeros(sim.Nx_padded, dtype=sim.dtype, device=dev) for k in ("mass", "stiff")}
+    prop.backward(P, V, 0, sim.N, u_offset=2, eta=prop.eta_memory(), g_mass=grads["mass"],
+                  g_stiff=grads["stiff"], sensors=adj, adjoint=adjoint)
+    grads = _finalize(sim, grads, dev)
+    if sim.pml is not None:
+        grads = _zero_outside(grads, pml_nodes(sim), dev)
+    return cost, grads, um, {}
+
+
+# ------------------------------------------------------ reconstruction (strip)
+class _Reconstruction:
+    """The forward/backward machinery of the strip-replayed reverse march over a batch.
+
+    One object drives `reconstruction_sensitivity` (one shot) and the batched
+    misfit gradient: a forward sweep that records the traces and the strip, then
+    adjoint levels that rebuild the forward field one level ahead of the adjoint
+    in the same kernel.
+    """
+
+    def __init__(self, sim, mat, sources, sensors, strip_dtype=None, checkpoints=None):
+        self.sim, self.mat = sim, mat
+        self.prop = prop = Propagator(sim, mat, len(sources))
+        self.dev = dev = prop.device
+        self.B = len(sources)
+        self.src, self.signal = source_points(prop, mat, sources)
+        self.rec = sensor_points(prop, sensors)
+        self.sensors = sensors
+        self.nrec = self.rec.points.columns
+        self.bf16 = strip_dtype == "bfloat16"
+        if strip_dtype not in (None, "bfloat16"):
+            raise ValueError(f"strip_dtype {strip_dtype!r}: None or 'bfloat16'")
+        if self.bf16 and sim.precision != "float32":
+            raise NotImplementedError("a bfloat16 strip is built for float32 runs")
+        strip, self.valid = reconstruction_nodes(sim)
+        self.count = strip.shape[1]
+        smap = np.full(sim.Nx_padded, -1, dtype=np.int32)
+        if self.count:
+            smap.ravel()[sim.flatten_indices(strip)] = np.arange(self.count, dtype=np.int32)
+        self.strip_map = torch.as_tensor(smap, device=dev)
+        self.U = torch.zeros((2, self.B, *sim.Nx_padded), dtype=sim.dtype, device=dev)
+        self.aux = prop.aux_memory()
+        itemsize = 2 if self.bf16 else np.dtype(sim.np_dtype).itemsize
+        snapshot = 2 * self.U[0].nbytes + sum(a.nbytes for a in self.aux)
+        segment = sim.N
+        if checkpoints is not None and self.count:
+            # the strip of a segment costs about one snapshot (cuwave/sensitivity.py:575-583)
+            segment = min(sim.N, max(2, snapshot // (2 * self.count * itemsize * self.B)))
+        self.segment = segment
+        self.checkpoints = checkpoints or 0
+        self.starts = list(range(0, sim.N, segment)) + [sim.N]
+        self.record = torch.zeros((self.B, sim.N, self.nrec), dtype=sim.dtype, device=dev)
+        self.recorded = 0
+        self.steps = 0
+        self.snapshots = {}
+
+    def strip_window(self, s: int, e: int) -> dict | None:
+        if not self.count:
+            return None
+        first = -1 if s == 0 else s  # a zero row stands for step -1
+        rows = e - first
+        dtype = torch.int16 if self.bf16 else self.sim.dtype
+        buf = torch.zeros((self.B, rows, self.count), dtype=dtype, device=self.dev)
+        return {"map": self.strip_map, "record": buf, "count": self.count, "first": first,
+                "rows": rows, "bf16": int(self.bf16)}
+
+    def advance(self, a: int, b: int, strip=None) -> None:
+        first = a == self.recorded
+        if first:
+            self.recorded = b
+        self.prop.forward(self.U, a, b, aux=self.aux, sources=self.src, signal=self.signal,
+                          sensors=self.rec if first else None,
+                          record=self.record if first else None, strip=strip)
+        self.steps += b - a
+
+    def store(self, k: int) -> None:
+        self.snapshots[k] = [f.clone() for f in (self.U, *self.aux)]
+
+    def restore(self, k: int) -> None:
+        fields = (self.U, *self.aux)
+        if k == 0:
+            for f in fields:
+                f.zero_()
+            return
+        for f, saved in zip(fields, self.snapshots[k]):
+            f.copy_(saved)
+
+    def run(self, objectives, gradient_mode):
+        """Forward, objectives, adjoint; returns per-shot costs and traces.
+
+        `objectives(record)` takes the (B, N, nrec) record and returns a list of
+        costs and the (B, N, nrec) derivative; `gradient_mode` is a dict of the
+        accumulators and how the kernel adds into them.
+        """
+        sim = self.sim
+        self.lam = torch.zeros((2, self.B, *sim.Nx_padded), dtype=sim.dtype, device=self.dev)
+        self.eta = self.prop.eta_memory()
+        self.costs = None
+        self.seed = None
+        self.gradient_mode = gradient_mode
+        self.objectives = objectives
+        for action, *segs in checkpoint_schedule(len(self.starts) - 1, self.checkpoints):
+            if action == "advance":
+                self.advance(self.starts[segs[0]], self.starts[segs[1]])
+            elif action == "store":
+                self.store(segs[0])
+            elif action == "restore":
+                self.restore(segs[0])
+            elif action == "drop":
+                del self.snapshots[segs[0]]
+            else:
+                self.leaf(segs[0])
+        return self.costs
+
+    def leaf(self, k: int) -> None:
+        sim, prop = self.sim, self.prop
+        s, e = self.starts[k], self.starts[k + 1]
+        strip = self.strip_window(s, e)
+        self.advance(s, e, strip)
+        if self.costs is None:
+            self.costs, dphi = self.objectives(self.record)
+            w = torch.as_tensor(sim.adjoint_weights(self.sensors), device=self.dev)
+            self.adjoint = (torch.flip(torch.as_tensor(dphi, dtype=sim.dtype), (1,)) / w).contiguous()
+            lin = sim.flatten_indices(self.sensors)
+            self.adj = sensor_points(prop, self.sensors, weight=sim.excitation_weights(self.mat, lin),
+                                     shot_stride=sim.N * self.nrec)
+        if s == 0:
+            top = self.U[(e - 1) % 2]
+            self.seed = float(torch.linalg.vector_norm(top * torch.as_tensor(self.valid, device=self.dev)))
+        common = dict(eta=self.eta, sensors=self.adj, adjoint=self.adjoint, sources=self.src,
+                      signal=self.signal, strip=strip, **self.gradient_mode)
+        # the top level reads u^(e - 1) as the forward left it; below it the
+        # reverse march rebuilds u^(n - 1) one level ahead
+        prop.backward(self.lam, self.U, e - 1, e, reverse=False, **common)
+        low = s if s == 0 else s + 1
+        if e - 1 > low:
+            prop.backward(self.lam, self.U, low, e - 1, reverse=True, **common)
+        if s > 0:
+            prop.backward(self.lam, self.U, s, s + 1, reverse=False, **common)
+
+    def drift(self) -> float:
+        """Norm of the rebuilt u^(-1), which is zero, over that of the top state."""
+        valid = torch.as_tensor(self.valid, device=self.dev)
+        rebuilt = float(torch.linalg.vector_norm(self.U[1] * valid))
+        return rebuilt / self.seed if self.seed else float("inf")
+
+
+def reconstruction_sensitivity(sim: Simulation, source: Source, indicator, sensors, objective,
+                               strip_dtype: str | None = None, checkpoints: int | None = None):
+    """Cost and gradients as `sensitivity`, storing a boundary strip and not the field.
+
+    The lossless step run with its outer slots swapped marches the forward field
+    backwards; the nodes the PML makes irreversible are recorded as a strip and
+    replayed (cuwave/sensitivity.py:500-750). Here the reverse step rides inside
+    the adjoint kernel, one level ahead of the adjoint. `info` carries the
+    `strip` node count, the `drift` of the rebuilt initial state, the steps per
+    `segment` and the forward `sweeps`.
+    """
+    _require(sim, source, sensors)
+    if checkpoints is not None and checkpoints < 0:
+        raise ValueError(f"checkpoints must be non-negative: {checkpoints}")
+    mat = sim.build_materials(indicator)
+    run = _Reconstruction(sim, mat, [source], sensors, strip_dtype, checkpoints)
+    dev = run.dev
+    grads = {k: torch.zeros(sim.Nx_padded, dtype=sim.dtype, device=dev) for k in ("mass", "stiff")}
+
+    def objectives(record):
+        cost, dphi = objective(record[0])
+        return [cost], torch.as_tensor(dphi, dtype=sim.dtype, device=dev)[None]
+
+    costs = run.run(objectives, dict(g_mass=grads["mass"], g_stiff=grads["stiff"], skip_layer=True))
+    grads = _finalize(sim, grads, dev)
+    grads = _zero_outside(grads, ~run.valid, dev)
+    info = {"strip": run.count, "drift": run.drift(), "segment": run.segment,
+            "sweeps": run.steps / sim.N}
+    return costs[0], grads, run.record[0], info
+
+
+# ---------------------------------------------------------------- superposition
+def _refill(sim: Simulation, u: torch.Tensor) -> None:
+    """The ghost refill of every face, as the reference's refill kernels (scalar.cu:535-555)."""
+    from .boundary import Dirichlet
+
+    for d, pair in enumerate(sim.boundary):
+        for side, cond in enumerate(pair):
+            ghost, mirror, wall = (0, 2, 1) if side == 0 else (sim.Nx[d] - 1, sim.Nx[d] - 3, sim.Nx[d] - 2)
+            inner = [slice(1, n - 1) for n in sim.Nx]
+            g_ix, m_ix, w_ix = list(inner), list(inner), list(inner)
+            g_ix[d], m_ix[d], w_ix[d] = ghost, mirror, wall
+            if cond == Dirichlet:
+                u[tuple(g_ix)] = -u[tuple(m_ix)]
+                u[tuple(w_ix)] = 0.0
+            else:
+                u[tuple(g_ix)] = u[tuple(m_ix)]
+
+
+def superposition_sensitivity(sim: Simulation, source: Source, indicator, sensors, objective,
+                              scale: float = 1.0):
+    """Cost and gradients in three field slots, for a lossless operator.
+
+    B(u, lambda) = [B(w, w) - B(u, u)] / 2k with w = u + k lambda rebuilt by
+    running the closed lossless domain backwards (cuwave/sensitivity.py:753-893).
+    `info` carries `scale` and the `cancellation` the subtraction cost.
+    """
+    if sim.pml is not None:
+        raise NotImplementedError(
+            "superposition_sensitivity runs the field backwards, which a pml does not allow; "
+            "use sensitivity or reconstruction_sensitivity"
+        )
+    _require(sim, source, sensors)
+    mat = sim.build_materials(indicator)
+    prop = Propagator(sim, mat, 1)
+    dev = prop.device
+    npd = sim.np_dtype
+    lin_src = torch.as_tensor(sim.flatten_indices(source.position).astype(np.int64), device=dev)
+    w_src = torch.as_tensor(sim.excitation_weights(mat, sim.flatten_indices(source.position)), device=dev)
+    sig = torch.as_tensor(np.asarray(as_numpy(source.signal), dtype=npd)[: sim.N], device=dev)
+    lin_rec = torch.as_tensor(sim.flatten_indices(sensors).astype(np.int64), device=dev)
+    accs = {k: torch.zeros(sim.Nx_padded, dtype=sim.dtype, device=dev) for k in ("mass", "stiff")}
+    factors = sim.step_factors()
+
+    def coefficients(sign):
+        ratios = [-sign / (2.0 * d) ** 2 / float(f) for d, f in zip(sim.dx, factors)]
+        if not np.allclose(ratios, ratios[0]):
+            raise ValueError(f"step factors {factors} are not proportional to 1 / dx**2")
+        return float(npd(sign / (2.0 * sim.dt) ** 2)), float(npd(ratios[0]))
+
+    def frechet(sign, u0, u1, u2):
+        # the mass density alone: the stiffness factors are zero (cuwave/scalar.py:836-856)
+        saved = [prop.grid.f[d] for d in range(sim.ndim)]
+        for d in range(sim.ndim):
+            prop.grid.f[d] = float(npd(-sign * 0 / (2.0 * sim.dx[d]) ** 2))
+        prop.superposed(u0, u1, u2, accs["mass"], accs["stiff"], float(npd(sign / (2.0 * sim.dt) ** 2)),
+                        0.0, step=False)
+        for d in range(sim.ndim):
+            prop.grid.f[d] = saved[d]
+
+    def excite(u, lin, w, row):
+        u.view(-1).index_add_(0, lin, w * row)
+
+    U = torch.zeros((3, *sim.Nx_padded), dtype=sim.dtype, device=dev)
+    um = torch.zeros((sim.N, len(lin_rec)), dtype=sim.dtype, device=dev)
+    ft, fs = coefficients(-1.0)
+    for t in range(sim.N):
+        u0, u1, u2 = U[t % 3], U[(t + 1) % 3], U[(t + 2) % 3]
+        prop.superposed(u0, u1, u2, accs["mass"], accs["stiff"], ft, fs, step=True)
+        excite(u2, lin_src, w_src, sig[t])
+        _refill(sim, u2)
+        um[t] = u2.view(-1)[lin_rec]
+    t = sim.N - 1
+    frechet(-1.0, U[t % 3], U[(t + 1) % 3], U[(t + 2) % 3])
+    cost, dphi = objective(um)
+    before = math.sqrt(sum(float(torch.linalg.vector_norm(f)) ** 2 for f in accs.values()))
+    back_lin = torch.cat((lin_rec, lin_src))
+    back_w = torch.cat((torch.as_tensor(sim.excitation_weights(mat, sim.flatten_indices(sensors)), device=dev), w_src))
+    back_sig = torch.cat((adjoint_signal(sim, dphi, sensors, scale, ADJOINT_DELAY),
+                          torch.flip(sig, (0,))), dim=1).contiguous()
+    ft, fs = coefficients(1.0)
+
+    def backward(t, row, mass=True):
+        u0, u1, u2 = (U[(sim.N + i - t) % 3] for i in (1, 0, 2))
+        prop.superposed(u0, u1, u2, accs["mass"], accs["stiff"], ft if mass else 0.0, fs, step=True)
+        excite(u2, back_lin, back_w, back_sig[row])
+        _refill(sim, u2)
+
+    backward(0, 0, mass=False)
+    for t in range(sim.N - 1):
+        backward(t + 1, t + 1)
+    last = [U[(1 + i) % 3] for i in (1, 0, 2)]
+    frechet(1.0, *last)
+    after = math.sqrt(sum(float(torch.linalg.vector_norm(f)) ** 2 for f in accs.values()))
+    cancellation = before / after if after > 0.0 else float("inf")
+    limit = CANCELLATION_LIMIT[sim.precision]
+    if cancellation > limit:
+        warnings.warn(
+            f"superposition scale={scale:g} leaves a cancellation of {cancellation:.1e} in "
+            f"{sim.precision}, past the usable {limit:.0e}: the gradient is largely round-off. "
+            f"Raise scale by about {cancellation / limit:.0e}.",
+            RuntimeWarning,
+            stacklevel=2,
+        )
+    accs = _finalize(sim, accs, dev)
+    norm = torch.tensor(npd(1.0 / (2.0 * scale)), device=dev)
+    for f in accs.values():
+        f.mul_(norm)
+    return cost, accs, um, {"scale": scale, "cancellation": cancellation}

@@ -1,0 +1,213 @@
+"""Application glue: points off the grid, receiver arrays, and the FWI misfit.
+
+`distribute`, `point_source`, `shots` and `Sensors` place sources and receivers
+at physical coordinates by multilinear interpolation (cuwave/utils.py:148-365);
+`misfit_gradient` sums the misfit and its gradient over a shot list
+(cuwave/utils.py:410-459). Here every shot of the list advances in one batched
+launch a step, and the kernel sums the shots' gradients as it goes, already
+chained onto the indicator by the parametrization's Jacobian
+(cuwave/utils.py:19-24).
+"""
+
+from __future__ import annotations
+
+import itertools
+from collections.abc import Callable, Sequence
+
+import numpy as np
+import torch
+
+from .sensitivity import (
+    _Reconstruction,
+    _finalize,
+    _zero_outside,
+    l2_misfit,
+    reconstruction_sensitivity,
+    sensitivity,
+)
+from .wave import Simulation, Source, as_numpy, simulate_shots
+
+TOL = 1e-6  # for `distribute`, so exact-boundary coordinates survive rounding
+
+
+def interior_slice(sim: Simulation) -> tuple[slice, ...]:
+    """Index tuple selecting the interior nodes, dropping the ghosts and the padding."""
+    return tuple(slice(1, n - 1) for n in sim.Nx)
+
+
+def _corner_weights(sim: Simulation, coords: np.ndarray):
+    """Cell corners and multilinear weights of `coords`, (ndim, num, 2**ndim) and (num, 2**ndim)."""
+    index = np.empty_like(coords)
+    for d in range(sim.ndim):
+        nodal = coords[:, d] / sim.dx[d] + 1.0
+        if nodal.min() < 1.0 - TOL or nodal.max() > sim.Nx[d] - 2.0 + TOL:
+            length = (sim.Nx[d] - 3) * sim.dx[d]
+            raise ValueError(
+                f"axis {d} coordinate outside the domain [0, {length:g}]: "
+                f"[{float(coords[:, d].min()):g}, {float(coords[:, d].max()):g}]"
+            )
+        index[:, d] = np.clip(nodal, 1.0, sim.Nx[d] - 2.0)
+    base = np.stack([np.clip(np.floor(index[:, d]), 1, sim.Nx[d] - 3).astype(np.int32)
+                     for d in range(sim.ndim)])
+    offset = index.T - base
+    corners = np.array(list(itertools.product((0, 1), repeat=sim.ndim)))
+    nodes = base[:, :, None] + corners.T[:, None, :]
+    weights = np.ones((len(coords), len(corners)))
+    for d in range(sim.ndim):
+        w = offset[d][:, None]
+        weights *= np.where(corners[None, :, d] == 1, w, 1.0 - w)
+    return nodes, weights
+
+
+def distribute(sim: Simulation, coords) -> tuple[np.ndarray, np.ndarray]:
+    """Multilinear interpolation of `coords` (num, ndim) onto the grid.
+
+    Returns the (ndim, num * 2**ndim) corner nodes and the (num, 2**ndim) weights
+    in the simulation's precision (cuwave/utils.py:185-247, nodal unknowns).
+    """
+    coords = np.atleast_2d(np.asarray(coords, dtype=float))
+    if coords.shape[1] != sim.ndim:
+        raise ValueError(f"coordinates need {sim.ndim} components, not {coords.shape[1]}")
+    nodes, weights = _corner_weights(sim, coords)
+    return nodes.reshape(sim.ndim, -1).astype(np.int32), weights.astype(sim.np_dtype)
+
+
+def point_source(sim: Simulation, coords, signal) -> Source:
+    """A `Source` injecting `signal` at `coords`, divided by the cell volume (cuwave/utils.py:250-277)."""
+    nodes, weights = distribute(sim, coords)
+    count = weights.shape[0]
+    signal = np.asarray(as_numpy(signal), dtype=sim.np_dtype)
+    if signal.ndim == 1:
+        signal = np.broadcast_to(signal[:, None], (signal.shape[0], count))
+    if signal.shape[1] != count:
+        raise ValueError(f"{signal.shape[1]} signal columns for {count} sources")
+    columns = (signal[:, :, None] * weights).reshape(signal.shape[0], -1)
+    return Source(nodes, np.ascontiguousarray(columns / np.prod(sim.dx), dtype=sim.np_dtype))
+
+
+def shots(sim: Simulation, coords, signal) -> list[Source]:
+    """One single-coordinate `Source` per coordinate: the shot list of an inversion."""
+    return [point_source(sim, c, signal) for c in np.atleast_2d(coords)]
+
+
+def stack(sources: Sequence[Source]) -> Source:
+    """Fire several sources in one simulation, their columns concatenated."""
+    return Source(
+        np.concatenate([np.asarray(as_numpy(s.position)) for s in sources], axis=1),
+        np.ascontiguousarray(np.concatenate([as_numpy(s.signal) for s in sources], axis=1)),
+    )
+
+
+class Sensors:
+    """Receivers at arbitrary coordinates, multilinearly interpolated onto the grid."""
+
+    def __init__(self, sim: Simulation, coords) -> None:
+        self.sim = sim
+        self.coordinates = np.atleast_2d(np.asarray(coords, dtype=float))
+        self.count = len(self.coordinates)
+        self.nodes, self.weights = distribute(sim, self.coordinates)
+
+    def _w(self, like: torch.Tensor) -> torch.Tensor:
+        return torch.as_tensor(self.weights, device=like.device, dtype=like.dtype)
+
+    def traces(self, record: torch.Tensor) -> torch.Tensor:
+        """(N, count * 2**ndim) node record -> (N, count) receiver traces."""
+        return (record.reshape(record.shape[0], self.count, -1) * self._w(record)).sum(2)
+
+    def scatter(self, dphi: torch.Tensor) -> torch.Tensor:
+        """The transpose of `traces`: (N, count) -> (N, count * 2**ndim)."""
+        return (dphi[:, :, None] * self._w(dphi)).reshape(dphi.shape[0], -1)
+
+    def objective(self, objective: Callable) -> Callable:
+        """Lift a receiver-space objective into the node space the adjoint wants."""
+
+        def wrapped(record):
+            cost, dphi = objective(self.traces(record))
+            return cost, self.scatter(dphi)
+
+        return wrapped
+
+
+def _batches(items: list, batch: int | None):
+    step = len(items) if not batch else int(batch)
+    for i in range(0, len(items), step):
+        yield i, items[i : i + step]
+
+
+def measure(sim: Simulation, sources: Sequence[Source], indicator, sensors: Sensors,
+            devices=None, batch: int | None = None) -> list[torch.Tensor]:
+    """Receiver traces per shot, all shots of a batch in one launch a step."""
+    if devices is not None:
+        raise NotImplementedError("devices= shot dealing is not part of this build (one GPU)")
+    out = []
+    for _, chunk in _batches(list(sources), batch):
+        _, record = simulate_shots(sim, chunk, indicator, sensors.nodes)
+        out += [sensors.traces(r) for r in record]
+    return out
+
+
+def misfit(sim: Simulation, sources, indicator, sensors: Sensors, observed,
+           objective: Callable = l2_misfit, devices=None, batch: int | None = None) -> float:
+    """The cost `misfit_gradient` returns, forward passes only."""
+    traces = measure(sim, sources, indicator, sensors, devices, batch)
+    return sum(objective(data)(t)[0] for t, data in zip(traces, observed))
+
+
+def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, sensors: Sensors,
+                    observed, objective: Callable = l2_misfit, adjoint: Callable = sensitivity,
+                    devices=None, batch: int | None = None, checkpoints: int | None = None):
+    """Misfit summed over the shots and its derivative with respect to `indicator`.
+
+    The shots advance together, `batch` of them a launch (all by default): one
+    forward sweep records every shot's traces and boundary strip, then one
+    adjoint sweep rebuilds every forward field backwards and sums the gradients
+    of all shots in the kernel, chained onto the indicator by
+    `parametrization_jacobian`. `adjoint` keeps the reference's meaning for the
+    memory a variant is allowed: both `sensitivity` and
+    `reconstruction_sensitivity` give the exact transpose, which this sweep is
+    for either (the stored history is never needed); any other callable runs
+    shot by shot as the reference does.
+
+    Returns (cost, gradient), the gradient a tensor over the padded grid.
+    """
+    if devices is not None:
+        raise NotImplementedError("devices= shot dealing is not part of this build (one GPU)")
+    sources = list(sources)
+    if adjoint not in (sensitivity, reconstruction_sensitivity):
+        cost, gradient = 0.0, None
+        d_mass, d_stiff = sim.parametrization_jacobian(indicator)
+        for source, data in zip(sources, observed):
+            c, grads, _, _ = adjoint(sim, source, indicator, sensors.nodes,
+                                     sensors.objective(objective(data)))
+            g = d_mass * grads["mass"] + d_stiff * grads["stiff"]
+            cost += c
+            gradient = g if gradient is None else gradient + g
+        return cost, gradient
+    for s in sources:
+        sim.require_interior(s.position, "source")
+    sim.require_interior(sensors.nodes, "sensor")
+    jm, js = sim.parametrization_jacobian(indicator)
+    if not (np.isscalar(jm) and np.isscalar(js)):
+        raise NotImplementedError("a field-valued Jacobian goes through autograd (cuwave_amd.autograd)")
+    mat = sim.build_materials(indicator)
+    cost = 0.0
+    gradient = None
+    for start, chunk in _batches(sources, batch):
+        run = _Reconstruction(sim, mat, chunk, sensors.nodes, checkpoints=checkpoints)
+        if gradient is None:
+            gradient = torch.zeros(sim.Nx_padded, dtype=sim.dtype, device=run.dev)
+            valid = run.valid
+
+        def objectives(record, start=start, chunk=chunk):
+            costs, dphis = [], []
+            for b in range(len(chunk)):
+                c, dphi = sensors.objective(objective(observed[start + b]))(record[b])
+                costs.append(c)
+                dphis.append(torch.as_tensor(dphi, dtype=sim.dtype, device=record.device))
+            return costs, torch.stack(dphis)
+
+        costs = run.run(objectives, dict(g_mass=gradient, combine=(jm, js), skip_layer=True))
+        cost += sum(costs)
+    grads = _finalize(sim, {"g": gradient}, gradient.device)
+    grads = _zero_outside(grads, ~valid, gradient.device)
+    return cost, grads["g"]

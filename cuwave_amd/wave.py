@@ -1,0 +1,253 @@
+"""The grid contract, the simulation description and the forward driver.
+
+`Simulation` carries the reference's constructor arguments and grid contract
+(cuwave/wave.py:479-631): `Nx` counts the ghost nodes, the interior is
+`1 .. Nx[d] - 2`, and every field is allocated over `Nx_padded`, the fastest axis
+rounded up to a multiple of 32 so each row starts on a 128-byte boundary. The
+physics lives in subclasses (`scalar.py`) that supply the parametrization hooks.
+
+Fields are torch tensors on the GPU. A batch of shots is a leading axis: one
+launch steps every shot of the batch, the material read once for all of them.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import TYPE_CHECKING
+
+import numpy as np
+import torch
+
+from . import _abi
+from .boundary import Dirichlet, Neumann, canonical_boundary, face_mask, validate_pml
+from .stencils import STENCILS, flux_row, stable_dt  # noqa: F401  (re-exported)
+
+if TYPE_CHECKING:
+    from .boundary import Pml
+
+
+def padded_shape(Nx: tuple[int, ...]) -> tuple[int, ...]:
+    """Pad the fastest (last) axis of `Nx` up to a multiple of 32."""
+    return (*Nx[:-1], ((Nx[-1] + 31) // 32) * 32)
+
+
+def grid_coords(Nx: tuple[int, ...], dx: tuple[float, ...], dtype=np.float64) -> list[np.ndarray]:
+    """Padded grid coordinates for shape `Nx` at spacing `dx`, node 1 at the origin."""
+    axes = [(np.arange(n, dtype=dtype) - 1) * d for n, d in zip(padded_shape(Nx), dx)]
+    return np.meshgrid(*axes, indexing="ij")
+
+
+@dataclass
+class Source:
+    position: np.ndarray  # (ndim, num_sources) grid indices
+    signal: np.ndarray | torch.Tensor  # (N, num_sources) time series
+
+
+def as_numpy(a) -> np.ndarray:
+    """A host copy of an array or tensor."""
+    if isinstance(a, torch.Tensor):
+        return a.detach().cpu().numpy()
+    return np.asarray(a)
+
+
+@dataclass
+class Simulation:
+    """Grid, timestep and stencil shared by the wave equations.
+
+    `threads` is accepted for the reference's signature (its CUDA block shape);
+    the gfx950 kernels choose their own tiles. `damping`, `domain`, `dirichlet`
+    masks and `devices` are not part of this build and raise if set.
+    """
+
+    Nx: tuple[int, ...]
+    dx: tuple[float, ...]
+    N: int
+    dt: float
+    threads: tuple[int, ...] = (4, 64)
+    precision: str = "float32"
+    space_order: int = 2
+    stencil: str = "optimized"
+    boundary: tuple = None
+    damping: object | None = None
+    domain: object | None = None
+    dirichlet: object | None = None
+    pml: Pml | None = None
+    devices: tuple[int, ...] | None = None
+
+    default_boundary = Neumann
+
+    def __post_init__(self) -> None:
+        self.Nx = tuple(int(n) for n in self.Nx)
+        self.dx = tuple(float(d) for d in self.dx)
+        self.ndim = len(self.Nx)
+        if not 1 <= self.ndim <= 3:
+            raise ValueError(f"1 to 3 axes, not {self.ndim}")
+        if len(self.dx) != self.ndim:
+            raise ValueError("dx needs one spacing per axis")
+        self.Nx_padded = padded_shape(self.Nx)
+        strides = [1] * self.ndim
+        for d in range(self.ndim - 2, -1, -1):
+            strides[d] = strides[d + 1] * self.Nx_padded[d + 1]
+        self.strides = tuple(strides)
+        if self.precision not in ("float32", "float64"):
+            raise ValueError(f"precision is float32 or float64, not {self.precision!r}")
+        self.np_dtype = np.float32 if self.precision == "float32" else np.float64
+        self.dtype = torch.float32 if self.precision == "float32" else torch.float64
+        self.boundary = canonical_boundary(self.boundary, self.ndim, self.default_boundary)
+        if self.space_order % 2 != 0 or not 2 <= self.space_order <= 8:
+            raise ValueError("space_order must be 2, 4, 6 or 8")
+        if self.stencil not in STENCILS:
+            raise ValueError(f"unknown stencil {self.stencil!r}, expected one of {STENCILS}")
+        if min(self.Nx) < self.space_order // 2 + 3:
+            raise ValueError(
+                f"Nx {self.Nx} cannot mirror the taps of space_order {self.space_order}: "
+                f"at least {self.space_order // 2 + 3} nodes an axis"
+            )
+        if self.damping is not None:
+            raise NotImplementedError("a damping field (sponge) is not part of this build: use pml")
+        if self.domain is not None or self.dirichlet is not None:
+            raise NotImplementedError("domain and dirichlet masks are not part of this build")
+        if self.devices is not None:
+            raise NotImplementedError("the devices= axis split is not part of this build (one GPU)")
+        self.comp_stride = int(np.prod(self.Nx_padded))
+        if self.comp_stride >= 2**31:
+            raise ValueError(f"{self.comp_stride} nodes overflow the int32 flat index")
+        self.field_shape = self.Nx_padded
+        if self.pml is not None:
+            validate_pml(self)
+
+    # ----------------------------------------------------------- geometry
+    @property
+    def reach(self) -> int:
+        """Nodes one step reads past a point, which a reconstruction strip must cover."""
+        return self.space_order // 2
+
+    @property
+    def interior(self) -> tuple[slice, ...]:
+        return tuple(slice(0, n) for n in self.Nx)
+
+    def flatten_indices(self, position) -> np.ndarray:
+        """(ndim, num) grid indices as int32 flat indices of one field (cuwave/wave.py:364-382)."""
+        position = np.asarray(as_numpy(position), dtype=np.int64)
+        if position.ndim != 2 or position.shape[0] != self.ndim:
+            raise ValueError(f"position needs {self.ndim} rows, not {position.shape}")
+        lin = np.zeros(position.shape[1], dtype=np.int64)
+        for d in range(self.ndim):
+            if np.any(position[d] < 0) or np.any(position[d] >= self.Nx[d]):
+                raise ValueError(f"a position on axis {d} lies outside 0 .. {self.Nx[d] - 1}")
+            lin += position[d] * self.strides[d]
+        return lin.astype(np.int32)
+
+    def require_interior(self, position, what: str) -> None:
+        """Raise if a node of `position` is a ghost (cuwave/sensitivity.py:103-118)."""
+        rows = np.asarray(as_numpy(position))
+        if rows.size == 0:
+            return
+        lo = int(rows.min())
+        if lo < 1:
+            raise ValueError(f"{what} on a ghost node: index {lo} < 1")
+        for d in range(self.ndim):
+            hi = int(rows[d].max())
+            if hi > self.Nx[d] - 2:
+                raise ValueError(
+                    f"{what} on a ghost node: axis {d} index {hi} exceeds Nx[{d}] - 2 = {self.Nx[d] - 2}"
+                )
+
+    def cell_weights(self) -> np.ndarray:
+        """The cell volume W a node owns, 1/2 per wall row it sits on (cuwave/wave.py:336-344)."""
+        w = np.ones(self.Nx_padded, dtype=self.np_dtype)
+        for d in range(self.ndim):
+            for index in (1, self.Nx[d] - 2):
+                face = [slice(None)] * self.ndim
+                face[d] = index
+                w[tuple(face)] *= 0.5
+        return w
+
+    def sensor_cell_weights(self, sensors) -> np.ndarray:
+        """Cell weights at the sensor nodes only (cuwave/wave.py:347-354)."""
+        rows = np.asarray(as_numpy(sensors))
+        w = np.ones(rows.shape[1], dtype=self.np_dtype)
+        for d in range(self.ndim):
+            for index in (1, self.Nx[d] - 2):
+                w = np.where(rows[d] == index, w * 0.5, w)
+        return w
+
+    def abi_grid(self, nbatch: int) -> _abi.Grid:
+        """The grid struct of the C ABI for a batch of `nbatch` shots."""
+        g = _abi.Grid()
+        g.ndim, g.order, g.nbatch = self.ndim, self.space_order, int(nbatch)
+        g.dtype = 0 if self.precision == "float32" else 1
+        for d in range(self.ndim):
+            g.n[d], g.np[d] = self.Nx[d], self.Nx_padded[d]
+        g.dirichlet = face_mask(self, Dirichlet)
+        g.neumann = face_mask(self, Neumann)
+        for k, v in enumerate(flux_row(self.space_order, self.stencil)):
+            g.op[k] = float(v)
+        for d, f in enumerate(self.step_factors()):
+            g.f[d] = float(f)
+        return g
+
+    # --------------------------------------------------- hooks (subclass)
+    def step_factors(self) -> list:
+        raise NotImplementedError
+
+    def source_factor(self) -> float:
+        raise NotImplementedError
+
+    def adjoint_mass_factor(self) -> float:
+        """Scale of the inertia gradient density: `1 / dt**2` (cuwave/wave.py:597-599)."""
+        return 1.0 / self.dt**2
+
+
+# ------------------------------------------------------------------- forward
+def simulate_shots(sim: Simulation, sources, indicator, sensors=None, record_every=None):
+    """Run every shot of `sources` forward in one batch under material `indicator`.
+
+    Returns the (B, *Nx) final interior fields, the (B, N, num_sensors) records when
+    `sensors` is given, and the (B, S, *Nx) host snapshots every `record_every`
+    steps when that is given.
+    """
+    from .engine import Propagator, sensor_points, source_points
+
+    sources = list(sources)
+    for s in sources:
+        sim.require_interior(s.position, "source")
+    mat = sim.build_materials(indicator)
+    prop = Propagator(sim, mat, len(sources))
+    src, signal = source_points(prop, mat, sources)
+    U = torch.zeros((2, len(sources), *sim.Nx_padded), dtype=sim.dtype, device=prop.device)
+    aux = prop.aux_memory()
+    rec = record = None
+    if sensors is not None:
+        sim.require_interior(sensors, "sensor")
+        rec = sensor_points(prop, sensors)
+        record = torch.zeros((len(sources), sim.N, rec.points.columns), dtype=sim.dtype,
+                             device=prop.device)
+    interior = (slice(None), *sim.interior)
+    snapshots = []
+    if record_every is None:
+        prop.forward(U, 0, sim.N, aux=aux, sources=src, signal=signal, sensors=rec, record=record)
+    else:
+        for t in range(sim.N):
+            prop.forward(U, t, t + 1, aux=aux, sources=src, signal=signal, sensors=rec,
+                         record=record)
+            if t % record_every == 0:
+                snapshots.append(U[t % 2][interior].cpu().numpy())
+    out = (U[(sim.N - 1) % 2][interior].clone(),)
+    if sensors is not None:
+        out += (record,)
+    if record_every is not None:
+        out += (np.stack(snapshots, axis=1),)
+    return out if len(out) > 1 else out[0]
+
+
+def simulate(sim: Simulation, source: Source, indicator, sensors=None, record_every=None):
+    """Run `sim` forward under `source` and material `indicator` (cuwave/wave.py:1071-1163).
+
+    Returns the final interior field, followed by the (N, num_sensors) record when
+    `sensors` is given and the stacked host snapshots when `record_every` is.
+    """
+    out = simulate_shots(sim, [source], indicator, sensors, record_every)
+    if not isinstance(out, tuple):
+        return out[0]
+    return tuple(o[0] for o in out)

@@ -1,0 +1,191 @@
+/* cuwave_amd: the C ABI of the MI355X (gfx950) finite-difference path.
+ *
+ * This library replaces the reference's kernel ABI for the pressure family:
+ * `compile_kernels(sim)` returning a module whose `get_function(name)` is
+ * launched as `kernel(grid, block, args)` with raw device pointers and
+ * by-value scalars (cuwave/wave.py:637-647, cuwave/cpu.py:85-153; the CPU
+ * fallback's C form is `launch_<name>(gx, gy, gz, bx, by, bz, void **args)`,
+ * cuwave/kernels/cpu.hpp:93-98). One entry point here stands for a whole
+ * step sequence of several of those kernels, fused, over a batch of shots.
+ *
+ * Conventions:
+ * - Every pointer is device memory owned by the caller; nothing allocates.
+ * - Fields are C-contiguous over the padded grid, the fastest axis padded to
+ *   a multiple of 32 (cuwave/wave.py:120-122); a batch of shots is a leading
+ *   axis; a time-level slot holds the whole batch.
+ * - `dtype` 0 is float32, 1 is float64; every field of a call has it.
+ * - Launches are asynchronous on `stream` (a hipStream_t; 0 the null stream).
+ * - A return of 0 is success; anything else is an error whose text
+ *   `cw_last_error()` returns (thread-local). Arguments are checked before
+ *   any launch, so an error leaves the device untouched.
+ * - Calls on one stream are ordered; calls on different streams that share
+ *   an output race.
+ */
+#ifndef CUWAVE_AMD_H
+#define CUWAVE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CW_ABI_VERSION 1
+
+/* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
+ * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
+ * cell-flux row (cuwave/stencils.py:186-213) */
+typedef struct cw_grid {
+  int32_t ndim;      /* 1, 2 or 3 */
+  int32_t order;     /* space order: 2, 4, 6 or 8 */
+  int32_t n[3];      /* logical extents, ghosts included (Simulation.Nx) */
+  int32_t np[3];     /* padded extents (Simulation.Nx_padded) */
+  int32_t nbatch;    /* shots one call advances */
+  int32_t dtype;     /* 0 float32, 1 float64 */
+  int32_t dirichlet; /* faces (bit 2 axis + side) held odd, Dirichlet */
+  int32_t neumann;   /* faces refilled even, Neumann */
+  double op[4];      /* op[k - 1]: the cell-flux weight k of the full radius */
+  double f[3];       /* per-axis step factors (Simulation.step_factors) */
+} cw_grid;
+
+/* the material the step reads (cuwave/scalar.py:407-420, 739-745) */
+typedef struct cw_materials {
+  const void *stiff; /* nodal stiffness, its ghosts mirrored */
+  const void *minv;  /* nodal inverse inertia, or NULL to derive 1 / stiff */
+  void *cell;        /* (ndim, field) harmonic cell stiffness, from prepare */
+  void *inv;         /* (field) the inverse inertia the kernels read */
+} cw_materials;
+
+/* the convolutional PML behind the faces (cuwave/boundary.py:272-376) */
+typedef struct cw_pml {
+  int32_t lo[3], hi[3];     /* layer rows behind the low / high face, 0 none */
+  const void *profile[3];   /* (4, n[d]): a, b at nodes and at half points */
+  void *aux[3];             /* forward memory: (nbatch, 3, slab nodes) */
+  void *eta[3][2];          /* adjoint memory, two buffers alternating */
+} cw_pml;
+
+/* points a step injects at or records from, bucketed by the library's tiles
+ * (cw_point_lists); `col` indexes the signal / record column */
+typedef struct cw_points {
+  const int32_t *starts; /* (tiles + 1) CSR offsets, NULL for no points */
+  const int32_t *lin;    /* flat node index within one shot's field */
+  const int32_t *shot;   /* the shot of the entry, -1 for every shot */
+  const int32_t *col;    /* column in the signal / record row */
+  const void *weight;    /* injection weight, NULL for a record */
+  int32_t columns;       /* row length of the signal / record */
+  int64_t shot_stride;   /* elements between shots of the signal / record,
+                            0 when one table serves every shot */
+} cw_points;
+
+/* the boundary strip a reverse march replays (cuwave/sensitivity.py:160-195) */
+typedef struct cw_strip {
+  const int32_t *map; /* (field) strip column of a node, -1 off the strip */
+  void *record;       /* (nbatch, rows, count): row t - first holds step t */
+  int32_t count;      /* strip nodes */
+  int32_t first;      /* the step of record row 0 */
+  int32_t rows;       /* record rows a shot holds */
+  int32_t bf16;       /* 1: record stored as bfloat16 bits (float32 only) */
+} cw_strip;
+
+/* time levels in slots: level k sits in slot (k + offset) mod slots, each
+ * slot holding the batch; slot_stride and batch_stride in elements */
+typedef struct cw_levels {
+  void *base;
+  int64_t slot_stride;
+  int64_t batch_stride;
+  int32_t slots;
+  int32_t offset;
+} cw_levels;
+
+/* forward steps t0 .. t1 - 1 of a batch (cuwave/wave.py:1071-1163): per
+ * step the flux-divergence update (cuwave/kernels/scalar.cu:346-466), the
+ * CPML memories of every lined axis (:499-532), the source injection
+ * (cuwave/kernels/common.cuh:456-464), the ghost refill (scalar.cu:535-555),
+ * the sensor record (common.cuh:483-490) and the strip record
+ * (cuwave/sensitivity.py:608-633), fused into one kernel. */
+typedef struct cw_forward_args {
+  cw_grid grid;
+  cw_materials mat;
+  cw_pml pml;            /* lo = hi = 0 on every axis for no layer */
+  cw_levels u;           /* the field levels */
+  cw_points sources;     /* signal (N, columns) per shot_stride */
+  const void *signal;
+  cw_points sensors;     /* record (nbatch, N, columns) */
+  void *record;
+  cw_strip strip;        /* strip.map NULL for none */
+  int32_t steps;         /* N, the rows of the signal and record tables */
+} cw_forward_args;
+
+/* adjoint steps over levels n = n1 - 1 down to n0 (cuwave/sensitivity.py:
+ * 384-497, 500-750): per level the exact transpose W^-1 A^T W of the step
+ * with both gradient densities, the inertia one by parts in time
+ * (cuwave/kernels/scalar_sensitivity.cu:443-515), the transposed CPML
+ * (:517-597), the adjoint load with its share of the inertia gradient
+ * (cuwave/kernels/common.cuh:467-480), and, when `reverse` is set, the
+ * reverse march of the forward field one level ahead with the source
+ * re-injected, the strip replayed and the ghosts refilled
+ * (cuwave/sensitivity.py:678-689), fused into one kernel. The reverse runs
+ * at levels n <= steps - 2, writing u^(n - 1) over u^(n + 1) and replaying
+ * the strip row of step n - 1 (zero rows stand for steps before 0). */
+typedef struct cw_backward_args {
+  cw_grid grid;
+  cw_materials mat;
+  cw_pml pml;
+  cw_levels lam;           /* adjoint levels, two slots */
+  cw_levels u;             /* forward levels: two slots reversed, or history */
+  int32_t reverse;         /* 1: rebuild u^(n - 1) from u^n, u^(n + 1) */
+  cw_points sensors;       /* adjoint load: signal (nbatch, N, columns) */
+  const void *adjoint_signal; /* row N - 1 - n drives level n */
+  cw_points sources;       /* re-injected by the reverse march */
+  const void *signal;
+  cw_strip strip;
+  void *g_mass, *g_stiff;  /* accumulators (cuwave/scalar.py:751-756) */
+  int64_t grad_batch_stride; /* 0: the shots sum into one field */
+  int32_t combine;         /* 1: g_mass += jm gm + js gs (g_stiff unused) */
+  double jac_mass, jac_stiff;
+  int32_t skip_layer;      /* 1: no gradient inside the layer (zeroed after) */
+  double mass_factor;      /* 1 / dt^2 (cuwave/wave.py:597-599) */
+  int32_t steps;           /* N */
+} cw_backward_args;
+
+/* the superposition step (cuwave/kernels/scalar_sensitivity.cu:599-668) and
+ * the unfused Frechet densities (:352-385), one call a step */
+typedef struct cw_superposed_args {
+  cw_grid grid;
+  cw_materials mat;
+  const void *u0, *u1;
+  void *u2;
+  void *acc_mass, *acc_stiff;
+  double ft, fs;
+  int32_t step; /* 1: superposed step; 0: Frechet densities of (u0, u1, u2) */
+} cw_superposed_args;
+
+int cw_version(void);
+const char *cw_last_error(void);
+
+/* the tiles a step kernel walks, for bucketing points */
+int cw_tile_count(const cw_grid *grid, int32_t *tiles);
+
+/* bucket `count` points by tile into CSR: `starts` (tiles + 1), `order`
+ * (count) the point index in bucketed order, stable; host memory */
+int cw_point_lists(const cw_grid *grid, const int32_t *lin, int32_t count,
+                   int32_t *starts, int32_t *order);
+
+/* the inverse inertia and the harmonic cell stiffness of every axis
+ * (cuwave/kernels/scalar.cu:48-64 `sc sp / (sc + sp)`, :454) */
+int cw_scalar_prepare(const cw_grid *grid, const cw_materials *mat,
+                      void *stream);
+
+int cw_scalar_forward(const cw_forward_args *args, int32_t t0, int32_t t1,
+                      void *stream);
+
+int cw_scalar_backward(const cw_backward_args *args, int32_t n0, int32_t n1,
+                       void *stream);
+
+int cw_scalar_superposed(const cw_superposed_args *args, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
