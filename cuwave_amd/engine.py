@@ -31,7 +31,13 @@ class PointList:
 
 
 class Propagator:
-    """The kernels of one simulation bound to its material, for `nbatch` shots at once."""
+    """The kernels of one simulation bound to its material, for `nbatch` shots at once.
+
+    Set `timings` to a list to have every forward and adjoint call bracketed by
+    HIP events on the launch stream: entries (kind, launches, start, end).
+    """
+
+    timings: list | None = None
 
     def __init__(self, sim, mat: dict, nbatch: int):
         self.device = _abi.require_device()
@@ -123,6 +129,25 @@ class Propagator:
         return PointList(pts, tuple(keep))
 
     # ------------------------------------------------------------ launches
+    def _timed(self, kind: str, launches: int):
+        import contextlib
+
+        timings = Propagator.timings
+        if timings is None or launches <= 0:
+            return contextlib.nullcontext()
+
+        @contextlib.contextmanager
+        def bracket():
+            stream = torch.cuda.current_stream()
+            start = torch.cuda.Event(enable_timing=True)
+            end = torch.cuda.Event(enable_timing=True)
+            start.record(stream)
+            yield
+            end.record(stream)
+            timings.append((kind, launches, start, end))
+
+        return bracket()
+
     def forward(self, u: torch.Tensor, t0: int, t1: int, *, offset: int = 0, aux=None,
                 sources: PointList | None = None, signal: torch.Tensor | None = None,
                 sensors: PointList | None = None, record: torch.Tensor | None = None,
@@ -143,7 +168,8 @@ class Propagator:
             a.strip = _abi.Strip(_abi.ptr(strip["map"]), _abi.ptr(strip["record"]), strip["count"],
                                  strip["first"], strip["rows"], int(strip.get("bf16", 0)))
         a.steps = self.sim.N
-        _abi.call("cw_scalar_forward", ctypes.byref(a), int(t0), int(t1), _abi.stream())
+        with self._timed("forward", t1 - t0):
+            _abi.call("cw_scalar_forward", ctypes.byref(a), int(t0), int(t1), _abi.stream())
 
     def backward(self, lam: torch.Tensor, u: torch.Tensor, n0: int, n1: int, *, eta,
                  g_mass: torch.Tensor, g_stiff: torch.Tensor | None = None, u_offset: int = 0,
@@ -178,7 +204,8 @@ class Propagator:
         a.skip_layer = int(skip_layer)
         a.mass_factor = float(self.sim.np_dtype(self.sim.adjoint_mass_factor()))
         a.steps = self.sim.N
-        _abi.call("cw_scalar_backward", ctypes.byref(a), int(n0), int(n1), _abi.stream())
+        with self._timed("backward", n1 - n0):
+            _abi.call("cw_scalar_backward", ctypes.byref(a), int(n0), int(n1), _abi.stream())
 
     def superposed(self, u0, u1, u2, acc_mass, acc_stiff, ft: float, fs: float, step: bool) -> None:
         """One superposition step, or the Frechet densities of a triplet (cw_scalar_superposed)."""

@@ -1,0 +1,67 @@
+"""Shared builders for the tests: simulations from the golden vectors, both for the
+product (`cuwave_amd`) and for the CPU oracle (`oracle/driver.py`)."""
+
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+GOLDEN = ROOT / "tests" / "golden"
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+
+def golden(name: str):
+    return np.load(GOLDEN / f"{name}.npz")
+
+
+def golden_names(prefix: str) -> list[str]:
+    return sorted(p.stem for p in GOLDEN.glob(f"{prefix}*.npz"))
+
+
+def product_sim(z, **override):
+    """The `cuwave_amd` simulation a golden case stores."""
+    from cuwave_amd import boundary, scalar
+
+    bmap = {0: boundary.Neumann, 1: boundary.Dirichlet}
+    bnd = tuple((bmap[int(a)], bmap[int(b)]) for a, b in z["boundary"])
+    pml = None
+    if "pml" in z:
+        w, c, r, f = (float(v) for v in z["pml"])
+        pml = boundary.Pml(int(w), c, r, f, tuple(int(x) for x in z["pml_faces"]))
+    common = dict(precision=str(z["precision"]), space_order=int(z["space_order"]),
+                  stencil=str(z["stencil"]), boundary=bnd, pml=pml)
+    common.update(override)
+    args = (tuple(int(n) for n in z["Nx"]), tuple(float(d) for d in z["dx"]), int(z["N"]),
+            float(z["dt"]))
+    if str(z["kind"]) == "ScalarWave":
+        return scalar.ScalarWave(*args, wavespeed=float(z["wavespeed"]),
+                                 density=float(z["density"]), **common)
+    r1, r2, k1, k2 = (float(v) for v in z["materials"])
+    return scalar.AcousticWave(*args, rho1=r1, rho2=r2, kappa1=k1, kappa2=k2, **common)
+
+
+def oracle_sim(z):
+    from oracle import driver
+
+    return driver.sim_from_npz(z)
+
+
+def rel_max(a, b) -> float:
+    """max |a - b| / max |b|."""
+    a = a.detach().cpu().numpy() if hasattr(a, "detach") else np.asarray(a)
+    b = np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def rel_l2(a, b) -> float:
+    a = a.detach().cpu().numpy() if hasattr(a, "detach") else np.asarray(a)
+    b = np.asarray(b)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def host(a) -> np.ndarray:
+    return a.detach().cpu().numpy() if hasattr(a, "detach") else np.asarray(a)

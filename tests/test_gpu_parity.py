@@ -1,0 +1,171 @@
+"""Parity of the HIP path with the reference, on the GPU, through the C ABI.
+
+The golden vectors are the reference's own outputs (its CPU backend, IEEE
+arithmetic). The kernels round as that backend does, so the forward field and
+traces are expected bit for bit; the bar the tests hold every result to is the
+stated tolerance: 1e-5 relative L2 for float32 fields and traces, 1e-4
+relative for float32 gradients, 1e-10 for float64.
+"""
+
+from __future__ import annotations
+
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import golden, golden_names, host, product_sim, rel_l2, rel_max
+
+pytestmark = pytest.mark.gpu
+
+S = importlib.import_module("cuwave_amd.sensitivity")
+from cuwave_amd import utils, wave  # noqa: E402
+
+F32_FIELD, F32_GRAD, F64 = 1e-5, 1e-4, 1e-10
+
+
+def tol(sim, grad=False):
+    if sim.precision == "float64":
+        return F64
+    return F32_GRAD if grad else F32_FIELD
+
+
+@pytest.mark.parametrize("name", golden_names("fwd_") + ["config1_forward"])
+def test_forward_matches_the_reference(name):
+    z = golden(name)
+    sim = product_sim(z)
+    print(f"case {name}", flush=True)
+    field, traces = wave.simulate(sim, wave.Source(z["src_pos"], z["src_sig"]), z["indicator"],
+                                  z["sensors"])
+    assert rel_l2(field, z["field"]) < tol(sim)
+    assert rel_l2(traces, z["traces"]) < tol(sim)
+    # the kernels round as the reference's IEEE path does: the same bits
+    np.testing.assert_array_equal(host(field), z["field"])
+    np.testing.assert_array_equal(host(traces), z["traces"])
+
+
+def _objective(z, sim):
+    obs = z["observed"] if "observed" in z else np.zeros((sim.N, z["sensors"].shape[1]), sim.np_dtype)
+    return S.l2_misfit(obs)
+
+
+@pytest.mark.parametrize("name", golden_names("adj_"))
+def test_adjoint_variants_match_the_reference(name):
+    z = golden(name)
+    sim = product_sim(z)
+    print(f"case {name}", flush=True)
+    src = wave.Source(z["src_pos"], z["src_sig"])
+    args = (sim, src, z["indicator"], z["sensors"], _objective(z, sim))
+    cost, g, traces, _ = S.sensitivity(*args)
+    assert abs(cost - float(z["cost"])) <= tol(sim) * abs(float(z["cost"]))
+    assert rel_l2(traces, z["traces"]) < tol(sim)
+    assert rel_max(g["mass"], z["g_mass"]) < tol(sim, True)
+    assert rel_max(g["stiff"], z["g_stiff"]) < tol(sim, True)
+    _, g, _, info = S.reconstruction_sensitivity(*args)
+    assert rel_max(g["mass"], z["r_g_mass"]) < tol(sim, True)
+    assert rel_max(g["stiff"], z["r_g_stiff"]) < tol(sim, True)
+    if "r_strip" in z:
+        assert info["strip"] == int(z["r_strip"])
+    assert info["drift"] < (1e-4 if sim.precision == "float32" else 1e-12)
+    if "c_g_mass" in z:
+        _, g, _, info = S.reconstruction_sensitivity(*args, checkpoints=2)
+        assert rel_max(g["mass"], z["c_g_mass"]) < tol(sim, True)
+        assert rel_max(g["stiff"], z["c_g_stiff"]) < tol(sim, True)
+        assert info["segment"] == int(z["c_segment"])
+        assert info["sweeps"] == float(z["c_sweeps"])
+    if "b_g_mass" in z:
+        # a bfloat16 strip trades exactness for memory: held to the exact gradient
+        _, g, _, _ = S.reconstruction_sensitivity(*args, strip_dtype="bfloat16")
+        assert rel_l2(g["mass"], z["r_g_mass"]) < 1e-2
+    if "s_g_mass" in z:
+        _, g, _, info = S.superposition_sensitivity(*args, scale=float(z["s_scale"]))
+        assert rel_max(g["mass"], z["s_g_mass"]) < tol(sim, True)
+        assert rel_max(g["stiff"], z["s_g_stiff"]) < tol(sim, True)
+
+
+@pytest.mark.parametrize("tag", ["f64", "f32"])
+def test_batched_misfit_gradient_matches_the_reference(tag):
+    z = golden(f"misfit_acoustic_o4_{tag}")
+    sim = product_sim(z)
+    sensors = utils.Sensors(sim, z["rec_coords"])
+    sources = [wave.Source(p, s) for p, s in zip(z["src_pos"], z["src_sig"])]
+    for kw in ({}, {"batch": 2, "checkpoints": 1}):
+        cost, gradient = utils.misfit_gradient(sim, sources, z["indicator"], sensors,
+                                               list(z["observed"]), **kw)
+        assert abs(cost - float(z["cost"])) <= tol(sim) * float(z["cost"])
+        assert rel_max(gradient, z["r_gradient"]) < tol(sim, True)
+
+
+def test_a_batch_steps_every_shot_as_alone():
+    # shots in one launch: each bitwise its single-shot run
+    z = golden("fwd_pml_acoustic_o8_f32")
+    sim = product_sim(z)
+    rng = np.random.default_rng(3)
+    sources = []
+    for k in range(5):
+        pos = np.array([[rng.integers(2, sim.Nx[0] - 2)], [rng.integers(2, sim.Nx[1] - 2)]])
+        sources.append(wave.Source(pos, z["src_sig"][:, :1] * (1 + k)))
+    fields, records = wave.simulate_shots(sim, sources, z["indicator"], z["sensors"])
+    for k, s in enumerate(sources):
+        f, r = wave.simulate(sim, s, z["indicator"], z["sensors"])
+        assert torch.equal(fields[k], f) and torch.equal(records[k], r)
+
+
+def test_one_adjoint_level_is_the_transpose_of_one_forward_step():
+    # <W m A^T p, u> = <W m p, A u> in float64, through the kernels themselves
+    from cuwave_amd import boundary, scalar
+    from cuwave_amd.engine import Propagator
+
+    for order, bnd in ((2, boundary.Neumann), (4, boundary.Dirichlet), (8, ((boundary.Dirichlet, boundary.Neumann), (boundary.Neumann, boundary.Dirichlet)))):
+        sim = scalar.AcousticWave((37, 45), (0.1, 0.1), 1, 0.3 * wave.stable_dt((0.1, 0.1), 1.0, order),
+                                  precision="float64", space_order=order, boundary=bnd,
+                                  rho1=1.0, rho2=3.0, kappa1=1.0, kappa2=5.0)
+        rng = np.random.default_rng(0)
+        mat = sim.build_materials(0.2 + rng.random(sim.Nx_padded))
+        prop = Propagator(sim, mat, 1)
+        dev = prop.device
+        mask = np.zeros(sim.Nx_padded)
+        mask[tuple(slice(1, n - 1) for n in sim.Nx)] = 1.0
+        U = torch.zeros((2, 1, *sim.Nx_padded), dtype=torch.float64, device=dev)
+        L = torch.zeros((2, 1, *sim.Nx_padded), dtype=torch.float64, device=dev)
+        # level -1 (slot 1) is u with its ghosts refilled, level -2 (slot 0) zero;
+        # lambda^1 (slot 1) is p, lambda^2 zero
+        for T in (U, L):
+            T[1, 0] = torch.as_tensor(rng.standard_normal(sim.Nx_padded) * mask, device=dev)
+            S._refill(sim, T[1, 0])
+        u, p = host(U[1, 0]) * mask, host(L[1, 0]) * mask
+        prop.forward(U, 0, 1)
+        Au = host(U[0, 0]) * mask - 2 * u
+        g = torch.zeros(sim.Nx_padded, dtype=torch.float64, device=dev)
+        prop.backward(L, U, 0, 1, eta=prop.eta_memory(), g_mass=g, g_stiff=g.clone())
+        Ap = host(L[0, 0]) * mask - 2 * p
+        Wm = sim.cell_weights() * mask / mat["minv_host"]
+        scale = np.sqrt(np.sum(Wm * Ap**2) * np.sum(Wm * u**2))
+        assert abs(np.sum(Wm * Ap * u) - np.sum(Wm * p * Au)) / scale < 1e-12, order
+
+
+def test_the_gradient_matches_finite_differences_in_float64():
+    z = golden("adj_pml_o8_dirichlet_f64")
+    sim = product_sim(z, )
+    sim.N = 90
+    src = wave.Source(z["src_pos"], z["src_sig"][: sim.N])
+    obs = host(wave.simulate(sim, src, z["indicator"], z["sensors"])[1]) * 0.7
+    objective = S.l2_misfit(obs)
+    indicator = np.array(z["indicator"])
+    _, grads, _, _ = S.reconstruction_sensitivity(sim, src, indicator, z["sensors"], objective)
+    jm, js = sim.parametrization_jacobian()
+    gradient = host(jm * grads["mass"] + js * grads["stiff"])
+
+    def cost():
+        return objective(wave.simulate(sim, src, indicator, z["sensors"])[1])[0]
+
+    for node in [(15, 15), (20, 12), (12, 22), (25, 18)]:
+        base, h = indicator[node], 1e-6
+        indicator[node] = base + h
+        plus = cost()
+        indicator[node] = base - h
+        minus = cost()
+        indicator[node] = base
+        fd = (plus - minus) / (2 * h)
+        assert abs(gradient[node] - fd) <= 1e-6 * abs(fd), (node, gradient[node], fd)

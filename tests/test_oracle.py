@@ -1,0 +1,232 @@
+"""The CPU oracle, pinned before it is trusted.
+
+Three kinds of pin, each chosen so a plausible mistake in the restatement fails
+one of them:
+
+- the golden vectors the reference itself wrote (tests/golden/, made by
+  tests/golden/make_golden.py from the reference's own CPU backend): the oracle
+  must reproduce them bit for bit, forward, adjoint, reconstruction with and
+  without checkpoints, the bfloat16 strip, superposition and the shot-summed
+  misfit gradient;
+- the transpose identity <W m A^T p, u> = <W m p, A u> of one adjoint step against
+  one forward step in float64 (what the reference's own TransposeTest pins,
+  tests/sensitivity_test.py:694-767), which a dropped or misplaced tap breaks;
+- central finite differences of the forward cost against the adjoint gradient
+  in float64, outside the layer, which any wrong factor breaks.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from helpers import golden, golden_names, oracle_sim
+
+from oracle import driver as D
+
+
+def test_stencil_rows_match_the_reference():
+    z = golden("stencils")
+    for R in range(1, 5):
+        for kind in ("taylor", "optimized"):
+            np.testing.assert_array_equal(D.cell_coefficients(R, kind), z[f"cell_{kind}_{R}"])
+            np.testing.assert_array_equal(D.staggered_weights(R, kind), z[f"stag_{kind}_{R}"])
+    for order in (2, 4, 6, 8):
+        for kind in ("taylor", "optimized"):
+            assert D.stable_dt((1e-3, 2e-3), 1500.0, order, kind) == float(z[f"stable_dt_{kind}_{order}"])
+
+
+def test_stencil_rows_are_consistent():
+    # consistency of the flux: sum (2k - 1) c_k = 1 for every row (the symbol's slope)
+    for R in range(1, 5):
+        for kind in ("taylor", "optimized"):
+            c = D.cell_coefficients(R, kind)
+            assert abs(float(c @ (2 * np.arange(1, R + 1) - 1.0)) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("name", golden_names("fwd_") + ["config1_forward"])
+def test_forward_matches_the_reference_bit_for_bit(name):
+    z = golden(name)
+    sim = oracle_sim(z)
+    field, traces = D.simulate(sim, z["src_pos"], z["src_sig"], z["indicator"], z["sensors"])
+    np.testing.assert_array_equal(field, z["field"])
+    np.testing.assert_array_equal(traces, z["traces"])
+
+
+def _objective(z, sim):
+    if "observed" in z:
+        return D.l2_misfit(z["observed"])
+    return D.l2_misfit(np.zeros((sim.N, z["sensors"].shape[1]), sim.dtype))
+
+
+@pytest.mark.parametrize("name", golden_names("adj_"))
+def test_adjoints_match_the_reference_bit_for_bit(name):
+    z = golden(name)
+    sim = oracle_sim(z)
+    args = (sim, z["src_pos"], z["src_sig"], z["indicator"], z["sensors"], _objective(z, sim))
+    cost, g, traces, _ = D.sensitivity(*args)
+    assert cost == float(z["cost"])
+    np.testing.assert_array_equal(g["mass"], z["g_mass"])
+    np.testing.assert_array_equal(g["stiff"], z["g_stiff"])
+    np.testing.assert_array_equal(traces, z["traces"])
+    _, g, _, info = D.reconstruction_sensitivity(*args)
+    np.testing.assert_array_equal(g["mass"], z["r_g_mass"])
+    np.testing.assert_array_equal(g["stiff"], z["r_g_stiff"])
+    assert info["drift"] == float(z["r_drift"])
+    if "c_g_mass" in z:
+        _, g, _, info = D.reconstruction_sensitivity(*args, checkpoints=2)
+        np.testing.assert_array_equal(g["mass"], z["c_g_mass"])
+        np.testing.assert_array_equal(g["stiff"], z["c_g_stiff"])
+        assert info["segment"] == int(z["c_segment"])
+        assert info["sweeps"] == float(z["c_sweeps"])
+    if "b_g_mass" in z:
+        _, g, _, _ = D.reconstruction_sensitivity(*args, strip_dtype="bfloat16")
+        np.testing.assert_array_equal(g["mass"], z["b_g_mass"])
+    if "s_g_mass" in z:
+        _, g, _, info = D.superposition_sensitivity(*args, scale=float(z["s_scale"]))
+        np.testing.assert_array_equal(g["mass"], z["s_g_mass"])
+        np.testing.assert_array_equal(g["stiff"], z["s_g_stiff"])
+        assert info["cancellation"] == float(z["s_cancellation"])
+
+
+@pytest.mark.parametrize("tag", ["f64", "f32"])
+def test_misfit_gradient_matches_the_reference(tag):
+    z = golden(f"misfit_acoustic_o4_{tag}")
+    sim = oracle_sim(z)
+    sensors = D.Sensors(sim, z["rec_coords"])
+    np.testing.assert_array_equal(sensors.nodes, z["sensor_nodes"])
+    np.testing.assert_array_equal(sensors.weights, z["sensor_weights"])
+    sources = list(zip(z["src_pos"], z["src_sig"]))
+    for adjoint, key in ((D.sensitivity, ""), (D.reconstruction_sensitivity, "r_")):
+        cost, gradient = D.misfit_gradient(sim, sources, z["indicator"], sensors,
+                                           list(z["observed"]), adjoint=adjoint)
+        assert cost == float(z[key + "cost"])
+        np.testing.assert_array_equal(gradient, z[key + "gradient"])
+
+
+# ----------------------------------------------------------- mathematical pins
+def _box(ndim, order, faces, kind):
+    Nx = {1: (41,), 2: (37, 29), 3: (27, 23, 21)}[ndim]
+    dx = (0.1,) * ndim
+    boundary = {
+        "neumann": ((0, 0),) * ndim,
+        "dirichlet": ((1, 1),) * ndim,
+        "mixed": tuple(((1, 0) if d % 2 == 0 else (0, 1)) for d in range(ndim)),
+    }[faces]
+    dt = 0.3 * D.stable_dt(dx, 1.0, order)
+    if kind == "scalar":
+        return D.Sim("ScalarWave", Nx, dx, 1, dt, "float64", order, boundary=boundary,
+                     wavespeed=1.0, density=1.0)
+    return D.Sim("AcousticWave", Nx, dx, 1, dt, "float64", order, boundary=boundary,
+                 materials=(1.0, 3.0, 1.0, 5.0))
+
+
+def _transpose_error(sim):
+    rng = np.random.default_rng(0)
+    indicator = 0.2 + rng.random(sim.Nx_padded)
+    mat = sim.build_materials(indicator)
+    st = D.Steps(sim, mat)
+    mask = np.zeros(sim.Nx_padded)
+    mask[tuple(slice(1, n - 1) for n in sim.Nx)] = 1.0
+    u, p = (rng.standard_normal(sim.Nx_padded) * mask for _ in "up")
+    st.refill(u)
+    st.refill(p)
+    zero = np.zeros_like(u)
+    Au = st.fd(zero, u, np.zeros_like(u)) - 2 * u
+    Ap = np.zeros_like(p)
+    st.fn("adjoint_step")(ctypes.byref(st.g), D._ptr(zero), D._ptr(p), D._ptr(Ap),
+                          D._ptr(mat["stiff"]), D._ptr(st.minv), ctypes.c_int(sim.derive_inertia),
+                          None, ctypes.c_double(sim.dt), st.f, ctypes.c_int(st.dirichlet))
+    Ap = Ap - 2 * p
+    minv = mat["minv"] if mat["minv"] is not None else 1.0 / mat["stiff"]
+    Wm = sim.apply_cell_weights(mask.copy()) / minv
+    scale = float(np.sqrt(np.sum(Wm * Ap**2) * np.sum(Wm * u**2)))
+    return abs(float(np.sum(Wm * Ap * u) - np.sum(Wm * p * Au))) / scale
+
+
+@pytest.mark.parametrize("ndim,order", [(1, 2), (1, 4), (2, 2), (2, 4), (2, 6), (2, 8), (3, 2), (3, 4)])
+@pytest.mark.parametrize("faces", ["neumann", "dirichlet", "mixed"])
+@pytest.mark.parametrize("kind", ["scalar", "acoustic"])
+def test_one_adjoint_step_is_the_transpose_of_one_forward_step(ndim, order, faces, kind):
+    assert _transpose_error(_box(ndim, order, faces, kind)) < 1e-12
+
+
+def test_a_planted_tap_error_breaks_the_transpose():
+    # the identity is a pin: one wrong weight in the adjoint's table must fail it
+    sim = _box(2, 8, "mixed", "acoustic")
+    lib = D.library()
+    rng = np.random.default_rng(0)
+    indicator = 0.2 + rng.random(sim.Nx_padded)
+    mat = sim.build_materials(indicator)
+    st = D.Steps(sim, mat)
+    mask = np.zeros(sim.Nx_padded)
+    mask[tuple(slice(1, n - 1) for n in sim.Nx)] = 1.0
+    u, p = (rng.standard_normal(sim.Nx_padded) * mask for _ in "up")
+    zero = np.zeros_like(u)
+    Au = st.fd(zero, u, np.zeros_like(u)) - 2 * u
+    planted = sim.grid()
+    planted.op[4 * 3 + 2] *= 1.001  # radius 4, weight 3
+    Ap = np.zeros_like(p)
+    lib.cwo_adjoint_step_f64(ctypes.byref(planted), D._ptr(zero), D._ptr(p), D._ptr(Ap),
+                             D._ptr(mat["stiff"]), D._ptr(st.minv), ctypes.c_int(0), None,
+                             ctypes.c_double(sim.dt), st.f, ctypes.c_int(st.dirichlet))
+    Ap -= 2 * p
+    Wm = sim.apply_cell_weights(mask.copy()) / mat["minv"]
+    scale = float(np.sqrt(np.sum(Wm * Ap**2) * np.sum(Wm * u**2)))
+    assert abs(float(np.sum(Wm * Ap * u) - np.sum(Wm * p * Au))) / scale > 1e-6
+
+
+def _pml_case(order):
+    z = golden("adj_pml_o8_dirichlet_f64" if order == 8 else "adj_pml_o4_mixed_f64")
+    sim = oracle_sim(z)
+    sim.N = 90
+    return z, sim
+
+
+@pytest.mark.parametrize("order", [4, 8])
+def test_the_adjoint_gradient_matches_finite_differences(order):
+    z, sim = _pml_case(order)
+    obs = D.simulate(sim, z["src_pos"], z["src_sig"][: sim.N], z["indicator"], z["sensors"])[1] * 0.7
+    objective = D.l2_misfit(obs)
+    indicator = np.array(z["indicator"])
+    _, grads, _, _ = D.sensitivity(sim, z["src_pos"], z["src_sig"][: sim.N], indicator,
+                                   z["sensors"], objective)
+    jm, js = sim.parametrization_jacobian()
+    gradient = jm * grads["mass"] + js * grads["stiff"]
+    for node in [(15, 15), (20, 12), (12, 22), (25, 18)]:
+        h = 1e-6
+        base = indicator[node]
+
+        def cost():
+            return objective(D.simulate(sim, z["src_pos"], z["src_sig"][: sim.N], indicator,
+                                        z["sensors"])[1])[0]
+
+        indicator[node] = base + h
+        plus = cost()
+        indicator[node] = base - h
+        minus = cost()
+        indicator[node] = base
+        fd = (plus - minus) / (2 * h)
+        assert abs(gradient[node] - fd) <= 1e-6 * max(abs(fd), 1e-300), (node, gradient[node], fd)
+
+
+def test_the_profiles_vanish_at_the_front_and_mirror_across_the_wall():
+    # the layer's closed form (cuwave/boundary.py:327-361): damped on the wall node,
+    # undamped from the front node (width - 1 in) on, even about the wall node
+    z = golden("adj_pml_o4_mixed_f64")
+    sim = oracle_sim(z)
+    width = sim.pml.width
+    for d, ((lo, hi), prof) in enumerate(zip(sim.pml_widths(), sim.pml_profiles())):
+        if prof is None:
+            continue
+        a_node, b_node = prof[0], prof[1]
+        n = sim.Nx[d]
+        for lined, wall, front, ghost, mirror in ((lo, 1, width, 0, 2),
+                                                  (hi, n - 2, n - 1 - width, n - 1, n - 3)):
+            if not lined:
+                continue
+            assert a_node[wall] < 0.0 and 0.0 < b_node[wall] < 1.0
+            assert a_node[front] == 0.0 and b_node[front] == 1.0
+            assert a_node[ghost] == a_node[mirror] and b_node[ghost] == b_node[mirror]
