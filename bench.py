@@ -197,7 +197,7 @@ def main():
     p.add_argument("--shots", type=int, default=SHOTS)
     p.add_argument("--nt", type=int, default=STEPS)
     p.add_argument("--roi", type=int, default=ROI)
-    p.add_argument("--cpu-sample", type=int, default=200, help="oracle steps for the CPU baseline")
+    p.add_argument("--cpu-sample", type=int, default=1500, help="oracle steps for the CPU baseline")
     p.add_argument("--no-cpu", action="store_true")
     args = p.parse_args()
     if args.gpus != 1:

@@ -147,6 +147,7 @@ SIGNATURES = {
     "cw_last_error": ([], ctypes.c_char_p),
     "cw_tile_count": ([ctypes.POINTER(Grid), ctypes.POINTER(I32)], I32),
     "cw_point_lists": ([ctypes.POINTER(Grid), VP, I32, VP, VP], I32),
+    "cw_scalar_grad_groups": ([ctypes.POINTER(Grid), ctypes.POINTER(Pml), ctypes.POINTER(I32)], I32),
     "cw_scalar_prepare": ([ctypes.POINTER(Grid), ctypes.POINTER(Materials), VP], I32),
     "cw_scalar_forward": ([ctypes.POINTER(ForwardArgs), I32, I32, VP], I32),
     "cw_scalar_backward": ([ctypes.POINTER(BackwardArgs), I32, I32, VP], I32),

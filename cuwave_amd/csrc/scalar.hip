@@ -3,6 +3,7 @@
 // time loops that launch one fused kernel a step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -10,6 +11,7 @@
 
 #include "../../include/cuwave_amd.h"
 #include "scalar_kernels.hpp"
+#include "stream.hpp"
 
 namespace {
 
@@ -180,6 +182,60 @@ template <typename T, int NDIM, int R> struct Prepare {
   }
 };
 
+// ---------------------------------------------------------- stream box
+constexpr int kChunk = 32; // rows a streaming wave marches
+
+// the stream box of a 2D grid: every node order / 2 + 2 off the walls, out
+// of reach of the wall mirror, the refill, the held nodes and the adjoint's
+// reflected taps; empty on other grids and on grids too small for one
+cw::Box stream_box(const cw_grid &g) {
+  cw::Box box{};
+  const int margin = g.order / 2 + 2;
+  box.y0 = margin;
+  box.y1 = g.n[0] - margin;
+  box.x0 = margin;
+  box.x1 = g.ndim == 2 ? g.n[1] - margin : 0;
+  box.chunk = kChunk;
+  if (g.ndim != 2 || box.y0 >= box.y1 || box.x0 >= box.x1) {
+    box.y0 = box.y1 = box.x0 = box.x1 = 0;
+    return box;
+  }
+  box.tx0 = box.x0 / cw::kLanes;
+  box.tx1 = (box.x1 - 1) / cw::kLanes + 1;
+  return box;
+}
+
+int box_items(const cw::Box &b, int nbatch, int group) {
+  if (b.y0 >= b.y1)
+    return 0;
+  const int nchunk = (b.y1 - b.y0 + b.chunk - 1) / b.chunk;
+  return nchunk * (b.tx1 - b.tx0) * ((nbatch + group - 1) / group);
+}
+
+// where the general kernels run: the band around the box, or every tile
+cw::Region band_of(const cw_grid &g, const cw::Box &b) {
+  cw::Region r{};
+  if (b.y0 >= b.y1)
+    return r;
+  r.band = 1;
+  r.y0 = b.y0;
+  r.y1 = b.y1;
+  r.x0 = b.x0;
+  r.x1 = b.x1;
+  const int W = g.n[1] - 2;
+  r.count = (b.y0 - 1) * W + (g.n[0] - 1 - b.y1) * W +
+            (b.y1 - b.y0) * ((b.x0 - 1) + (g.n[1] - 1 - b.x1));
+  return r;
+}
+
+int general_blocks(const cw::Geom<float> &g, const cw::Region &r) {
+  return r.band ? (r.count + cw::kThreads - 1) / cw::kThreads : g.ntiles;
+}
+
+// shots a thread or wave of the adjoint takes, sharing one gradient
+// accumulator: the caller's accumulator holds ceil(nbatch / group) groups
+int adjoint_group(int nbatch) { return nbatch >= 2 ? 2 : 1; }
+
 template <typename T, int NDIM, int R> struct Forward {
   static int run(const cw_forward_args &a, int t0, int t1,
                  hipStream_t stream) {
@@ -200,10 +256,22 @@ template <typename T, int NDIM, int R> struct Forward {
     p.strip.rows = a.strip.rows;
     p.strip.bf16 = a.strip.bf16;
     p.steps = a.steps;
-    for (int t = t0; t < t1; ++t)
-      hipLaunchKernelGGL((cw::forward_kernel<T, NDIM, R>), dim3(p.g.ntiles),
-                         dim3(cw::kThreads), 0, stream, p, t);
-    return hip_status("forward_kernel");
+    const cw::Box box = stream_box(a.grid);
+    p.region = band_of(a.grid, box);
+    // the band is thin: a thread a shot; a whole grid: a thread all shots
+    p.group = p.region.band ? 1 : p.g.nbatch;
+    const dim3 grid(general_blocks(make_geom<float>(a.grid), p.region),
+                    (p.g.nbatch + p.group - 1) / p.group);
+    const int items = box_items(box, p.g.nbatch, 1);
+    for (int t = t0; t < t1; ++t) {
+      if (grid.x)
+        hipLaunchKernelGGL((cw::forward_kernel<T, NDIM, R>), grid,
+                           dim3(cw::kThreads), 0, stream, p, t);
+      if (NDIM == 2 && items)
+        hipLaunchKernelGGL((cw::forward_stream<T, R>), dim3(items),
+                           dim3(cw::kLanes), 0, stream, p, box, t);
+    }
+    return hip_status("forward");
   }
 };
 
@@ -239,10 +307,39 @@ template <typename T, int NDIM, int R> struct Backward {
     p.skip_layer = a.skip_layer;
     p.mf = (T)a.mass_factor;
     p.steps = a.steps;
-    for (int n = n1 - 1; n >= n0; --n)
-      hipLaunchKernelGGL((cw::backward_kernel<T, NDIM, R>), dim3(p.g.ntiles),
-                         dim3(cw::kThreads), 0, stream, p, n);
-    return hip_status("backward_kernel");
+    const cw::Box box = stream_box(a.grid);
+    p.region = band_of(a.grid, box);
+    p.group = adjoint_group(p.g.nbatch);
+    const dim3 grid(general_blocks(make_geom<float>(a.grid), p.region),
+                    (p.g.nbatch + p.group - 1) / p.group);
+    const int items = box_items(box, p.g.nbatch, p.group);
+    for (int n = n1 - 1; n >= n0; --n) {
+      if (grid.x)
+        hipLaunchKernelGGL((cw::backward_kernel<T, NDIM, R>), grid,
+                           dim3(cw::kThreads), 0, stream, p, n);
+      if (NDIM == 2 && items)
+        launch_stream(p, box, items, n, stream);
+    }
+    return hip_status("backward");
+  }
+
+  static void launch_stream(const cw::BwdParams<T> &p, const cw::Box &box,
+                            int items, int n, hipStream_t stream) {
+#define CW_BWD(SS, ST)                                                         \
+  hipLaunchKernelGGL((cw::backward_stream<T, R, SS, ST>), dim3(items),         \
+                     dim3(cw::kLanes), 0, stream, p, box, n)
+    if (p.stiff_grad) {
+      if (p.group == 2)
+        CW_BWD(2, true);
+      else
+        CW_BWD(1, true);
+    } else {
+      if (p.group == 2)
+        CW_BWD(2, false);
+      else
+        CW_BWD(1, false);
+    }
+#undef CW_BWD
   }
 };
 
@@ -320,6 +417,17 @@ int cw_point_lists(const cw_grid *grid, const int32_t *lin, int32_t count,
   std::vector<int32_t> fill(starts, starts + g.ntiles);
   for (int j = 0; j < count; ++j)
     order[fill[tile[j]]++] = j;
+  return 0;
+}
+
+int cw_scalar_grad_groups(const cw_grid *grid, const cw_pml *pml,
+                          int32_t *groups) {
+  if (int e = check_grid(grid))
+    return e;
+  if (!pml || !groups)
+    return fail("grad groups need the pml and an output");
+  const int S = adjoint_group(grid->nbatch);
+  *groups = (grid->nbatch + S - 1) / S;
   return 0;
 }
 

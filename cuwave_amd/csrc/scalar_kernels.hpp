@@ -1,5 +1,7 @@
-// The pressure-family kernels for gfx950: forward step, fused adjoint step,
-// superposition step, material preparation.
+// The pressure-family kernels for gfx950, general form: any dimension, walls,
+// PML, points. On a 2D grid they cover only the band of nodes within reach of
+// a wall (the streaming kernels of stream.hpp take everything else); in 3D and
+// on grids too small for a stream they cover every node.
 //
 // Every kernel rounds as the reference's kernels do on its IEEE CPU path: no
 // contraction (built with -ffp-contract=off), an fma exactly where the
@@ -8,6 +10,10 @@
 // (cuwave/kernels/scalar.cu:63) are computed once by `prepare_kernel`, in
 // the same operand order, so reading them is bitwise the in-place division;
 // the inverse inertia 1 / sc likewise.
+//
+// A launch's second grid axis is the shot group: a thread steps its node for
+// the `group` shots of its group, which share the material loads and, in the
+// adjoint, one gradient accumulator of the caller's (cw_scalar_grad_groups).
 #pragma once
 
 #include "common.hpp"
@@ -35,6 +41,11 @@ template <typename T> struct Levels {
   }
 };
 
+// the adjoint memory of a slab node: the recursion's three variables, then
+// the two tap fields its neighbours' gathers read (scalar_sensitivity.cu:
+// 232-251 `pml_tap`, computed once by the owner instead of by every reader)
+constexpr int kEta = 5;
+
 template <typename T> struct Layer {
   int lo[3], hi[3];
   const T *profile[3];
@@ -49,8 +60,18 @@ template <typename T> struct Strip {
   int bf16;
 };
 
+// where the general kernels run: every tile, or (2D) the band of nodes
+// outside the stream box rows [y0, y1) x columns [x0, x1)
+struct Region {
+  int band;
+  int y0, y1, x0, x1;
+  int count; // band nodes
+};
+
 template <typename T> struct FwdParams {
   Geom<T> g;
+  Region region;
+  int group; // shots a thread steps
   const T *inv, *cell;
   Layer<T> pml;
   Levels<T> u;
@@ -64,6 +85,8 @@ template <typename T> struct FwdParams {
 
 template <typename T> struct BwdParams {
   Geom<T> g;
+  Region region;
+  int group;
   const T *inv, *cell, *stiff;
   Layer<T> pml;
   Levels<T> lam, u;
@@ -136,6 +159,142 @@ template <typename T> __device__ __forceinline__ T harmonic(T a, T b) {
   return a * b / (a + b);
 }
 
+// the forward CPML of axis d at a slab node (cuwave/kernels/scalar.cu:
+// 499-532): the three memories updated in place, their share of the field
+// returned. `an .. bh1` are the profile at the node and at the half point
+// below it.
+template <typename T>
+__device__ __forceinline__ T pml_forward(T *aux, int64_t size, T Dp, T Dm,
+                                         T gp, T gm, T f, T mi, T an, T bn,
+                                         T ah, T bh, T ah1, T bh1) {
+  const T pp = bh * aux[0] + ah * Dp;
+  const T pm = bh1 * aux[size] + ah1 * Dm;
+  const T z = bn * aux[2 * size] + an * f * (gp * (Dp + pp) - gm * (Dm + pm));
+  aux[0] = pp;
+  aux[size] = pm;
+  aux[2 * size] = z;
+  return mi * (f * (gp * pp - gm * pm) + z);
+}
+
+// the adjoint memory of a slab node stepped by lambda (scalar_sensitivity.cu:
+// 517-542 with q = W lambda), then the two tap fields of the new memory
+// (:232-251): cur and nxt address the node's kEta rows, `size` apart
+template <typename T>
+__device__ __forceinline__ void pml_adjoint_update(const T *cur, T *nxt,
+                                                   int64_t size, T q, T gp,
+                                                   T gm, T f, T an, T bn,
+                                                   T ah, T bh, T ah1, T bh1) {
+  const T ez = cur[2 * size];
+  const T ep = bh * (cur[0] + an * f * gp * ez) + f * gp * q;
+  const T em = bh1 * (cur[size] - an * f * gm * ez) - f * gm * q;
+  const T zz = bn * ez + q;
+  nxt[0] = ep;
+  nxt[size] = em;
+  nxt[2 * size] = zz;
+  const T e2 = an * f * zz;
+  nxt[3 * size] = ah * ep + gp * (1 + ah) * e2;
+  nxt[4 * size] = ah1 * em - gm * (1 + ah1) * e2;
+}
+
+// d(Dp)/d(u at row q) and d(Dm)/d(u at row q) (scalar_sensitivity.cu:219-230)
+template <typename T, int R>
+__device__ __forceinline__ T plus_tap(int q, const T *op) {
+  if (q >= 1)
+    return q <= R ? op[q - 1] : (T)0;
+  return 1 - q <= R ? -op[-q] : (T)0;
+}
+template <typename T, int R>
+__device__ __forceinline__ T minus_tap(int q, const T *op) {
+  if (q >= 0)
+    return q + 1 <= R ? op[q] : (T)0;
+  return -q <= R ? -op[-q - 1] : (T)0;
+}
+
+// scalar_sensitivity.cu:137-154 transposed_divergence_taps over taps
+// l[0 .. 2R] and cell means: hi[j - 1] the cell (a + j - 1, a + j),
+// lo[j - 1] the cell (a - j, a - j + 1)
+template <typename T, int R>
+__device__ __forceinline__ T transposed(const T *l, const T *chi,
+                                        const T *clo, const T *op) {
+  T div = 0, lh = l[R], ll = l[R];
+#pragma unroll
+  for (int j = 1; j <= R; ++j) {
+    const T lh1 = l[R + j], ll1 = l[R - j];
+    div += op[j - 1] * (chi[j - 1] * (lh1 - lh) - clo[j - 1] * (ll - ll1));
+    lh = lh1;
+    ll = ll1;
+  }
+  return div;
+}
+
+// scalar_sensitivity.cu:116-135 stiffness_gradient_taps
+template <typename T, int R>
+__device__ __forceinline__ T stiffness_density(const T *u, const T *l, T lc,
+                                               T sc, T sp, T sm, T factor,
+                                               const T *op) {
+  const T dgp = sp * sp / ((sc + sp) * (sc + sp));
+  const T dgm = sm * sm / ((sc + sm) * (sc + sm));
+  T Dp, Dm;
+  cell_diffs<T, R>(u, op, Dp, Dm);
+  return factor * (dgp * Dp * (l[R + 1] - lc) + dgm * Dm * (lc - l[R - 1]));
+}
+
+// ------------------------------------------------------------ node mapping
+// the 2D band node t, or false past the band
+template <typename T>
+__device__ __forceinline__ bool band_node(const Region &r, const Geom<T> &g,
+                                          int t, int *a) {
+  const int W = g.n[1] - 2;
+  const int top = (r.y0 - 1) * W;
+  if (t < top) {
+    a[0] = 1 + t / W;
+    a[1] = 1 + t % W;
+    return true;
+  }
+  t -= top;
+  const int bot = (g.n[0] - 1 - r.y1) * W;
+  if (t < bot) {
+    a[0] = r.y1 + t / W;
+    a[1] = 1 + t % W;
+    return true;
+  }
+  t -= bot;
+  const int left = r.x0 - 1, right = g.n[1] - 1 - r.x1, per = left + right;
+  if (per <= 0 || t >= per * (r.y1 - r.y0))
+    return false;
+  const int k = t % per;
+  a[0] = r.y0 + t / per;
+  a[1] = k < left ? 1 + k : r.x1 + (k - left);
+  return true;
+}
+
+// this thread's node: its axes a, its tile, and whether every tap of its
+// tile lies on stored nodes at distance `reach` (a tile-wide flag)
+template <int NDIM, typename T>
+__device__ __forceinline__ bool general_node(const Region &r, const Geom<T> &g,
+                                             int reach, int *a, int &tile,
+                                             bool &deep) {
+  if (NDIM == 2 && r.band) {
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= r.count || !band_node(r, g, t, a))
+      return false;
+    tile = (a[0] / TileShape<2>::TY) * g.tf + a[1] / TileShape<2>::TX;
+    deep = false;
+    return true;
+  }
+  tile = xcd_tile(blockIdx.x, g.ntiles);
+  int lo[3], hi[3];
+  tile_node<NDIM>(g, tile, a, lo, hi);
+  bool inside = true;
+  deep = true;
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    inside &= a[d] > 0 && a[d] < g.n[d] - 1;
+    deep &= lo[d] >= reach && hi[d] <= g.n[d] - 1 - reach;
+  }
+  return inside;
+}
+
 // ----------------------------------------------------------- preparation
 // inv = minv or 1 / stiff; cell[d] = the harmonic mean of the cell (a, a + 1)
 // along axis d, for every node that has a next node along d
@@ -166,16 +325,9 @@ template <typename T, int NDIM, int R>
 __global__ __launch_bounds__(kThreads) void forward_kernel(const FwdParams<T> p,
                                                            const int t) {
   const Geom<T> &g = p.g;
-  const int tile = xcd_tile(blockIdx.x, g.ntiles);
-  int a[3], lo[3], hi[3];
-  tile_node<NDIM>(g, tile, a, lo, hi);
-  bool inside = true, deep = true;
-#pragma unroll
-  for (int d = 0; d < NDIM; ++d) {
-    inside &= a[d] > 0 && a[d] < g.n[d] - 1;
-    deep &= lo[d] >= R && hi[d] <= g.n[d] - 1 - R;
-  }
-  if (!inside)
+  int a[3], tile;
+  bool deep;
+  if (!general_node<NDIM>(p.region, g, R, a, tile, deep))
     return;
   int64_t idx = 0;
   bool held = false;
@@ -215,8 +367,10 @@ __global__ __launch_bounds__(kThreads) void forward_kernel(const FwdParams<T> p,
   T *const u2s = p.u.at(t);
   const T *const u1s = p.u.at(t - 1);
   const T *const u0s = p.u.at(t - 2);
+  const int b0 = blockIdx.y * p.group;
+  const int b1 = min(b0 + p.group, g.nbatch);
 
-  for (int b = 0; b < g.nbatch; ++b) {
+  for (int b = b0; b < b1; ++b) {
     const int64_t ob = (int64_t)b * p.u.batch;
     const T *__restrict__ u1 = u1s + ob;
     T taps[NDIM][2 * R + 1];
@@ -241,7 +395,7 @@ __global__ __launch_bounds__(kThreads) void forward_kernel(const FwdParams<T> p,
       for (int d = 0; d < NDIM; ++d) {
         if (prow[d] < 0)
           continue;
-        const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d];
+        const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d], ad = a[d];
         const T *an = p.pml.profile[d], *bn = an + N, *ah = an + 2 * N,
                 *bh = an + 3 * N;
         const int64_t size = slab_size<NDIM>(g, d, L, H);
@@ -249,16 +403,11 @@ __global__ __launch_bounds__(kThreads) void forward_kernel(const FwdParams<T> p,
                  slab_index<NDIM>(g, a, d, prow[d], L, H);
         T Dp, Dm;
         cell_diffs<T, R>(taps[d], g.op, Dp, Dm);
-        const int ad = a[d];
-        const T pp = bh[ad] * aux[0] + ah[ad] * Dp;
-        const T pm = bh[ad - 1] * aux[size] + ah[ad - 1] * Dm;
-        const T z = bn[ad] * aux[2 * size] +
-                    an[ad] * g.f[d] * (gp[d] * (Dp + pp) - gm[d] * (Dm + pm));
-        aux[0] = pp;
-        aux[size] = pm;
-        aux[2 * size] = z;
+        const T share = pml_forward(aux, size, Dp, Dm, gp[d], gm[d], g.f[d], mi,
+                                    an[ad], bn[ad], ah[ad], bh[ad], ah[ad - 1],
+                                    bh[ad - 1]);
         if (!held)
-          v += mi * (g.f[d] * (gp[d] * pp - gm[d] * pm) + z);
+          v += share;
       }
     }
     for (int e = s0; e < s1; ++e)
@@ -294,65 +443,6 @@ __global__ __launch_bounds__(kThreads) void forward_kernel(const FwdParams<T> p,
   }
 }
 
-// ------------------------------------------------------- adjoint helpers
-// scalar_sensitivity.cu:137-154 transposed_divergence_taps over taps
-// l[0 .. 2R] and cell means: hi[j - 1] the cell (a + j - 1, a + j),
-// lo[j - 1] the cell (a - j, a - j + 1)
-template <typename T, int R>
-__device__ __forceinline__ T transposed(const T *l, const T *chi,
-                                        const T *clo, const T *op) {
-  T div = 0, lh = l[R], ll = l[R];
-#pragma unroll
-  for (int j = 1; j <= R; ++j) {
-    const T lh1 = l[R + j], ll1 = l[R - j];
-    div += op[j - 1] * (chi[j - 1] * (lh1 - lh) - clo[j - 1] * (ll - ll1));
-    lh = lh1;
-    ll = ll1;
-  }
-  return div;
-}
-
-// scalar_sensitivity.cu:116-135 stiffness_gradient_taps
-template <typename T, int R>
-__device__ __forceinline__ T stiffness_density(const T *u, const T *l, T lc,
-                                               T sc, T sp, T sm, T factor,
-                                               const T *op) {
-  const T dgp = sp * sp / ((sc + sp) * (sc + sp));
-  const T dgm = sm * sm / ((sc + sm) * (sc + sm));
-  T Dp, Dm;
-  cell_diffs<T, R>(u, op, Dp, Dm);
-  return factor * (dgp * Dp * (l[R + 1] - lc) + dgm * Dm * (lc - l[R - 1]));
-}
-
-// scalar_sensitivity.cu:219-230 d(Dp)/d(u at row q) and d(Dm)/d(u at row q)
-template <typename T, int R>
-__device__ __forceinline__ T plus_tap(int q, const T *op) {
-  if (q >= 1)
-    return q <= R ? op[q - 1] : (T)0;
-  return 1 - q <= R ? -op[-q] : (T)0;
-}
-template <typename T, int R>
-__device__ __forceinline__ T minus_tap(int q, const T *op) {
-  if (q >= 0)
-    return q + 1 <= R ? op[q] : (T)0;
-  return -q <= R ? -op[-q - 1] : (T)0;
-}
-
-// scalar_sensitivity.cu:232-251 pml_tap: the memory of slab row `ai` read
-// back off the row q from it; `cell` the harmonic means along the axis
-template <typename T, int R>
-__device__ __forceinline__ T pml_tap(const T *eta, int64_t size, int64_t i,
-                                     const T *cell, int64_t iidx, int o,
-                                     const T *profile, int N, T f, int ai,
-                                     int q, const T *op) {
-  const T *an = profile, *ah = profile + 2 * N;
-  const T gp = cell[iidx], gm = cell[iidx - o];
-  const T ez = an[ai] * f * eta[2 * size + i];
-  const T cp = ah[ai] * eta[i] + gp * (1 + ah[ai]) * ez;
-  const T cm = ah[ai - 1] * eta[size + i] - gm * (1 + ah[ai - 1]) * ez;
-  return cp * plus_tap<T, R>(q, op) + cm * minus_tap<T, R>(q, op);
-}
-
 // --------------------------------------------------------- adjoint step
 // level n of the adjoint (see cw_backward_args): lambda^n over the slot of
 // lambda^(n + 2), its gradients against u^n, the transposed layer, and the
@@ -362,16 +452,9 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
     const BwdParams<T> p, const int n) {
   constexpr int REACH = R + 2; // scalar_sensitivity.cu:44 WALL_REACH
   const Geom<T> &g = p.g;
-  const int tile = xcd_tile(blockIdx.x, g.ntiles);
-  int a[3], lo[3], hi[3];
-  tile_node<NDIM>(g, tile, a, lo, hi);
-  bool inside = true, deep = true;
-#pragma unroll
-  for (int d = 0; d < NDIM; ++d) {
-    inside &= a[d] > 0 && a[d] < g.n[d] - 1;
-    deep &= lo[d] >= REACH && hi[d] <= g.n[d] - 1 - REACH;
-  }
-  if (!inside)
+  int a[3], tile;
+  bool deep;
+  if (!general_node<NDIM>(p.region, g, REACH, a, tile, deep))
     return;
   int64_t idx = 0;
   bool held = false;
@@ -388,20 +471,6 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
     near[d] = !deep && (a[d] < REACH || a[d] > g.n[d] - 1 - REACH);
   const T mi = p.inv[idx];
   const T sc = p.stiff[idx];
-  // the cell means each deep axis reads: hi side cells a .. a + R - 1 and
-  // lo side cells a - R .. a - 1
-  T chi[NDIM][R], clo[NDIM][R];
-#pragma unroll
-  for (int d = 0; d < NDIM; ++d)
-#pragma unroll
-    for (int j = 1; j <= R; ++j) {
-      if (near[d]) {
-        chi[d][j - 1] = clo[d][j - 1] = 0;
-        continue;
-      }
-      chi[d][j - 1] = p.cell[d * g.field + idx + (int64_t)(j - 1) * g.s[d]];
-      clo[d][j - 1] = p.cell[d * g.field + idx - (int64_t)j * g.s[d]];
-    }
   T sp[NDIM], sm[NDIM];
   if (p.stiff_grad) {
 #pragma unroll
@@ -443,15 +512,18 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
   const T *const l1s = p.lam.at(n + 1);
   const T *const u1s = p.u.at(n);
   T *const uos = p.u.at(n + 1);
+  const int gi = blockIdx.y;
+  const int b0 = gi * p.group;
+  const int b1 = min(b0 + p.group, g.nbatch);
   const bool shared = p.gbatch == 0;
   T accm = 0, accs = 0;
   if (grad && shared) {
-    accm = p.gmass[idx];
+    accm = p.gmass[(int64_t)gi * g.field + idx];
     if (!p.combine)
-      accs = p.gstiff[idx];
+      accs = p.gstiff[(int64_t)gi * g.field + idx];
   }
 
-  for (int b = 0; b < g.nbatch; ++b) {
+  for (int b = b0; b < b1; ++b) {
     const T *__restrict__ l1 = l1s + (int64_t)b * p.lam.batch;
     T *__restrict__ l0 = l0s + (int64_t)b * p.lam.batch;
     const T *__restrict__ u1 = u1s + (int64_t)b * p.u.batch;
@@ -462,6 +534,7 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
     for (int d = 0; d < NDIM; ++d) {
       const int odd = g.dirichlet >> (2 * d);
       T lt[2 * R + 1];
+      T ch[R], cl[R];
       if (near[d]) {
 #pragma unroll
         for (int j = -R; j <= R; ++j)
@@ -472,36 +545,33 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
         for (int j = -R; j <= R; ++j)
           ks[j + R] = reflected(p.stiff, idx, g.s[d], a[d], g.n[d], 0, j);
         ks[R] = sc;
-        T ch[R], cl[R];
 #pragma unroll
         for (int j = 1; j <= R; ++j) {
           ch[j - 1] = harmonic(ks[R + j - 1], ks[R + j]);
           cl[j - 1] = harmonic(ks[R - j + 1], ks[R - j]);
         }
-        div_l += g.f[d] * transposed<T, R>(lt, ch, cl, g.op);
-        if (p.stiff_grad) {
-          T ut[2 * R + 1];
-#pragma unroll
-          for (int j = -R; j <= R; ++j)
-            ut[j + R] = reflected(u1, idx, g.s[d], a[d], g.n[d], odd, j);
-          ut[R] = uc;
-          lt[R] = lw;
-          gsum += stiffness_density<T, R>(ut, lt, lw, sc, sp[d], sm[d],
-                                          g.f[d], g.op);
-        }
       } else {
 #pragma unroll
         for (int j = -R; j <= R; ++j)
           lt[j + R] = l1[idx + (int64_t)j * g.s[d]];
-        div_l += g.f[d] * transposed<T, R>(lt, chi[d], clo[d], g.op);
-        if (p.stiff_grad) {
-          T ut[2 * R + 1];
 #pragma unroll
-          for (int j = -R; j <= R; ++j)
-            ut[j + R] = u1[idx + (int64_t)j * g.s[d]];
-          gsum += stiffness_density<T, R>(ut, lt, lc, sc, sp[d], sm[d],
-                                          g.f[d], g.op);
+        for (int j = 1; j <= R; ++j) {
+          ch[j - 1] = p.cell[d * g.field + idx + (int64_t)(j - 1) * g.s[d]];
+          cl[j - 1] = p.cell[d * g.field + idx - (int64_t)j * g.s[d]];
         }
+      }
+      div_l += g.f[d] * transposed<T, R>(lt, ch, cl, g.op);
+      if (p.stiff_grad && grad) {
+        T ut[2 * R + 1];
+#pragma unroll
+        for (int j = -R; j <= R; ++j)
+          ut[j + R] = near[d] ? reflected(u1, idx, g.s[d], a[d], g.n[d], odd, j)
+                              : u1[idx + (int64_t)j * g.s[d]];
+        ut[R] = uc;
+        if (near[d])
+          lt[R] = lw;
+        gsum += stiffness_density<T, R>(ut, lt, near[d] ? lw : lc, sc, sp[d],
+                                        sm[d], g.f[d], g.op);
       }
     }
     T ln = -lo_ + (T)2 * lc + mi * div_l;
@@ -522,12 +592,9 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
       for (int d = 0; d < NDIM; ++d) {
         if (!ext[d])
           continue;
-        const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d], o = g.s[d];
+        const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d];
         const int64_t size = slab_size<NDIM>(g, d, L, H);
-        const T *eta = p.pml.eta[d][n & 1] + (int64_t)b * 3 * size;
-        const T *cell = p.cell + d * g.field;
-        const T *prof = p.pml.profile[d];
-        const T f = g.f[d];
+        const T *eta = p.pml.eta[d][n & 1] + (int64_t)b * kEta * size;
         T sum = 0;
         int at[3];
 #pragma unroll
@@ -538,10 +605,9 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
           const int row = slab_row(a[d] + k, N, L, H);
           if (row >= 0) {
             at[d] = a[d] + k;
-            sum += pml_tap<T, R>(eta, size,
-                                 slab_index<NDIM>(g, at, d, row, L, H), cell,
-                                 idx + (int64_t)k * o, o, prof, N, f,
-                                 a[d] + k, -k, g.op);
+            const int64_t i = slab_index<NDIM>(g, at, d, row, L, H);
+            sum += eta[3 * size + i] * plus_tap<T, R>(-k, g.op) +
+                   eta[4 * size + i] * minus_tap<T, R>(-k, g.op);
           }
         }
 #pragma unroll
@@ -557,10 +623,9 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
             const int row = slab_row(ai, N, L, H);
             if (row >= 0 && ghost - ai >= -R && ghost - ai <= R) {
               at[d] = ai;
-              fold += pml_tap<T, R>(
-                  eta, size, slab_index<NDIM>(g, at, d, row, L, H), cell,
-                  idx + (int64_t)(ai - a[d]) * o, o, prof, N, f, ai,
-                  ghost - ai, g.op);
+              const int64_t i = slab_index<NDIM>(g, at, d, row, L, H);
+              fold += eta[3 * size + i] * plus_tap<T, R>(ghost - ai, g.op) +
+                      eta[4 * size + i] * minus_tap<T, R>(ghost - ai, g.op);
             }
           }
           sum += ((g.dirichlet >> (2 * d + high)) & 1) ? -fold : fold;
@@ -586,21 +651,18 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
       for (int d = 0; d < NDIM; ++d) {
         if (prow[d] < 0)
           continue;
-        const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d];
+        const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d], ad = a[d];
         const int64_t size = slab_size<NDIM>(g, d, L, H);
         const int64_t i = slab_index<NDIM>(g, a, d, prow[d], L, H);
-        const T *cur = p.pml.eta[d][n & 1] + (int64_t)b * 3 * size + i;
-        T *nxt = p.pml.eta[d][(n + 1) & 1] + (int64_t)b * 3 * size + i;
-        const T *an = p.pml.profile[d], *bn = an + N, *bh = an + 3 * N;
-        const T f = g.f[d];
-        const T gpd = p.cell[d * g.field + idx];
-        const T gmd = p.cell[d * g.field + idx - g.s[d]];
-        const T q = held ? (T)0 : w * ln;
-        const T ez = cur[2 * size];
-        nxt[0] = bh[a[d]] * (cur[0] + an[a[d]] * f * gpd * ez) + f * gpd * q;
-        nxt[size] =
-            bh[a[d] - 1] * (cur[size] - an[a[d]] * f * gmd * ez) - f * gmd * q;
-        nxt[2 * size] = bn[a[d]] * ez + q;
+        const T *an = p.pml.profile[d], *bn = an + N, *ah = an + 2 * N,
+                *bh = an + 3 * N;
+        pml_adjoint_update(p.pml.eta[d][n & 1] + (int64_t)b * kEta * size + i,
+                           p.pml.eta[d][(n + 1) & 1] +
+                               (int64_t)b * kEta * size + i,
+                           size, held ? (T)0 : w * ln,
+                           p.cell[d * g.field + idx],
+                           p.cell[d * g.field + idx - g.s[d]], g.f[d], an[ad],
+                           bn[ad], ah[ad], bh[ad], ah[ad - 1], bh[ad - 1]);
       }
     }
     if (grad) {
@@ -619,14 +681,13 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
     if (reverse) {
       // the lossless step run backwards: u^(n - 1) from u^n and u^(n + 1)
       T taps[NDIM][2 * R + 1];
-      const bool fwd_deep = deep; // REACH > R: deep here is deep for the step
 #pragma unroll
       for (int d = 0; d < NDIM; ++d)
 #pragma unroll
         for (int j = -R; j <= R; ++j)
-          taps[d][j + R] = fwd_deep ? u1[idx + (int64_t)j * g.s[d]]
-                                    : mirrored(u1, idx, g.s[d], a[d], g.n[d],
-                                               g.dirichlet >> (2 * d), j);
+          taps[d][j + R] = deep ? u1[idx + (int64_t)j * g.s[d]]
+                                : mirrored(u1, idx, g.s[d], a[d], g.n[d],
+                                           g.dirichlet >> (2 * d), j);
       T div[NDIM];
 #pragma unroll
       for (int d = 0; d < NDIM; ++d)
@@ -663,9 +724,9 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
     }
   }
   if (grad && shared) {
-    p.gmass[idx] = accm;
+    p.gmass[(int64_t)gi * g.field + idx] = accm;
     if (!p.combine)
-      p.gstiff[idx] = accs;
+      p.gstiff[(int64_t)gi * g.field + idx] = accs;
   }
 }
 

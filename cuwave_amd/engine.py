@@ -78,9 +78,30 @@ class Propagator:
                 for _, size in self.slabs]
 
     def eta_memory(self) -> list[torch.Tensor]:
-        """Zeroed adjoint layer memory, two alternating buffers per lined axis."""
-        return [torch.zeros((2, self.nbatch, 3, size), dtype=self.sim.dtype, device=self.device)
+        """Zeroed adjoint layer memory, two alternating buffers per lined axis.
+
+        A buffer holds (nbatch, 5, slab): the recursion's three variables and
+        the two tap fields the neighbours' transposed taps read.
+        """
+        return [torch.zeros((2, self.nbatch, 5, size), dtype=self.sim.dtype, device=self.device)
                 for _, size in self.slabs]
+
+    def accumulator(self) -> torch.Tensor:
+        """Zeroed shot-summed gradient accumulator: (groups, *Nx_padded), summed by `reduce`.
+
+        The adjoint kernels step a group of shots together and add them into
+        that group's field, so no two threads write one node (cw_scalar_grad_groups).
+        """
+        groups = _abi.I32()
+        _abi.call("cw_scalar_grad_groups", ctypes.byref(self.grid), ctypes.byref(self.pml),
+                  ctypes.byref(groups))
+        return torch.zeros((groups.value, *self.sim.Nx_padded), dtype=self.sim.dtype,
+                           device=self.device)
+
+    @staticmethod
+    def reduce(acc: torch.Tensor) -> torch.Tensor:
+        """The gradient an accumulator holds: the sum of its groups."""
+        return acc[0].clone() if acc.shape[0] == 1 else acc.sum(0)
 
     def _layer(self, aux=None, eta=None) -> _abi.Pml:
         layer = _abi.Pml()

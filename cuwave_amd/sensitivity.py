@@ -180,10 +180,10 @@ def sensitivity(sim: Simulation, source: Source, indicator, sensors, objective):
     adj = sensor_points(prop, sensors, weight=sim.excitation_weights(mat, lin),
                         shot_stride=sim.N * nrec)
     P = torch.zeros((2, 1, *sim.Nx_padded), dtype=sim.dtype, device=dev)
-    grads = {k: torch.zeros(sim.Nx_padded, dtype=sim.dtype, device=dev) for k in ("mass", "stiff")}
-    prop.backward(P, V, 0, sim.N, u_offset=2, eta=prop.eta_memory(), g_mass=grads["mass"],
-                  g_stiff=grads["stiff"], sensors=adj, adjoint=adjoint)
-    grads = _finalize(sim, grads, dev)
+    acc = {k: prop.accumulator() for k in ("mass", "stiff")}
+    prop.backward(P, V, 0, sim.N, u_offset=2, eta=prop.eta_memory(), g_mass=acc["mass"],
+                  g_stiff=acc["stiff"], sensors=adj, adjoint=adjoint)
+    grads = _finalize(sim, {k: prop.reduce(a) for k, a in acc.items()}, dev)
     if sim.pml is not None:
         grads = _zero_outside(grads, pml_nodes(sim), dev)
     return cost, grads, um, {}
@@ -343,14 +343,14 @@ def reconstruction_sensitivity(sim: Simulation, source: Source, indicator, senso
     mat = sim.build_materials(indicator)
     run = _Reconstruction(sim, mat, [source], sensors, strip_dtype, checkpoints)
     dev = run.dev
-    grads = {k: torch.zeros(sim.Nx_padded, dtype=sim.dtype, device=dev) for k in ("mass", "stiff")}
+    acc = {k: run.prop.accumulator() for k in ("mass", "stiff")}
 
     def objectives(record):
         cost, dphi = objective(record[0])
         return [cost], torch.as_tensor(dphi, dtype=sim.dtype, device=dev)[None]
 
-    costs = run.run(objectives, dict(g_mass=grads["mass"], g_stiff=grads["stiff"], skip_layer=True))
-    grads = _finalize(sim, grads, dev)
+    costs = run.run(objectives, dict(g_mass=acc["mass"], g_stiff=acc["stiff"], skip_layer=True))
+    grads = _finalize(sim, {k: run.prop.reduce(a) for k, a in acc.items()}, dev)
     grads = _zero_outside(grads, ~run.valid, dev)
     info = {"strip": run.count, "drift": run.drift(), "segment": run.segment,
             "sweeps": run.steps / sim.N}

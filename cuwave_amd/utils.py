@@ -194,9 +194,8 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
     gradient = None
     for start, chunk in _batches(sources, batch):
         run = _Reconstruction(sim, mat, chunk, sensors.nodes, checkpoints=checkpoints)
-        if gradient is None:
-            gradient = torch.zeros(sim.Nx_padded, dtype=sim.dtype, device=run.dev)
-            valid = run.valid
+        acc = run.prop.accumulator()
+        valid = run.valid
 
         def objectives(record, start=start, chunk=chunk):
             costs, dphis = [], []
@@ -206,8 +205,10 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
                 dphis.append(torch.as_tensor(dphi, dtype=sim.dtype, device=record.device))
             return costs, torch.stack(dphis)
 
-        costs = run.run(objectives, dict(g_mass=gradient, combine=(jm, js), skip_layer=True))
+        costs = run.run(objectives, dict(g_mass=acc, combine=(jm, js), skip_layer=True))
         cost += sum(costs)
+        part = run.prop.reduce(acc)
+        gradient = part if gradient is None else gradient + part
     grads = _finalize(sim, {"g": gradient}, gradient.device)
     grads = _zero_outside(grads, ~valid, gradient.device)
     return cost, grads["g"]

@@ -61,7 +61,10 @@ typedef struct cw_pml {
   int32_t lo[3], hi[3];     /* layer rows behind the low / high face, 0 none */
   const void *profile[3];   /* (4, n[d]): a, b at nodes and at half points */
   void *aux[3];             /* forward memory: (nbatch, 3, slab nodes) */
-  void *eta[3][2];          /* adjoint memory, two buffers alternating */
+  void *eta[3][2];          /* adjoint memory, two buffers alternating, each
+                               (nbatch, 5, slab nodes): the three recursion
+                               variables, then the two tap fields the
+                               neighbours' transposed taps read; zeroed */
 } cw_pml;
 
 /* points a step injects at or records from, bucketed by the library's tiles
@@ -140,7 +143,8 @@ typedef struct cw_backward_args {
   const void *signal;
   cw_strip strip;
   void *g_mass, *g_stiff;  /* accumulators (cuwave/scalar.py:751-756) */
-  int64_t grad_batch_stride; /* 0: the shots sum into one field */
+  int64_t grad_batch_stride; /* per-shot fields; 0: the shots sum into the
+                                cw_scalar_grad_groups fields */
   int32_t combine;         /* 1: g_mass += jm gm + js gs (g_stiff unused) */
   double jac_mass, jac_stiff;
   int32_t skip_layer;      /* 1: no gradient inside the layer (zeroed after) */
@@ -170,6 +174,13 @@ int cw_tile_count(const cw_grid *grid, int32_t *tiles);
  * (count) the point index in bucketed order, stable; host memory */
 int cw_point_lists(const cw_grid *grid, const int32_t *lin, int32_t count,
                    int32_t *starts, int32_t *order);
+
+/* the gradient accumulators a shot-summed adjoint call (grad_batch_stride 0)
+ * adds into: `groups` fields one after the other, field k summing the shots
+ * of group k (the kernels step a group's shots together), summed by the
+ * caller once the sweep is done */
+int cw_scalar_grad_groups(const cw_grid *grid, const cw_pml *pml,
+                          int32_t *groups);
 
 /* the inverse inertia and the harmonic cell stiffness of every axis
  * (cuwave/kernels/scalar.cu:48-64 `sc sp / (sc + sp)`, :454) */
