@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -31,6 +31,7 @@ class Grid(ctypes.Structure):
         ("n", I32 * 3),
         ("np", I32 * 3),
         ("nbatch", I32),
+        ("lanes", I32),
         ("dtype", I32),
         ("dirichlet", I32),
         ("neumann", I32),
@@ -60,7 +61,8 @@ class Points(ctypes.Structure):
         ("shot", VP),
         ("col", VP),
         ("weight", VP),
-        ("columns", I32),
+        ("row_stride", I64),
+        ("col_stride", I64),
         ("shot_stride", I64),
     ]
 
@@ -80,7 +82,6 @@ class Levels(ctypes.Structure):
     _fields_ = [
         ("base", VP),
         ("slot_stride", I64),
-        ("batch_stride", I64),
         ("slots", I32),
         ("offset", I32),
     ]
@@ -116,7 +117,6 @@ class BackwardArgs(ctypes.Structure):
         ("strip", Strip),
         ("g_mass", VP),
         ("g_stiff", VP),
-        ("grad_batch_stride", I64),
         ("combine", I32),
         ("jac_mass", F64),
         ("jac_stiff", F64),
@@ -147,7 +147,6 @@ SIGNATURES = {
     "cw_last_error": ([], ctypes.c_char_p),
     "cw_tile_count": ([ctypes.POINTER(Grid), ctypes.POINTER(I32)], I32),
     "cw_point_lists": ([ctypes.POINTER(Grid), VP, I32, VP, VP], I32),
-    "cw_scalar_grad_groups": ([ctypes.POINTER(Grid), ctypes.POINTER(Pml), ctypes.POINTER(I32)], I32),
     "cw_scalar_prepare": ([ctypes.POINTER(Grid), ctypes.POINTER(Materials), VP], I32),
     "cw_scalar_forward": ([ctypes.POINTER(ForwardArgs), I32, I32, VP], I32),
     "cw_scalar_backward": ([ctypes.POINTER(BackwardArgs), I32, I32, VP], I32),

@@ -1,11 +1,12 @@
 // Device-side geometry shared by the pressure kernels for gfx950.
 //
-// A launch walks tiles of one shot's grid: 64 consecutive nodes of the fast
-// (contiguous) axis by 4 rows of the next axis (256 nodes on a 1D grid), one
-// thread a node, each thread stepping its node for every shot of the batch so
-// the material loads and the precomputed cell means serve all of them. Tiles
-// are dealt to the 8 XCDs in contiguous runs, so vertically adjacent tiles,
-// which share their halo rows, sit behind the same L2.
+// Field layout: a level of a batched field is (*np, lanes): the shots of one
+// node sit next to each other (lanes, a power of two >= the shot count, the
+// extra lanes padding that stays zero). A wave's 64 lanes are then 64 / lanes
+// neighbouring nodes times their shots: every field access of a wave is one
+// contiguous run (128 B for 32 float shots), a stencil tap along any axis is
+// another such run, and a material value is one address the node's lanes
+// share. Materials, cell means and gradients are per node.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,8 +15,11 @@
 namespace cw {
 
 constexpr int kThreads = 256;
+constexpr int kWave = 64;
 constexpr int kXcds = 8;
 
+// the point-list tiles: 64 nodes of the fast axis by 4 rows of the next one
+// (256 nodes on a 1D grid); a node's tile names the list it scans
 template <int NDIM> struct TileShape {
   static constexpr int TX = NDIM == 1 ? 256 : 64;
   static constexpr int TY = kThreads / TX;
@@ -25,18 +29,20 @@ template <int NDIM> struct TileShape {
 template <typename T> struct Geom {
   int n[3];          // logical extents (ghosts included)
   int np[3];         // padded extents
-  int s[3];          // element strides, s[ndim - 1] == 1
-  int64_t field;     // elements of one shot's field
-  int nbatch;
+  int s[3];          // node strides, s[ndim - 1] == 1
+  int64_t nodes;     // nodes of one field
+  int lanes, lshift; // shot lanes a node holds, log2 of it
+  int nbatch;        // shots that run (lanes beyond are padding)
   int dirichlet;     // held (odd) faces, bit 2 axis + side
   int neumann;       // refilled even faces
-  int tf, tm, ts;    // tiles along the fast, middle and slow axes
+  int tf, tm, ts;    // point tiles along the fast, middle and slow axes
   int ntiles;
+  int interior;      // interior nodes (every axis 1 .. n - 2)
   T op[4];           // op[k - 1], the full-radius cell weights
   T f[3];            // step factors per axis
 };
 
-// block -> tile: contiguous runs of tiles per XCD, the hardware dealing
+// block -> work item: contiguous runs of items per XCD, the hardware dealing
 // consecutive blocks round-robin over the XCDs
 __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
   const int q = ntiles / kXcds, r = ntiles % kXcds;
@@ -44,36 +50,15 @@ __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
   return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
 }
 
-// the node of this thread in tile `tile`: a[d] per axis, the tile's first
-// node per axis in lo[d] and its last in hi[d]
+// the point tile of node a
 template <int NDIM, typename T>
-__device__ __forceinline__ void tile_node(const Geom<T> &g, int tile, int *a,
-                                          int *lo, int *hi) {
+__device__ __forceinline__ int point_tile(const Geom<T> &g, const int *a) {
   constexpr int TX = TileShape<NDIM>::TX, TY = TileShape<NDIM>::TY;
-  const int tx = tile % g.tf, rest = tile / g.tf;
-  const int ty = rest % g.tm, tz = rest / g.tm;
-  const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
-  if (NDIM == 1) {
-    a[0] = tx * TX + lx;
-    lo[0] = tx * TX;
-    hi[0] = lo[0] + TX - 1;
-  } else if (NDIM == 2) {
-    a[1] = tx * TX + lx;
-    lo[1] = tx * TX;
-    hi[1] = lo[1] + TX - 1;
-    a[0] = ty * TY + ly;
-    lo[0] = ty * TY;
-    hi[0] = lo[0] + TY - 1;
-  } else {
-    a[2] = tx * TX + lx;
-    lo[2] = tx * TX;
-    hi[2] = lo[2] + TX - 1;
-    a[1] = ty * TY + ly;
-    lo[1] = ty * TY;
-    hi[1] = lo[1] + TY - 1;
-    a[0] = tz;
-    lo[0] = hi[0] = tz;
-  }
+  if (NDIM == 1)
+    return a[0] / TX;
+  if (NDIM == 2)
+    return (a[0] / TY) * g.tf + a[1] / TX;
+  return (a[0] * g.tm + a[1] / TY) * g.tf + a[2] / TX;
 }
 
 template <typename T> __device__ __forceinline__ T fmat(T a, T b, T c);
@@ -89,29 +74,31 @@ __device__ __forceinline__ bool held_wall(int a, int N, int odd) {
   return ((odd & 1) && a == 1) || ((odd & 2) && a == N - 2);
 }
 
-// tap j of the line through idx at node a, past the stored ghost the value
-// of its mirror about the wall node, negated past a face of `odd`
+// tap j of the line through element e at node a (s the element stride of
+// the axis), past the stored ghost the value of its mirror about the wall
+// node, negated past a face of `odd`
 template <typename T>
-__device__ __forceinline__ T mirrored(const T *__restrict__ u, int64_t idx,
-                                      int s, int a, int N, int odd, int j) {
+__device__ __forceinline__ T mirrored(const T *__restrict__ u, int64_t e,
+                                      int64_t s, int a, int N, int odd, int j) {
   int b = a + j;
   const bool neg = (b < 0 && (odd & 1)) || (b > N - 1 && (odd & 2));
   b = b < 0 ? 2 - b : b > N - 1 ? 2 * (N - 2) - b : b;
-  const T v = u[idx + (int64_t)(b - a) * s];
+  const T v = u[e + (int64_t)(b - a) * s];
   return neg ? -v : v;
 }
 
 // tap j through the image about the wall node, for a field no refill keeps
 // past it, zero on a held wall node
 template <typename T>
-__device__ __forceinline__ T reflected(const T *__restrict__ u, int64_t idx,
-                                       int s, int a, int N, int odd, int j) {
+__device__ __forceinline__ T reflected(const T *__restrict__ u, int64_t e,
+                                       int64_t s, int a, int N, int odd,
+                                       int j) {
   int b = a + j;
   const bool neg = (b < 1 && (odd & 1)) || (b > N - 2 && (odd & 2));
   b = b < 1 ? 2 - b : b > N - 2 ? 2 * (N - 2) - b : b;
   if (held_wall(b, N, odd))
     return (T)0;
-  const T v = u[idx + (int64_t)(b - a) * s];
+  const T v = u[e + (int64_t)(b - a) * s];
   return neg ? -v : v;
 }
 
@@ -146,6 +133,15 @@ __device__ __forceinline__ int64_t slab_size(const Geom<T> &g, int d, int lo,
     if (e != d)
       plane *= g.np[e];
   return plane * (int64_t)(lo + hi);
+}
+
+// the sum over a node's shot lanes (a butterfly: every lane of the node ends
+// with the same bits); every lane of the node must take part
+template <typename T>
+__device__ __forceinline__ T shot_sum(T v, int lanes) {
+  for (int m = 1; m < lanes; m <<= 1)
+    v += __shfl_xor(v, m, kWave);
+  return v;
 }
 
 // round-to-nearest-even bfloat16 bits of a float, as a narrowed record keeps

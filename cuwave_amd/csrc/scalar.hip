@@ -1,6 +1,6 @@
 // The C ABI of include/cuwave_amd.h for the pressure family: argument
 // checks, template dispatch over (precision, dimension, radius), and the
-// time loops that launch one fused kernel a step.
+// time loops that launch the fused kernels of a step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +31,13 @@ int hip_status(const char *where) {
   return 0;
 }
 
+int log2_of(int v) {
+  int k = 0;
+  while ((1 << k) < v)
+    ++k;
+  return k;
+}
+
 int check_grid(const cw_grid *g) {
   if (!g)
     return fail("grid is NULL");
@@ -42,18 +49,21 @@ int check_grid(const cw_grid *g) {
     return fail("dtype must be 0 (float32) or 1 (float64)");
   if (g->nbatch < 1)
     return fail("nbatch must be at least 1");
+  if (g->lanes < g->nbatch || g->lanes > cw::kWave ||
+      (g->lanes & (g->lanes - 1)))
+    return fail("lanes must be a power of two from nbatch to 64");
   const int R = g->order / 2;
-  int64_t field = 1;
+  int64_t nodes = 1;
   for (int d = 0; d < g->ndim; ++d) {
     if (g->n[d] < R + 3)
       return fail("every axis needs order / 2 + 3 nodes to mirror its taps");
     const int want = d == g->ndim - 1 ? (g->n[d] + 31) / 32 * 32 : g->n[d];
     if (g->np[d] != want)
       return fail("np must be n with the last axis padded to a multiple of 32");
-    field *= g->np[d];
+    nodes *= g->np[d];
   }
-  if (field >= (int64_t)1 << 31)
-    return fail("one shot's field overflows the int32 node index");
+  if (nodes * g->lanes >= (int64_t)1 << 31)
+    return fail("one level of the batch overflows the int32 element index");
   return 0;
 }
 
@@ -71,7 +81,9 @@ template <typename T> cw::Geom<T> make_geom(const cw_grid &gr) {
   }
   for (int d = nd; d < 3; ++d)
     g.s[d] = 0;
-  g.field = stride;
+  g.nodes = stride;
+  g.lanes = gr.lanes;
+  g.lshift = log2_of(gr.lanes);
   g.nbatch = gr.nbatch;
   g.dirichlet = gr.dirichlet;
   g.neumann = gr.neumann;
@@ -81,6 +93,9 @@ template <typename T> cw::Geom<T> make_geom(const cw_grid &gr) {
   g.tm = nd >= 2 ? (g.n[nd - 2] + TY - 1) / TY : 1;
   g.ts = nd == 3 ? g.n[0] : 1;
   g.ntiles = g.tf * g.tm * g.ts;
+  g.interior = 1;
+  for (int d = 0; d < nd; ++d)
+    g.interior *= g.n[d] - 2;
   for (int k = 0; k < 4; ++k)
     g.op[k] = (T)gr.op[k];
   for (int d = 0; d < 3; ++d)
@@ -95,8 +110,9 @@ template <typename T> cw::Points<T> make_points(const cw_points &p) {
   q.shot = p.shot;
   q.col = p.col;
   q.weight = (const T *)p.weight;
-  q.columns = p.columns;
-  q.shot_stride = p.shot_stride;
+  q.rs = p.row_stride;
+  q.cs = p.col_stride;
+  q.ss = p.shot_stride;
   return q;
 }
 
@@ -104,7 +120,6 @@ template <typename T> cw::Levels<T> make_levels(const cw_levels &l) {
   cw::Levels<T> q{};
   q.base = (T *)l.base;
   q.slot = l.slot_stride;
-  q.batch = l.batch_stride;
   q.slots = l.slots;
   q.offset = l.offset;
   return q;
@@ -120,6 +135,17 @@ template <typename T> cw::Layer<T> make_layer(const cw_pml &p, int ndim) {
     q.eta[d][0] = (T *)p.eta[d][0];
     q.eta[d][1] = (T *)p.eta[d][1];
   }
+  return q;
+}
+
+template <typename T> cw::Strip<T> make_strip(const cw_strip &s) {
+  cw::Strip<T> q{};
+  q.map = s.map;
+  q.record = s.record;
+  q.count = s.count;
+  q.first = s.first;
+  q.rows = s.rows;
+  q.bf16 = s.bf16;
   return q;
 }
 
@@ -143,6 +169,17 @@ int check_layer(const cw_grid &g, const cw_pml &p, bool forward) {
     if (forward ? !p.aux[d] : (!p.eta[d][0] || !p.eta[d][1]))
       return fail("a lined axis needs its memory");
   }
+  return 0;
+}
+
+int check_points(const cw_points &p, bool weighted, const char *what) {
+  if (!p.starts)
+    return 0;
+  if (!p.lin || !p.shot || !p.col || (weighted && !p.weight))
+    return fail(std::string(what) + ": a point list needs lin, shot, col" +
+                (weighted ? " and weight" : ""));
+  if (p.row_stride < 0 || p.col_stride < 0 || p.shot_stride < 0)
+    return fail(std::string(what) + ": strides must be non-negative");
   return 0;
 }
 
@@ -171,7 +208,7 @@ template <typename T, int NDIM, int R> struct Prepare {
   static int run(const cw_grid &gr, const cw_materials &m,
                  hipStream_t stream) {
     const cw::Geom<T> g = make_geom<T>(gr);
-    const int blocks = (int)((g.field + cw::kThreads - 1) / cw::kThreads);
+    const int blocks = (int)((g.nodes + cw::kThreads - 1) / cw::kThreads);
     if (R == 1) // one preparation serves every radius
       hipLaunchKernelGGL((cw::prepare_kernel<T, NDIM>), dim3(blocks),
                          dim3(cw::kThreads), 0, stream, g, (const T *)m.stiff,
@@ -183,7 +220,7 @@ template <typename T, int NDIM, int R> struct Prepare {
 };
 
 // ---------------------------------------------------------- stream box
-constexpr int kChunk = 32; // rows a streaming wave marches
+constexpr int kChunk = 32; // rows a streaming thread marches
 
 // the stream box of a 2D grid: every node order / 2 + 2 off the walls, out
 // of reach of the wall mirror, the refill, the held nodes and the adjoint's
@@ -191,28 +228,26 @@ constexpr int kChunk = 32; // rows a streaming wave marches
 cw::Box stream_box(const cw_grid &g) {
   cw::Box box{};
   const int margin = g.order / 2 + 2;
+  box.chunk = kChunk;
+  if (g.ndim != 2 || g.n[0] - margin <= margin || g.n[1] - margin <= margin)
+    return box;
   box.y0 = margin;
   box.y1 = g.n[0] - margin;
   box.x0 = margin;
-  box.x1 = g.ndim == 2 ? g.n[1] - margin : 0;
-  box.chunk = kChunk;
-  if (g.ndim != 2 || box.y0 >= box.y1 || box.x0 >= box.x1) {
-    box.y0 = box.y1 = box.x0 = box.x1 = 0;
-    return box;
-  }
-  box.tx0 = box.x0 / cw::kLanes;
-  box.tx1 = (box.x1 - 1) / cw::kLanes + 1;
+  box.x1 = g.n[1] - margin;
   return box;
 }
 
-int box_items(const cw::Box &b, int nbatch, int group) {
+int box_blocks(const cw::Box &b, int lanes) {
   if (b.y0 >= b.y1)
     return 0;
+  const int cols = cw::kThreads / lanes;
+  const int xa = b.x0 / cols * cols;
   const int nchunk = (b.y1 - b.y0 + b.chunk - 1) / b.chunk;
-  return nchunk * (b.tx1 - b.tx0) * ((nbatch + group - 1) / group);
+  return nchunk * ((b.x1 - xa + cols - 1) / cols);
 }
 
-// where the general kernels run: the band around the box, or every tile
+// where the general kernels run: the band around the box, or every node
 cw::Region band_of(const cw_grid &g, const cw::Box &b) {
   cw::Region r{};
   if (b.y0 >= b.y1)
@@ -228,13 +263,14 @@ cw::Region band_of(const cw_grid &g, const cw::Box &b) {
   return r;
 }
 
-int general_blocks(const cw::Geom<float> &g, const cw::Region &r) {
-  return r.band ? (r.count + cw::kThreads - 1) / cw::kThreads : g.ntiles;
+int general_blocks(const cw_grid &gr, const cw::Region &r) {
+  const int per = cw::kThreads / gr.lanes; // nodes a block takes
+  int64_t nodes = r.band ? r.count : 1;
+  if (!r.band)
+    for (int d = 0; d < gr.ndim; ++d)
+      nodes *= gr.n[d] - 2;
+  return (int)((nodes + per - 1) / per);
 }
-
-// shots a thread or wave of the adjoint takes, sharing one gradient
-// accumulator: the caller's accumulator holds ceil(nbatch / group) groups
-int adjoint_group(int nbatch) { return nbatch >= 2 ? 2 : 1; }
 
 template <typename T, int NDIM, int R> struct Forward {
   static int run(const cw_forward_args &a, int t0, int t1,
@@ -249,27 +285,18 @@ template <typename T, int NDIM, int R> struct Forward {
     p.signal = (const T *)a.signal;
     p.rec = make_points<T>(a.sensors);
     p.record = (T *)a.record;
-    p.strip.map = a.strip.map;
-    p.strip.record = a.strip.record;
-    p.strip.count = a.strip.count;
-    p.strip.first = a.strip.first;
-    p.strip.rows = a.strip.rows;
-    p.strip.bf16 = a.strip.bf16;
-    p.steps = a.steps;
+    p.strip = make_strip<T>(a.strip);
     const cw::Box box = stream_box(a.grid);
     p.region = band_of(a.grid, box);
-    // the band is thin: a thread a shot; a whole grid: a thread all shots
-    p.group = p.region.band ? 1 : p.g.nbatch;
-    const dim3 grid(general_blocks(make_geom<float>(a.grid), p.region),
-                    (p.g.nbatch + p.group - 1) / p.group);
-    const int items = box_items(box, p.g.nbatch, 1);
+    const int general = general_blocks(a.grid, p.region);
+    const int streams = NDIM == 2 ? box_blocks(box, a.grid.lanes) : 0;
     for (int t = t0; t < t1; ++t) {
-      if (grid.x)
-        hipLaunchKernelGGL((cw::forward_kernel<T, NDIM, R>), grid,
+      if (general)
+        hipLaunchKernelGGL((cw::forward_kernel<T, NDIM, R>), dim3(general),
                            dim3(cw::kThreads), 0, stream, p, t);
-      if (NDIM == 2 && items)
-        hipLaunchKernelGGL((cw::forward_stream<T, R>), dim3(items),
-                           dim3(cw::kLanes), 0, stream, p, box, t);
+      if (streams)
+        hipLaunchKernelGGL((cw::forward_stream<T, R>), dim3(streams),
+                           dim3(cw::kThreads), 0, stream, p, box, t);
     }
     return hip_status("forward");
   }
@@ -291,15 +318,9 @@ template <typename T, int NDIM, int R> struct Backward {
     p.adjoint = (const T *)a.adjoint_signal;
     p.src = make_points<T>(a.sources);
     p.signal = (const T *)a.signal;
-    p.strip.map = a.strip.map;
-    p.strip.record = a.strip.record;
-    p.strip.count = a.strip.count;
-    p.strip.first = a.strip.first;
-    p.strip.rows = a.strip.rows;
-    p.strip.bf16 = a.strip.bf16;
+    p.strip = make_strip<T>(a.strip);
     p.gmass = (T *)a.g_mass;
     p.gstiff = (T *)a.g_stiff;
-    p.gbatch = a.grad_batch_stride;
     p.combine = a.combine;
     p.jm = (T)a.jac_mass;
     p.js = (T)a.jac_stiff;
@@ -309,44 +330,31 @@ template <typename T, int NDIM, int R> struct Backward {
     p.steps = a.steps;
     const cw::Box box = stream_box(a.grid);
     p.region = band_of(a.grid, box);
-    p.group = adjoint_group(p.g.nbatch);
-    const dim3 grid(general_blocks(make_geom<float>(a.grid), p.region),
-                    (p.g.nbatch + p.group - 1) / p.group);
-    const int items = box_items(box, p.g.nbatch, p.group);
+    const int general = general_blocks(a.grid, p.region);
+    const int streams = NDIM == 2 ? box_blocks(box, a.grid.lanes) : 0;
     for (int n = n1 - 1; n >= n0; --n) {
-      if (grid.x)
-        hipLaunchKernelGGL((cw::backward_kernel<T, NDIM, R>), grid,
+      if (general)
+        hipLaunchKernelGGL((cw::backward_kernel<T, NDIM, R>), dim3(general),
                            dim3(cw::kThreads), 0, stream, p, n);
-      if (NDIM == 2 && items)
-        launch_stream(p, box, items, n, stream);
+      if (streams) {
+        if (p.stiff_grad)
+          hipLaunchKernelGGL((cw::backward_stream<T, R, true>), dim3(streams),
+                             dim3(cw::kThreads), 0, stream, p, box, n);
+        else
+          hipLaunchKernelGGL((cw::backward_stream<T, R, false>),
+                             dim3(streams), dim3(cw::kThreads), 0, stream, p,
+                             box, n);
+      }
     }
     return hip_status("backward");
-  }
-
-  static void launch_stream(const cw::BwdParams<T> &p, const cw::Box &box,
-                            int items, int n, hipStream_t stream) {
-#define CW_BWD(SS, ST)                                                         \
-  hipLaunchKernelGGL((cw::backward_stream<T, R, SS, ST>), dim3(items),         \
-                     dim3(cw::kLanes), 0, stream, p, box, n)
-    if (p.stiff_grad) {
-      if (p.group == 2)
-        CW_BWD(2, true);
-      else
-        CW_BWD(1, true);
-    } else {
-      if (p.group == 2)
-        CW_BWD(2, false);
-      else
-        CW_BWD(1, false);
-    }
-#undef CW_BWD
   }
 };
 
 template <typename T, int NDIM, int R> struct Superposed {
   static int run(const cw_superposed_args &a, hipStream_t stream) {
     const cw::Geom<T> g = make_geom<T>(a.grid);
-    hipLaunchKernelGGL((cw::superposed_kernel<T, NDIM, R>), dim3(g.ntiles),
+    const int blocks = general_blocks(a.grid, cw::Region{});
+    hipLaunchKernelGGL((cw::superposed_kernel<T, NDIM, R>), dim3(blocks),
                        dim3(cw::kThreads), 0, stream, g, (const T *)a.mat.inv,
                        (const T *)a.mat.cell, (const T *)a.u0, (const T *)a.u1,
                        (T *)a.u2, (T *)a.acc_mass, (T *)a.acc_stiff, (T)a.ft,
@@ -354,17 +362,6 @@ template <typename T, int NDIM, int R> struct Superposed {
     return hip_status("superposed_kernel");
   }
 };
-
-int check_points(const cw_points &p, bool weighted, const char *what) {
-  if (!p.starts)
-    return 0;
-  if (!p.lin || !p.shot || !p.col || (weighted && !p.weight))
-    return fail(std::string(what) + ": a point list needs lin, shot, col" +
-                (weighted ? " and weight" : ""));
-  if (p.columns < 1)
-    return fail(std::string(what) + ": columns must be positive");
-  return 0;
-}
 
 } // namespace
 
@@ -395,7 +392,7 @@ int cw_point_lists(const cw_grid *grid, const int32_t *lin, int32_t count,
   const int TY = cw::kThreads / TX;
   std::vector<int32_t> tile(count);
   for (int j = 0; j < count; ++j) {
-    if (lin[j] < 0 || lin[j] >= g.field)
+    if (lin[j] < 0 || lin[j] >= g.nodes)
       return fail("a point lies outside the field");
     int a[3] = {0, 0, 0};
     int64_t rest = lin[j];
@@ -417,17 +414,6 @@ int cw_point_lists(const cw_grid *grid, const int32_t *lin, int32_t count,
   std::vector<int32_t> fill(starts, starts + g.ntiles);
   for (int j = 0; j < count; ++j)
     order[fill[tile[j]]++] = j;
-  return 0;
-}
-
-int cw_scalar_grad_groups(const cw_grid *grid, const cw_pml *pml,
-                          int32_t *groups) {
-  if (int e = check_grid(grid))
-    return e;
-  if (!pml || !groups)
-    return fail("grad groups need the pml and an output");
-  const int S = adjoint_group(grid->nbatch);
-  *groups = (grid->nbatch + S - 1) / S;
   return 0;
 }
 
@@ -526,7 +512,7 @@ int cw_scalar_superposed(const cw_superposed_args *args, void *stream) {
     return fail("args is NULL");
   if (int e = check_grid(&args->grid))
     return e;
-  if (args->grid.nbatch != 1)
+  if (args->grid.nbatch != 1 || args->grid.lanes != 1)
     return fail("the superposition step takes one shot a call");
   if (!args->mat.inv || !args->mat.cell || !args->u0 || !args->u1 ||
       !args->u2 || !args->acc_mass || !args->acc_stiff)

@@ -1,7 +1,11 @@
 // The pressure-family kernels for gfx950, general form: any dimension, walls,
 // PML, points. On a 2D grid they cover only the band of nodes within reach of
-// a wall (the streaming kernels of stream.hpp take everything else); in 3D and
-// on grids too small for a stream they cover every node.
+// a wall (the streaming kernels of stream.hpp take everything else); in 1D
+// and 3D they cover every node.
+//
+// A thread steps one shot of one node: a block's 256 threads are
+// 256 / lanes nodes times their shot lanes, so the lanes of a node are
+// neighbouring threads of one wave (see common.hpp for the layout).
 //
 // Every kernel rounds as the reference's kernels do on its IEEE CPU path: no
 // contraction (built with -ffp-contract=off), an fma exactly where the
@@ -10,10 +14,6 @@
 // (cuwave/kernels/scalar.cu:63) are computed once by `prepare_kernel`, in
 // the same operand order, so reading them is bitwise the in-place division;
 // the inverse inertia 1 / sc likewise.
-//
-// A launch's second grid axis is the shot group: a thread steps its node for
-// the `group` shots of its group, which share the material loads and, in the
-// adjoint, one gradient accumulator of the caller's (cw_scalar_grad_groups).
 #pragma once
 
 #include "common.hpp"
@@ -26,13 +26,18 @@ template <typename T> struct Points {
   const int32_t *shot; // -1 for every shot
   const int32_t *col;
   const T *weight;
-  int columns;
-  int64_t shot_stride;
+  int64_t rs, cs, ss; // row, column and shot strides of the signal / record
+  __device__ __forceinline__ int64_t at(int row, int e, int b) const {
+    return (int64_t)row * rs + (int64_t)col[e] * cs + (int64_t)b * ss;
+  }
+  __device__ __forceinline__ bool takes(int e, int64_t idx, int b) const {
+    return lin[e] == idx && (shot[e] < 0 || shot[e] == b);
+  }
 };
 
 template <typename T> struct Levels {
   T *base;
-  int64_t slot, batch;
+  int64_t slot;
   int slots, offset;
   __device__ __forceinline__ T *at(int level) const {
     int k = (level + offset) % slots;
@@ -60,8 +65,8 @@ template <typename T> struct Strip {
   int bf16;
 };
 
-// where the general kernels run: every tile, or (2D) the band of nodes
-// outside the stream box rows [y0, y1) x columns [x0, x1)
+// where the general kernels run: every interior node, or (2D) the band of
+// nodes outside the stream box rows [y0, y1) x columns [x0, x1)
 struct Region {
   int band;
   int y0, y1, x0, x1;
@@ -71,7 +76,6 @@ struct Region {
 template <typename T> struct FwdParams {
   Geom<T> g;
   Region region;
-  int group; // shots a thread steps
   const T *inv, *cell;
   Layer<T> pml;
   Levels<T> u;
@@ -80,13 +84,11 @@ template <typename T> struct FwdParams {
   Points<T> rec;
   T *record;
   Strip<T> strip;
-  int steps;
 };
 
 template <typename T> struct BwdParams {
   Geom<T> g;
   Region region;
-  int group;
   const T *inv, *cell, *stiff;
   Layer<T> pml;
   Levels<T> lam, u;
@@ -97,7 +99,6 @@ template <typename T> struct BwdParams {
   const T *signal;
   Strip<T> strip;
   T *gmass, *gstiff;
-  int64_t gbatch;
   int combine, stiff_grad, skip_layer;
   T jm, js, mf;
   int steps;
@@ -160,9 +161,9 @@ template <typename T> __device__ __forceinline__ T harmonic(T a, T b) {
 }
 
 // the forward CPML of axis d at a slab node (cuwave/kernels/scalar.cu:
-// 499-532): the three memories updated in place, their share of the field
-// returned. `an .. bh1` are the profile at the node and at the half point
-// below it.
+// 499-532): the three memories, `size` elements apart, updated in place,
+// their share of the field returned. `an .. bh1` are the profile at the node
+// and at the half point below it.
 template <typename T>
 __device__ __forceinline__ T pml_forward(T *aux, int64_t size, T Dp, T Dm,
                                          T gp, T gm, T f, T mi, T an, T bn,
@@ -210,6 +211,23 @@ __device__ __forceinline__ T minus_tap(int q, const T *op) {
   return -q <= R ? -op[-q - 1] : (T)0;
 }
 
+// the transposed layer of one axis at a node: the tap fields of the slab
+// nodes within R along the axis, `at(k)` the element of the node k away or
+// -1 off the slab, `size` the elements between memory rows
+template <typename T, int R, typename At>
+__device__ __forceinline__ T gather(const T *__restrict__ eta, int64_t size,
+                                    const T *op, At at) {
+  T sum = 0;
+#pragma unroll
+  for (int k = -R; k <= R; ++k) {
+    const int64_t i = at(k);
+    if (i >= 0)
+      sum += eta[3 * size + i] * plus_tap<T, R>(-k, op) +
+             eta[4 * size + i] * minus_tap<T, R>(-k, op);
+  }
+  return sum;
+}
+
 // scalar_sensitivity.cu:137-154 transposed_divergence_taps over taps
 // l[0 .. 2R] and cell means: hi[j - 1] the cell (a + j - 1, a + j),
 // lo[j - 1] the cell (a - j, a - j + 1)
@@ -237,6 +255,31 @@ __device__ __forceinline__ T stiffness_density(const T *u, const T *l, T lc,
   T Dp, Dm;
   cell_diffs<T, R>(u, op, Dp, Dm);
   return factor * (dgp * Dp * (l[R + 1] - lc) + dgm * Dm * (lc - l[R - 1]));
+}
+
+// the gradient of one level added into the per-node accumulators: with one
+// lane the reference's chain (the densities subtracted from the accumulator
+// itself, which `cm` and `cs` then hold), with several the shots' densities
+// summed across the lanes first; every lane of the node takes part
+template <typename T>
+__device__ __forceinline__ void add_gradient(const BwdParams<T> &p,
+                                             int64_t idx, int b, T accm,
+                                             T accs, T cm, T cs) {
+  const int lanes = p.g.lanes;
+  if (p.combine) {
+    const T x = shot_sum(p.jm * cm + p.js * cs, lanes);
+    if (b == 0)
+      p.gmass[idx] = accm + x;
+  } else if (lanes == 1) {
+    p.gmass[idx] = cm;
+    p.gstiff[idx] = cs;
+  } else {
+    const T xm = shot_sum(cm, lanes), xs = shot_sum(cs, lanes);
+    if (b == 0) {
+      p.gmass[idx] = accm + xm;
+      p.gstiff[idx] = accs + xs;
+    }
+  }
 }
 
 // ------------------------------------------------------------ node mapping
@@ -268,31 +311,31 @@ __device__ __forceinline__ bool band_node(const Region &r, const Geom<T> &g,
   return true;
 }
 
-// this thread's node: its axes a, its tile, and whether every tap of its
-// tile lies on stored nodes at distance `reach` (a tile-wide flag)
+// this thread's node, every lane of a node the same: the band's, or the
+// interior's in row-major order
 template <int NDIM, typename T>
 __device__ __forceinline__ bool general_node(const Region &r, const Geom<T> &g,
-                                             int reach, int *a, int &tile,
-                                             bool &deep) {
-  if (NDIM == 2 && r.band) {
-    const int t = blockIdx.x * kThreads + threadIdx.x;
-    if (t >= r.count || !band_node(r, g, t, a))
-      return false;
-    tile = (a[0] / TileShape<2>::TY) * g.tf + a[1] / TileShape<2>::TX;
-    deep = false;
-    return true;
+                                             int *a) {
+  const int per = kThreads >> g.lshift;
+  const int t = xcd_tile(blockIdx.x, gridDim.x) * per +
+                (int)(threadIdx.x >> g.lshift);
+  if (NDIM == 2 && r.band)
+    return t < r.count && band_node(r, g, t, a);
+  if (t >= g.interior)
+    return false;
+  if (NDIM == 1) {
+    a[0] = 1 + t;
+  } else if (NDIM == 2) {
+    const int W = g.n[1] - 2;
+    a[0] = 1 + t / W;
+    a[1] = 1 + t % W;
+  } else {
+    const int W = g.n[2] - 2, P = (g.n[1] - 2) * W;
+    a[0] = 1 + t / P;
+    a[1] = 1 + (t % P) / W;
+    a[2] = 1 + t % W;
   }
-  tile = xcd_tile(blockIdx.x, g.ntiles);
-  int lo[3], hi[3];
-  tile_node<NDIM>(g, tile, a, lo, hi);
-  bool inside = true;
-  deep = true;
-#pragma unroll
-  for (int d = 0; d < NDIM; ++d) {
-    inside &= a[d] > 0 && a[d] < g.n[d] - 1;
-    deep &= lo[d] >= reach && hi[d] <= g.n[d] - 1 - reach;
-  }
-  return inside;
+  return true;
 }
 
 // ----------------------------------------------------------- preparation
@@ -303,7 +346,7 @@ __global__ __launch_bounds__(kThreads) void prepare_kernel(
     Geom<T> g, const T *__restrict__ stiff, const T *__restrict__ minv,
     T *__restrict__ inv, T *__restrict__ cell) {
   const int64_t idx = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (idx >= g.field)
+  if (idx >= g.nodes)
     return;
   int a[3];
   int64_t rest = idx;
@@ -316,7 +359,7 @@ __global__ __launch_bounds__(kThreads) void prepare_kernel(
   inv[idx] = minv ? minv[idx] : (T)1 / sc;
 #pragma unroll
   for (int d = 0; d < NDIM; ++d)
-    cell[d * g.field + idx] =
+    cell[d * g.nodes + idx] =
         a[d] + 1 < g.np[d] ? harmonic(sc, stiff[idx + g.s[d]]) : (T)0;
 }
 
@@ -325,23 +368,26 @@ template <typename T, int NDIM, int R>
 __global__ __launch_bounds__(kThreads) void forward_kernel(const FwdParams<T> p,
                                                            const int t) {
   const Geom<T> &g = p.g;
-  int a[3], tile;
-  bool deep;
-  if (!general_node<NDIM>(p.region, g, R, a, tile, deep))
+  int a[3];
+  if (!general_node<NDIM>(p.region, g, a))
     return;
+  const int sh = g.lshift;
+  const int b = threadIdx.x & (g.lanes - 1);
   int64_t idx = 0;
-  bool held = false;
+  bool held = false, deep = true;
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
     idx += (int64_t)a[d] * g.s[d];
     held |= held_wall(a[d], g.n[d], g.dirichlet >> (2 * d));
+    deep &= a[d] >= R && a[d] <= g.n[d] - 1 - R;
   }
+  const int64_t e = (idx << sh) + b;
   const T mi = p.inv[idx];
   T gp[NDIM], gm[NDIM];
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
-    gp[d] = p.cell[d * g.field + idx];
-    gm[d] = p.cell[d * g.field + idx - g.s[d]];
+    gp[d] = p.cell[d * g.nodes + idx];
+    gm[d] = p.cell[d * g.nodes + idx - g.s[d]];
   }
   int prow[NDIM];
   bool layer = false;
@@ -352,89 +398,73 @@ __global__ __launch_bounds__(kThreads) void forward_kernel(const FwdParams<T> p,
                   : -1;
     layer |= prow[d] >= 0;
   }
-  int s0 = 0, s1 = 0, r0 = 0, r1 = 0;
-  if (p.src.starts) {
-    s0 = p.src.starts[tile];
-    s1 = p.src.starts[tile + 1];
+  const int tile = point_tile<NDIM>(g, a);
+  const T *__restrict__ u1 = p.u.at(t - 1);
+  T taps[NDIM][2 * R + 1];
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    const int64_t se = (int64_t)g.s[d] << sh;
+#pragma unroll
+    for (int j = -R; j <= R; ++j)
+      taps[d][j + R] = deep ? u1[e + j * se]
+                            : mirrored(u1, e, se, a[d], g.n[d],
+                                       g.dirichlet >> (2 * d), j);
   }
-  if (p.rec.starts) {
-    r0 = p.rec.starts[tile];
-    r1 = p.rec.starts[tile + 1];
-  }
-  int strip_col = -1;
-  if (layer && p.strip.map)
-    strip_col = p.strip.map[idx];
-  T *const u2s = p.u.at(t);
-  const T *const u1s = p.u.at(t - 1);
-  const T *const u0s = p.u.at(t - 2);
-  const int b0 = blockIdx.y * p.group;
-  const int b1 = min(b0 + p.group, g.nbatch);
-
-  for (int b = b0; b < b1; ++b) {
-    const int64_t ob = (int64_t)b * p.u.batch;
-    const T *__restrict__ u1 = u1s + ob;
-    T taps[NDIM][2 * R + 1];
+  T div[NDIM];
 #pragma unroll
-    for (int d = 0; d < NDIM; ++d)
-#pragma unroll
-      for (int j = -R; j <= R; ++j)
-        taps[d][j + R] = deep ? u1[idx + (int64_t)j * g.s[d]]
-                              : mirrored(u1, idx, g.s[d], a[d], g.n[d],
-                                         g.dirichlet >> (2 * d), j);
-    T div[NDIM];
-#pragma unroll
-    for (int d = 0; d < NDIM; ++d)
-      div[d] = flux<T, R>(taps[d], gp[d], gm[d], g.op);
-    const T lap = combine_axes<T, NDIM, R>(div, g.f);
-    const T uc = taps[0][R];
-    T v = -u0s[ob + idx] + (T)2 * uc + mi * lap;
-    if (held)
-      v = 0;
-    if (layer) {
-#pragma unroll
-      for (int d = 0; d < NDIM; ++d) {
-        if (prow[d] < 0)
-          continue;
-        const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d], ad = a[d];
-        const T *an = p.pml.profile[d], *bn = an + N, *ah = an + 2 * N,
-                *bh = an + 3 * N;
-        const int64_t size = slab_size<NDIM>(g, d, L, H);
-        T *aux = p.pml.aux[d] + (int64_t)b * 3 * size +
-                 slab_index<NDIM>(g, a, d, prow[d], L, H);
-        T Dp, Dm;
-        cell_diffs<T, R>(taps[d], g.op, Dp, Dm);
-        const T share = pml_forward(aux, size, Dp, Dm, gp[d], gm[d], g.f[d], mi,
-                                    an[ad], bn[ad], ah[ad], bh[ad], ah[ad - 1],
-                                    bh[ad - 1]);
-        if (!held)
-          v += share;
-      }
-    }
-    for (int e = s0; e < s1; ++e)
-      if (p.src.lin[e] == idx && (p.src.shot[e] < 0 || p.src.shot[e] == b))
-        v += p.src.weight[e] *
-             p.signal[b * p.src.shot_stride + (int64_t)t * p.src.columns +
-                      p.src.col[e]];
-    T *__restrict__ u2 = u2s + ob;
-    u2[idx] = v;
-    // the ghost refill folded in: a mirror node writes its face ghost
+  for (int d = 0; d < NDIM; ++d)
+    div[d] = flux<T, R>(taps[d], gp[d], gm[d], g.op);
+  const T lap = combine_axes<T, NDIM, R>(div, g.f);
+  const T uc = taps[0][R];
+  T v = -p.u.at(t - 2)[e] + (T)2 * uc + mi * lap;
+  if (held)
+    v = 0;
+  if (layer) {
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) {
-      const int faces = (g.neumann | g.dirichlet) >> (2 * d);
-      const int odd = g.dirichlet >> (2 * d);
-      if ((faces & 1) && a[d] == 2)
-        u2[idx - 2 * (int64_t)g.s[d]] = (odd & 1) ? -v : v;
-      if ((faces & 2) && a[d] == g.n[d] - 3)
-        u2[idx + 2 * (int64_t)g.s[d]] = (odd & 2) ? -v : v;
+      if (prow[d] < 0)
+        continue;
+      const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d], ad = a[d];
+      const T *an = p.pml.profile[d], *bn = an + N, *ah = an + 2 * N,
+              *bh = an + 3 * N;
+      const int64_t size = slab_size<NDIM>(g, d, L, H) << sh;
+      T *aux = p.pml.aux[d] +
+               (slab_index<NDIM>(g, a, d, prow[d], L, H) << sh) + b;
+      T Dp, Dm;
+      cell_diffs<T, R>(taps[d], g.op, Dp, Dm);
+      const T share =
+          pml_forward(aux, size, Dp, Dm, gp[d], gm[d], g.f[d], mi, an[ad],
+                      bn[ad], ah[ad], bh[ad], ah[ad - 1], bh[ad - 1]);
+      if (!held)
+        v += share;
     }
-    for (int e = r0; e < r1; ++e)
-      if (p.rec.lin[e] == idx)
-        p.record[(int64_t)b * p.steps * p.rec.columns +
-                 (int64_t)t * p.rec.columns + p.rec.col[e]] = v;
-    if (strip_col >= 0) {
+  }
+  if (p.src.starts)
+    for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
+      if (p.src.takes(k, idx, b))
+        v += p.src.weight[k] * p.signal[p.src.at(t, k, b)];
+  T *__restrict__ u2 = p.u.at(t);
+  u2[e] = v;
+  // the ghost refill folded in: a mirror node writes its face ghost
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    const int faces = (g.neumann | g.dirichlet) >> (2 * d);
+    const int odd = g.dirichlet >> (2 * d);
+    const int64_t se = (int64_t)g.s[d] << sh;
+    if ((faces & 1) && a[d] == 2)
+      u2[e - 2 * se] = (odd & 1) ? -v : v;
+    if ((faces & 2) && a[d] == g.n[d] - 3)
+      u2[e + 2 * se] = (odd & 2) ? -v : v;
+  }
+  if (p.rec.starts)
+    for (int k = p.rec.starts[tile]; k < p.rec.starts[tile + 1]; ++k)
+      if (p.rec.takes(k, idx, b))
+        p.record[p.rec.at(t, k, b)] = v;
+  if (layer && p.strip.map) {
+    const int col = p.strip.map[idx];
+    if (col >= 0) {
       const int64_t at =
-          ((int64_t)b * p.strip.rows + (t - p.strip.first)) * p.strip.count +
-          strip_col;
+          (((int64_t)(t - p.strip.first) * p.strip.count + col) << sh) + b;
       if (p.strip.bf16)
         ((uint16_t *)p.strip.record)[at] = bf16_bits((float)v);
       else
@@ -452,10 +482,11 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
     const BwdParams<T> p, const int n) {
   constexpr int REACH = R + 2; // scalar_sensitivity.cu:44 WALL_REACH
   const Geom<T> &g = p.g;
-  int a[3], tile;
-  bool deep;
-  if (!general_node<NDIM>(p.region, g, REACH, a, tile, deep))
+  int a[3];
+  if (!general_node<NDIM>(p.region, g, a))
     return;
+  const int sh = g.lshift;
+  const int b = threadIdx.x & (g.lanes - 1);
   int64_t idx = 0;
   bool held = false;
   T w = 1; // the cell weight W of the node
@@ -465,10 +496,11 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
     held |= held_wall(a[d], g.n[d], g.dirichlet >> (2 * d));
     w *= (a[d] == 1 || a[d] == g.n[d] - 2) ? (T)0.5 : (T)1;
   }
+  const int64_t e = (idx << sh) + b;
   bool near[NDIM];
 #pragma unroll
   for (int d = 0; d < NDIM; ++d)
-    near[d] = !deep && (a[d] < REACH || a[d] > g.n[d] - 1 - REACH);
+    near[d] = a[d] < REACH || a[d] > g.n[d] - 1 - REACH;
   const T mi = p.inv[idx];
   const T sc = p.stiff[idx];
   T sp[NDIM], sm[NDIM];
@@ -495,257 +527,216 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(
   }
   const bool grad = !(p.skip_layer && layer);
   const int m = p.steps - 1 - n;
-  int r0 = 0, r1 = 0, s0 = 0, s1 = 0;
-  if (p.rec.starts) {
-    r0 = p.rec.starts[tile];
-    r1 = p.rec.starts[tile + 1];
-  }
+  const int tile = point_tile<NDIM>(g, a);
   const bool reverse = p.reverse && n <= p.steps - 2;
-  if (reverse && p.src.starts) {
-    s0 = p.src.starts[tile];
-    s1 = p.src.starts[tile + 1];
-  }
   int strip_col = -1;
   if (reverse && layer && p.strip.map)
     strip_col = p.strip.map[idx];
-  T *const l0s = p.lam.at(n);
-  const T *const l1s = p.lam.at(n + 1);
-  const T *const u1s = p.u.at(n);
-  T *const uos = p.u.at(n + 1);
-  const int gi = blockIdx.y;
-  const int b0 = gi * p.group;
-  const int b1 = min(b0 + p.group, g.nbatch);
-  const bool shared = p.gbatch == 0;
+  T *__restrict__ l0 = p.lam.at(n);
+  const T *__restrict__ l1 = p.lam.at(n + 1);
+  const T *__restrict__ u1 = p.u.at(n);
+  T *__restrict__ uo = p.u.at(n + 1);
+  const bool chain = g.lanes == 1 && !p.combine;
   T accm = 0, accs = 0;
-  if (grad && shared) {
-    accm = p.gmass[(int64_t)gi * g.field + idx];
+  if (grad) {
+    accm = p.gmass[idx];
     if (!p.combine)
-      accs = p.gstiff[(int64_t)gi * g.field + idx];
+      accs = p.gstiff[idx];
   }
 
-  for (int b = b0; b < b1; ++b) {
-    const T *__restrict__ l1 = l1s + (int64_t)b * p.lam.batch;
-    T *__restrict__ l0 = l0s + (int64_t)b * p.lam.batch;
-    const T *__restrict__ u1 = u1s + (int64_t)b * p.u.batch;
-    const T lc = l1[idx], lo_ = l0[idx], uc = u1[idx];
-    const T lw = held ? (T)0 : lc;
-    T div_l = 0, gsum = 0;
+  const T lc = l1[e], lo_ = l0[e], uc = u1[e];
+  const T lw = held ? (T)0 : lc;
+  T div_l = 0, gsum = 0;
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    const int odd = g.dirichlet >> (2 * d);
+    const int64_t se = (int64_t)g.s[d] << sh;
+    T lt[2 * R + 1];
+    T ch[R], cl[R];
+    if (near[d]) {
+#pragma unroll
+      for (int j = -R; j <= R; ++j)
+        lt[j + R] = reflected(l1, e, se, a[d], g.n[d], odd, j);
+      lt[R] = lc;
+      T ks[2 * R + 1];
+#pragma unroll
+      for (int j = -R; j <= R; ++j)
+        ks[j + R] = reflected(p.stiff, idx, g.s[d], a[d], g.n[d], 0, j);
+      ks[R] = sc;
+#pragma unroll
+      for (int j = 1; j <= R; ++j) {
+        ch[j - 1] = harmonic(ks[R + j - 1], ks[R + j]);
+        cl[j - 1] = harmonic(ks[R - j + 1], ks[R - j]);
+      }
+    } else {
+#pragma unroll
+      for (int j = -R; j <= R; ++j)
+        lt[j + R] = l1[e + j * se];
+#pragma unroll
+      for (int j = 1; j <= R; ++j) {
+        ch[j - 1] = p.cell[d * g.nodes + idx + (int64_t)(j - 1) * g.s[d]];
+        cl[j - 1] = p.cell[d * g.nodes + idx - (int64_t)j * g.s[d]];
+      }
+    }
+    div_l += g.f[d] * transposed<T, R>(lt, ch, cl, g.op);
+    if (p.stiff_grad && grad) {
+      T ut[2 * R + 1];
+#pragma unroll
+      for (int j = -R; j <= R; ++j)
+        ut[j + R] = near[d] ? reflected(u1, e, se, a[d], g.n[d], odd, j)
+                            : u1[e + j * se];
+      ut[R] = uc;
+      if (near[d])
+        lt[R] = lw;
+      gsum += stiffness_density<T, R>(ut, lt, near[d] ? lw : lc, sc, sp[d],
+                                      sm[d], g.f[d], g.op);
+    }
+  }
+  T ln = -lo_ + (T)2 * lc + mi * div_l;
+  if (held)
+    ln = 0;
+  // the inertia density by parts in time (scalar_sensitivity.cu:512-514)
+  T cm = chain ? accm : (T)0;
+  cm = cm - p.mf * uc * (ln - (T)2 * lc + lo_);
+  T cs = chain ? accs : (T)0;
+  cs = cs - p.mf * gsum;
+  if (reach) {
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) {
-      const int odd = g.dirichlet >> (2 * d);
-      T lt[2 * R + 1];
-      T ch[R], cl[R];
-      if (near[d]) {
+      if (!ext[d])
+        continue;
+      const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d];
+      const int64_t size = slab_size<NDIM>(g, d, L, H) << sh;
+      const T *eta = p.pml.eta[d][n & 1] + b;
+      int at[3];
 #pragma unroll
-        for (int j = -R; j <= R; ++j)
-          lt[j + R] = reflected(l1, idx, g.s[d], a[d], g.n[d], odd, j);
-        lt[R] = lc;
-        T ks[2 * R + 1];
+      for (int k = 0; k < NDIM; ++k)
+        at[k] = a[k];
+      T sum = gather<T, R>(eta, size, g.op, [&](int k) {
+        const int row = slab_row(a[d] + k, N, L, H);
+        if (row < 0)
+          return (int64_t)-1;
+        at[d] = a[d] + k;
+        return slab_index<NDIM>(g, at, d, row, L, H) << sh;
+      });
 #pragma unroll
-        for (int j = -R; j <= R; ++j)
-          ks[j + R] = reflected(p.stiff, idx, g.s[d], a[d], g.n[d], 0, j);
-        ks[R] = sc;
-#pragma unroll
-        for (int j = 1; j <= R; ++j) {
-          ch[j - 1] = harmonic(ks[R + j - 1], ks[R + j]);
-          cl[j - 1] = harmonic(ks[R - j + 1], ks[R - j]);
-        }
-      } else {
-#pragma unroll
-        for (int j = -R; j <= R; ++j)
-          lt[j + R] = l1[idx + (int64_t)j * g.s[d]];
-#pragma unroll
-        for (int j = 1; j <= R; ++j) {
-          ch[j - 1] = p.cell[d * g.field + idx + (int64_t)(j - 1) * g.s[d]];
-          cl[j - 1] = p.cell[d * g.field + idx - (int64_t)j * g.s[d]];
-        }
-      }
-      div_l += g.f[d] * transposed<T, R>(lt, ch, cl, g.op);
-      if (p.stiff_grad && grad) {
-        T ut[2 * R + 1];
-#pragma unroll
-        for (int j = -R; j <= R; ++j)
-          ut[j + R] = near[d] ? reflected(u1, idx, g.s[d], a[d], g.n[d], odd, j)
-                              : u1[idx + (int64_t)j * g.s[d]];
-        ut[R] = uc;
-        if (near[d])
-          lt[R] = lw;
-        gsum += stiffness_density<T, R>(ut, lt, near[d] ? lw : lc, sc, sp[d],
-                                        sm[d], g.f[d], g.op);
-      }
-    }
-    T ln = -lo_ + (T)2 * lc + mi * div_l;
-    if (held)
-      ln = 0;
-    if (grad && !shared) {
-      accm = p.gmass[(int64_t)b * p.gbatch + idx];
-      if (!p.combine)
-        accs = p.gstiff[(int64_t)b * p.gbatch + idx];
-    }
-    // the inertia density by parts in time (scalar_sensitivity.cu:512-514)
-    T cm = p.combine ? (T)0 : accm;
-    cm = cm - p.mf * uc * (ln - (T)2 * lc + lo_);
-    T cs = p.combine ? (T)0 : accs;
-    cs = cs - p.mf * gsum;
-    if (reach) {
-#pragma unroll
-      for (int d = 0; d < NDIM; ++d) {
-        if (!ext[d])
+      for (int high = 0; high < 2; ++high) {
+        // the reads past the wall that land on a, folded back
+        const int ghost = high ? 2 * (N - 2) - a[d] : 2 - a[d];
+        if (high ? ghost < N - 1 : ghost > 0)
           continue;
-        const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d];
-        const int64_t size = slab_size<NDIM>(g, d, L, H);
-        const T *eta = p.pml.eta[d][n & 1] + (int64_t)b * kEta * size;
-        T sum = 0;
-        int at[3];
+        T fold = 0;
 #pragma unroll
-        for (int e = 0; e < NDIM; ++e)
-          at[e] = a[e];
-#pragma unroll
-        for (int k = -R; k <= R; ++k) {
-          const int row = slab_row(a[d] + k, N, L, H);
-          if (row >= 0) {
-            at[d] = a[d] + k;
-            const int64_t i = slab_index<NDIM>(g, at, d, row, L, H);
-            sum += eta[3 * size + i] * plus_tap<T, R>(-k, g.op) +
-                   eta[4 * size + i] * minus_tap<T, R>(-k, g.op);
+        for (int k = 1; k <= R; ++k) {
+          const int ai = high ? N - 1 - k : k;
+          const int row = slab_row(ai, N, L, H);
+          if (row >= 0 && ghost - ai >= -R && ghost - ai <= R) {
+            at[d] = ai;
+            const int64_t i = slab_index<NDIM>(g, at, d, row, L, H) << sh;
+            fold += eta[3 * size + i] * plus_tap<T, R>(ghost - ai, g.op) +
+                    eta[4 * size + i] * minus_tap<T, R>(ghost - ai, g.op);
           }
         }
-#pragma unroll
-        for (int high = 0; high < 2; ++high) {
-          // the reads past the wall that land on a, folded back
-          const int ghost = high ? 2 * (N - 2) - a[d] : 2 - a[d];
-          if (high ? ghost < N - 1 : ghost > 0)
-            continue;
-          T fold = 0;
-#pragma unroll
-          for (int k = 1; k <= R; ++k) {
-            const int ai = high ? N - 1 - k : k;
-            const int row = slab_row(ai, N, L, H);
-            if (row >= 0 && ghost - ai >= -R && ghost - ai <= R) {
-              at[d] = ai;
-              const int64_t i = slab_index<NDIM>(g, at, d, row, L, H);
-              fold += eta[3 * size + i] * plus_tap<T, R>(ghost - ai, g.op) +
-                      eta[4 * size + i] * minus_tap<T, R>(ghost - ai, g.op);
-            }
-          }
-          sum += ((g.dirichlet >> (2 * d + high)) & 1) ? -fold : fold;
-        }
-        const T delta = mi * sum / w;
-        ln += delta;
-        cm -= p.mf * uc * delta;
+        sum += ((g.dirichlet >> (2 * d + high)) & 1) ? -fold : fold;
       }
+      const T delta = mi * sum / w;
+      ln += delta;
+      cm -= p.mf * uc * delta;
     }
-    for (int e = r0; e < r1; ++e)
-      if (p.rec.lin[e] == idx) {
-        const T load =
-            p.rec.weight[e] *
-            p.adjoint[(int64_t)b * p.rec.shot_stride +
-                      (int64_t)m * p.rec.columns + p.rec.col[e]];
+  }
+  if (p.rec.starts)
+    for (int k = p.rec.starts[tile]; k < p.rec.starts[tile + 1]; ++k)
+      if (p.rec.takes(k, idx, b)) {
+        const T load = p.rec.weight[k] * p.adjoint[p.rec.at(m, k, b)];
         ln += load;
         cm += -p.mf * uc * load;
       }
-    l0[idx] = ln;
-    // the memory the next level gathers, from this level's final lambda
-    if (layer) {
+  l0[e] = ln;
+  // the memory the next level gathers, from this level's final lambda
+  if (layer) {
 #pragma unroll
-      for (int d = 0; d < NDIM; ++d) {
-        if (prow[d] < 0)
-          continue;
-        const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d], ad = a[d];
-        const int64_t size = slab_size<NDIM>(g, d, L, H);
-        const int64_t i = slab_index<NDIM>(g, a, d, prow[d], L, H);
-        const T *an = p.pml.profile[d], *bn = an + N, *ah = an + 2 * N,
-                *bh = an + 3 * N;
-        pml_adjoint_update(p.pml.eta[d][n & 1] + (int64_t)b * kEta * size + i,
-                           p.pml.eta[d][(n + 1) & 1] +
-                               (int64_t)b * kEta * size + i,
-                           size, held ? (T)0 : w * ln,
-                           p.cell[d * g.field + idx],
-                           p.cell[d * g.field + idx - g.s[d]], g.f[d], an[ad],
-                           bn[ad], ah[ad], bh[ad], ah[ad - 1], bh[ad - 1]);
-      }
+    for (int d = 0; d < NDIM; ++d) {
+      if (prow[d] < 0)
+        continue;
+      const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d], ad = a[d];
+      const int64_t size = slab_size<NDIM>(g, d, L, H) << sh;
+      const int64_t i = (slab_index<NDIM>(g, a, d, prow[d], L, H) << sh) + b;
+      const T *an = p.pml.profile[d], *bn = an + N, *ah = an + 2 * N,
+              *bh = an + 3 * N;
+      pml_adjoint_update(p.pml.eta[d][n & 1] + i, p.pml.eta[d][(n + 1) & 1] + i,
+                         size, held ? (T)0 : w * ln,
+                         p.cell[d * g.nodes + idx],
+                         p.cell[d * g.nodes + idx - g.s[d]], g.f[d], an[ad],
+                         bn[ad], ah[ad], bh[ad], ah[ad - 1], bh[ad - 1]);
     }
-    if (grad) {
-      if (p.combine) {
-        accm = accm + (p.jm * cm + p.js * cs);
-      } else {
-        accm = cm;
-        accs = cs;
-      }
-      if (!shared) {
-        p.gmass[(int64_t)b * p.gbatch + idx] = accm;
-        if (!p.combine)
-          p.gstiff[(int64_t)b * p.gbatch + idx] = accs;
-      }
-    }
-    if (reverse) {
-      // the lossless step run backwards: u^(n - 1) from u^n and u^(n + 1)
-      T taps[NDIM][2 * R + 1];
+  }
+  if (grad)
+    add_gradient(p, idx, b, accm, accs, cm, cs);
+  if (reverse) {
+    // the lossless step run backwards: u^(n - 1) from u^n and u^(n + 1)
+    T v;
+    if (strip_col >= 0) {
+      // a layer node is replayed from the forward sweep's record
+      const int64_t at =
+          (((int64_t)(n - 1 - p.strip.first) * p.strip.count + strip_col)
+           << sh) +
+          b;
+      v = p.strip.bf16 ? (T)bf16_value(((const uint16_t *)p.strip.record)[at])
+                       : ((const T *)p.strip.record)[at];
+    } else {
+      bool deep = true;
 #pragma unroll
       for (int d = 0; d < NDIM; ++d)
+        deep &= a[d] >= R && a[d] <= g.n[d] - 1 - R;
+      T taps[NDIM][2 * R + 1];
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d) {
+        const int64_t se = (int64_t)g.s[d] << sh;
 #pragma unroll
         for (int j = -R; j <= R; ++j)
-          taps[d][j + R] = deep ? u1[idx + (int64_t)j * g.s[d]]
-                                : mirrored(u1, idx, g.s[d], a[d], g.n[d],
+          taps[d][j + R] = deep ? u1[e + j * se]
+                                : mirrored(u1, e, se, a[d], g.n[d],
                                            g.dirichlet >> (2 * d), j);
+      }
       T div[NDIM];
 #pragma unroll
       for (int d = 0; d < NDIM; ++d)
-        div[d] = flux<T, R>(taps[d], p.cell[d * g.field + idx],
-                            p.cell[d * g.field + idx - g.s[d]], g.op);
+        div[d] = flux<T, R>(taps[d], p.cell[d * g.nodes + idx],
+                            p.cell[d * g.nodes + idx - g.s[d]], g.op);
       const T lap = combine_axes<T, NDIM, R>(div, g.f);
-      T *__restrict__ uo = uos + (int64_t)b * p.u.batch;
-      T v = -uo[idx] + (T)2 * uc + mi * lap;
+      v = -uo[e] + (T)2 * uc + mi * lap;
       if (held)
         v = 0;
-      for (int e = s0; e < s1; ++e)
-        if (p.src.lin[e] == idx && (p.src.shot[e] < 0 || p.src.shot[e] == b))
-          v += p.src.weight[e] *
-               p.signal[b * p.src.shot_stride +
-                        (int64_t)(n + 1) * p.src.columns + p.src.col[e]];
-      if (strip_col >= 0) {
-        const int64_t at = ((int64_t)b * p.strip.rows + (n - 1 - p.strip.first)) *
-                               p.strip.count +
-                           strip_col;
-        v = p.strip.bf16
-                ? (T)bf16_value(((const uint16_t *)p.strip.record)[at])
-                : ((const T *)p.strip.record)[at];
-      }
-      uo[idx] = v;
-#pragma unroll
-      for (int d = 0; d < NDIM; ++d) {
-        const int faces = (g.neumann | g.dirichlet) >> (2 * d);
-        const int odd = g.dirichlet >> (2 * d);
-        if ((faces & 1) && a[d] == 2)
-          uo[idx - 2 * (int64_t)g.s[d]] = (odd & 1) ? -v : v;
-        if ((faces & 2) && a[d] == g.n[d] - 3)
-          uo[idx + 2 * (int64_t)g.s[d]] = (odd & 2) ? -v : v;
-      }
+      if (p.src.starts)
+        for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
+          if (p.src.takes(k, idx, b))
+            v += p.src.weight[k] * p.signal[p.src.at(n + 1, k, b)];
     }
-  }
-  if (grad && shared) {
-    p.gmass[(int64_t)gi * g.field + idx] = accm;
-    if (!p.combine)
-      p.gstiff[(int64_t)gi * g.field + idx] = accs;
+    uo[e] = v;
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+      const int faces = (g.neumann | g.dirichlet) >> (2 * d);
+      const int odd = g.dirichlet >> (2 * d);
+      const int64_t se = (int64_t)g.s[d] << sh;
+      if ((faces & 1) && a[d] == 2)
+        uo[e - 2 * se] = (odd & 1) ? -v : v;
+      if ((faces & 2) && a[d] == g.n[d] - 3)
+        uo[e + 2 * se] = (odd & 2) ? -v : v;
+    }
   }
 }
 
 // --------------------------------------------------- superposition step
-// scalar_sensitivity.cu:599-668 (step = 1) and :352-385 (step = 0)
+// scalar_sensitivity.cu:599-668 (step = 1) and :352-385 (step = 0), one shot
 template <typename T, int NDIM, int R>
 __global__ __launch_bounds__(kThreads) void superposed_kernel(
     Geom<T> g, const T *__restrict__ inv, const T *__restrict__ cell,
     const T *__restrict__ u0, const T *__restrict__ u1, T *__restrict__ u2,
     T *__restrict__ acc_mass, T *__restrict__ acc_stiff, T ft, T fs,
     int step) {
-  const int tile = xcd_tile(blockIdx.x, g.ntiles);
-  int a[3], lo[3], hi[3];
-  tile_node<NDIM>(g, tile, a, lo, hi);
-  bool inside = true;
-#pragma unroll
-  for (int d = 0; d < NDIM; ++d)
-    inside &= a[d] > 0 && a[d] < g.n[d] - 1;
-  if (!inside)
+  const Region full{};
+  int a[3];
+  if (!general_node<NDIM>(full, g, a))
     return;
   int64_t idx = 0;
   bool held = false;
@@ -777,8 +768,8 @@ __global__ __launch_bounds__(kThreads) void superposed_kernel(
     for (int j = -R; j <= R; ++j)
       t[j + R] = mirrored(u1, idx, g.s[d], a[d], g.n[d],
                           g.dirichlet >> (2 * d), j);
-    div[d] = flux<T, R>(t, cell[d * g.field + idx],
-                        cell[d * g.field + idx - g.s[d]], g.op);
+    div[d] = flux<T, R>(t, cell[d * g.nodes + idx],
+                        cell[d * g.nodes + idx - g.s[d]], g.op);
   }
   const T lap = combine_axes<T, NDIM, R>(div, g.f);
   const T uc = u1[idx];

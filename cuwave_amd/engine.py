@@ -28,6 +28,7 @@ class PointList:
 
     points: _abi.Points
     keep: tuple  # the tensors the struct points into
+    count: int  # columns of the signal / record table
 
 
 class Propagator:
@@ -43,6 +44,7 @@ class Propagator:
         self.device = _abi.require_device()
         self.sim, self.nbatch = sim, int(nbatch)
         self.grid = sim.abi_grid(self.nbatch)
+        self.lanes = self.grid.lanes
         dtype = sim.dtype
         self.stiff = mat["stiff"]
         self.minv = mat.get("minv")
@@ -72,58 +74,87 @@ class Propagator:
                 self.slabs.append((d, (lo + hi) * plane))
 
     # ------------------------------------------------------------ memory
+    # Per-shot arrays end in the lanes axis: one node's shots side by side.
+    def _zeros(self, *shape) -> torch.Tensor:
+        return torch.zeros((*shape, self.lanes), dtype=self.sim.dtype, device=self.device)
+
+    def levels_memory(self, slots: int) -> torch.Tensor:
+        """Zeroed field levels: (slots, *Nx_padded, lanes)."""
+        return self._zeros(slots, *self.sim.Nx_padded)
+
     def aux_memory(self) -> list[torch.Tensor]:
-        """Zeroed forward layer memory, (nbatch, 3, slab) per lined axis."""
-        return [torch.zeros((self.nbatch, 3, size), dtype=self.sim.dtype, device=self.device)
-                for _, size in self.slabs]
+        """Zeroed forward layer memory, (3, slab, lanes) per lined axis."""
+        return [self._zeros(3, size) for _, size in self.slabs]
 
     def eta_memory(self) -> list[torch.Tensor]:
         """Zeroed adjoint layer memory, two alternating buffers per lined axis.
 
-        A buffer holds (nbatch, 5, slab): the recursion's three variables and
+        A buffer holds (5, slab, lanes): the recursion's three variables and
         the two tap fields the neighbours' transposed taps read.
         """
-        return [torch.zeros((2, self.nbatch, 5, size), dtype=self.sim.dtype, device=self.device)
-                for _, size in self.slabs]
+        return [self._zeros(2, 5, size) for _, size in self.slabs]
+
+    def record_memory(self, sensors: PointList) -> torch.Tensor:
+        """Zeroed sensor record: (N, sensors, lanes)."""
+        return self._zeros(self.sim.N, sensors.count)
 
     def accumulator(self) -> torch.Tensor:
-        """Zeroed shot-summed gradient accumulator: (groups, *Nx_padded), summed by `reduce`.
+        """Zeroed gradient accumulator over the padded grid; the kernels sum the shots into it."""
+        return torch.zeros(self.sim.Nx_padded, dtype=self.sim.dtype, device=self.device)
 
-        The adjoint kernels step a group of shots together and add them into
-        that group's field, so no two threads write one node (cw_scalar_grad_groups).
-        """
-        groups = _abi.I32()
-        _abi.call("cw_scalar_grad_groups", ctypes.byref(self.grid), ctypes.byref(self.pml),
-                  ctypes.byref(groups))
-        return torch.zeros((groups.value, *self.sim.Nx_padded), dtype=self.sim.dtype,
-                           device=self.device)
+    def shots(self, t: torch.Tensor) -> torch.Tensor:
+        """The (nbatch, ...) view of a per-shot array whose last axis is the lanes axis."""
+        return torch.movedim(t, -1, 0)[: self.nbatch]
 
-    @staticmethod
-    def reduce(acc: torch.Tensor) -> torch.Tensor:
-        """The gradient an accumulator holds: the sum of its groups."""
-        return acc[0].clone() if acc.shape[0] == 1 else acc.sum(0)
+    def lane_table(self, per_shot: torch.Tensor) -> torch.Tensor:
+        """A (nbatch, ...) per-shot table as the kernels read it: (..., lanes), padding zero."""
+        out = torch.zeros((*per_shot.shape[1:], self.lanes), dtype=self.sim.dtype,
+                          device=self.device)
+        out[..., : self.nbatch] = torch.movedim(per_shot.to(self.device, self.sim.dtype), 0, -1)
+        return out
 
     def _layer(self, aux=None, eta=None) -> _abi.Pml:
         layer = _abi.Pml()
         ctypes.memmove(ctypes.byref(layer), ctypes.byref(self.pml), ctypes.sizeof(layer))
-        for k, (d, _) in enumerate(self.slabs):
+        for k, (d, size) in enumerate(self.slabs):
             if aux is not None:
+                self._require_shape("pml memory", aux[k], (3, size, self.lanes))
                 layer.aux[d] = _abi.ptr(aux[k])
             if eta is not None:
+                self._require_shape("adjoint pml memory", eta[k], (2, 5, size, self.lanes))
                 layer.eta[d][0] = _abi.ptr(eta[k][0])
                 layer.eta[d][1] = _abi.ptr(eta[k][1])
         return layer
 
-    @staticmethod
-    def levels(tensor: torch.Tensor, offset: int = 0) -> _abi.Levels:
-        """Levels over a (slots, nbatch, *field) tensor, level k in slot (k + offset) % slots."""
-        slots, nbatch = tensor.shape[:2]
-        field = int(np.prod(tensor.shape[2:]))
-        return _abi.Levels(_abi.ptr(tensor), nbatch * field, field, slots, offset)
+    def _require_shape(self, what: str, tensor: torch.Tensor | None, shape: tuple) -> None:
+        # the kernels index raw pointers: a wrong shape would read past a buffer
+        if tensor is not None and tuple(tensor.shape) != tuple(shape):
+            raise ValueError(f"{what}: shape {tuple(tensor.shape)}, the kernels need {tuple(shape)}")
+        if tensor is not None and tensor.dtype not in (self.sim.dtype, torch.int16):
+            raise ValueError(f"{what}: dtype {tensor.dtype}, the kernels need {self.sim.dtype}")
+
+    def levels(self, tensor: torch.Tensor, offset: int = 0) -> _abi.Levels:
+        """Levels over a (slots, *Nx_padded, lanes) tensor, level k in slot (k + offset) % slots."""
+        slots = tensor.shape[0]
+        self._require_shape("levels", tensor, (slots, *self.sim.Nx_padded, self.lanes))
+        return _abi.Levels(_abi.ptr(tensor), int(np.prod(tensor.shape[1:])), slots, offset)
+
+    def _strip(self, strip: dict) -> _abi.Strip:
+        self._require_shape("strip record", strip["record"],
+                            (strip["rows"], strip["count"], self.lanes))
+        if tuple(strip["map"].shape) != tuple(self.sim.Nx_padded):
+            raise ValueError("strip map: one entry per node of the padded grid")
+        return _abi.Strip(_abi.ptr(strip["map"]), _abi.ptr(strip["record"]), strip["count"],
+                          strip["first"], strip["rows"], int(strip.get("bf16", 0)))
 
     # ------------------------------------------------------------ points
-    def point_list(self, lin, shot, col, columns: int, weight=None, shot_stride: int = 0) -> PointList:
-        """Bucket points by the kernels' tiles and upload them (cw_point_lists)."""
+    def point_list(self, lin, shot, col, columns: int, strides: tuple[int, int, int],
+                   weight=None) -> PointList:
+        """Bucket points by the kernels' tiles and upload them (cw_point_lists).
+
+        `strides` are the (row, column, shot) element strides of the signal or
+        record table the points read or write.
+        """
         lin = np.ascontiguousarray(lin, dtype=np.int32)
         count = len(lin)
         starts = np.zeros(self.ntiles + 1, dtype=np.int32)
@@ -142,12 +173,12 @@ class Propagator:
         t_col = upload(np.asarray(col, dtype=np.int32)[order], torch.int32)
         keep = [t_starts, t_lin, t_shot, t_col]
         pts = _abi.Points(_abi.ptr(t_starts), _abi.ptr(t_lin), _abi.ptr(t_shot), _abi.ptr(t_col),
-                          None, int(columns), int(shot_stride))
+                          None, *map(int, strides))
         if weight is not None:
             t_w = upload(np.asarray(weight, dtype=self.sim.np_dtype)[order], self.sim.dtype)
             keep.append(t_w)
             pts.weight = _abi.ptr(t_w)
-        return PointList(pts, tuple(keep))
+        return PointList(pts, tuple(keep), int(columns))
 
     # ------------------------------------------------------------ launches
     def _timed(self, kind: str, launches: int):
@@ -180,14 +211,15 @@ class Propagator:
         a.pml = self._layer(aux=aux)
         a.u = self.levels(u, offset)
         if sources is not None:
+            self._require_shape("source signal", signal, (self.sim.N, sources.count))
             a.sources = sources.points
             a.signal = _abi.ptr(signal)
         if sensors is not None:
+            self._require_shape("sensor record", record, (self.sim.N, sensors.count, self.lanes))
             a.sensors = sensors.points
             a.record = _abi.ptr(record)
         if strip is not None:
-            a.strip = _abi.Strip(_abi.ptr(strip["map"]), _abi.ptr(strip["record"]), strip["count"],
-                                 strip["first"], strip["rows"], int(strip.get("bf16", 0)))
+            a.strip = self._strip(strip)
         a.steps = self.sim.N
         with self._timed("forward", t1 - t0):
             _abi.call("cw_scalar_forward", ctypes.byref(a), int(t0), int(t1), _abi.stream())
@@ -197,8 +229,7 @@ class Propagator:
                  reverse: bool = False, sensors: PointList | None = None,
                  adjoint: torch.Tensor | None = None, sources: PointList | None = None,
                  signal: torch.Tensor | None = None, strip: dict | None = None,
-                 per_shot: bool = False, combine: tuple[float, float] | None = None,
-                 skip_layer: bool = False) -> None:
+                 combine: tuple[float, float] | None = None, skip_layer: bool = False) -> None:
         """Adjoint levels n1 - 1 down to n0 (cw_scalar_backward)."""
         a = _abi.BackwardArgs()
         a.grid = self.grid
@@ -208,17 +239,19 @@ class Propagator:
         a.u = self.levels(u, u_offset)
         a.reverse = int(reverse)
         if sensors is not None:
+            self._require_shape("adjoint signal", adjoint, (self.sim.N, sensors.count, self.lanes))
             a.sensors = sensors.points
             a.adjoint_signal = _abi.ptr(adjoint)
         if sources is not None:
+            self._require_shape("source signal", signal, (self.sim.N, sources.count))
             a.sources = sources.points
             a.signal = _abi.ptr(signal)
         if strip is not None:
-            a.strip = _abi.Strip(_abi.ptr(strip["map"]), _abi.ptr(strip["record"]), strip["count"],
-                                 strip["first"], strip["rows"], int(strip.get("bf16", 0)))
+            a.strip = self._strip(strip)
+        for what, acc in (("mass gradient", g_mass), ("stiffness gradient", g_stiff)):
+            self._require_shape(what, acc, tuple(self.sim.Nx_padded))
         a.g_mass = _abi.ptr(g_mass)
         a.g_stiff = _abi.ptr(g_stiff)
-        a.grad_batch_stride = self.field if per_shot else 0
         if combine is not None:
             a.combine = 1
             a.jac_mass, a.jac_stiff = float(combine[0]), float(combine[1])
@@ -230,11 +263,14 @@ class Propagator:
 
     def superposed(self, u0, u1, u2, acc_mass, acc_stiff, ft: float, fs: float, step: bool) -> None:
         """One superposition step, or the Frechet densities of a triplet (cw_scalar_superposed)."""
-        if self.nbatch != 1:
+        if self.lanes != 1:
             raise ValueError("the superposition step takes one shot")
         a = _abi.SuperposedArgs()
         a.grid = self.grid
         a.mat = self.mat
+        for what, t in (("u0", u0), ("u1", u1), ("u2", u2), ("mass density", acc_mass),
+                        ("stiffness density", acc_stiff)):
+            self._require_shape(what, t, tuple(self.sim.Nx_padded))
         a.u0, a.u1, a.u2 = _abi.ptr(u0), _abi.ptr(u1), _abi.ptr(u2)
         a.acc_mass, a.acc_stiff = _abi.ptr(acc_mass), _abi.ptr(acc_stiff)
         a.ft, a.fs, a.step = float(ft), float(fs), int(step)
@@ -267,12 +303,17 @@ def source_points(prop: Propagator, mat: dict, sources) -> tuple[PointList, torc
         column += len(lin)
     table = np.ascontiguousarray(np.concatenate(signals, axis=1), dtype=sim.np_dtype)
     pts = prop.point_list(np.concatenate(lins), np.concatenate(shots), np.concatenate(cols),
-                          column, weight=np.concatenate(weights))
+                          column, (column, 1, 0), weight=np.concatenate(weights))
     return pts, torch.as_tensor(table, device=prop.device)
 
 
-def sensor_points(prop: Propagator, sensors, weight=None, shot_stride: int = 0) -> PointList:
-    """Sensor nodes every shot shares, column k the k-th sensor."""
+def sensor_points(prop: Propagator, sensors, weight=None) -> PointList:
+    """Sensor nodes every shot shares, column k the k-th sensor.
+
+    They write (or, weighted, read) an (N, sensors, lanes) table: the sensor
+    record, or the adjoint signal `lane_table` lays out.
+    """
     lin = prop.sim.flatten_indices(sensors)
-    return prop.point_list(lin, -1, np.arange(len(lin), dtype=np.int32), len(lin),
-                           weight=weight, shot_stride=shot_stride)
+    count, lanes = len(lin), prop.lanes
+    return prop.point_list(lin, -1, np.arange(count, dtype=np.int32), count,
+                           (count * lanes, lanes, 1), weight=weight)

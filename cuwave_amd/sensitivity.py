@@ -168,22 +168,20 @@ def sensitivity(sim: Simulation, source: Source, indicator, sensors, objective):
     dev = prop.device
     src, signal = source_points(prop, mat, [source])
     rec = sensor_points(prop, sensors)
-    nrec = rec.points.columns
-    V = torch.zeros((sim.N + 2, 1, *sim.Nx_padded), dtype=sim.dtype, device=dev)
-    record = torch.zeros((1, sim.N, nrec), dtype=sim.dtype, device=dev)
+    V = prop.levels_memory(sim.N + 2)
+    record = prop.record_memory(rec)
     prop.forward(V, 0, sim.N, offset=2, aux=prop.aux_memory(), sources=src, signal=signal,
                  sensors=rec, record=record)
-    um = record[0]
+    um = prop.shots(record)[0]
     cost, dphi = objective(um)
-    adjoint = adjoint_signal(sim, dphi, sensors)
+    adjoint = prop.lane_table(adjoint_signal(sim, dphi, sensors)[None])
     lin = sim.flatten_indices(sensors)
-    adj = sensor_points(prop, sensors, weight=sim.excitation_weights(mat, lin),
-                        shot_stride=sim.N * nrec)
-    P = torch.zeros((2, 1, *sim.Nx_padded), dtype=sim.dtype, device=dev)
+    adj = sensor_points(prop, sensors, weight=sim.excitation_weights(mat, lin))
+    P = prop.levels_memory(2)
     acc = {k: prop.accumulator() for k in ("mass", "stiff")}
     prop.backward(P, V, 0, sim.N, u_offset=2, eta=prop.eta_memory(), g_mass=acc["mass"],
                   g_stiff=acc["stiff"], sensors=adj, adjoint=adjoint)
-    grads = _finalize(sim, {k: prop.reduce(a) for k, a in acc.items()}, dev)
+    grads = _finalize(sim, acc, dev)
     if sim.pml is not None:
         grads = _zero_outside(grads, pml_nodes(sim), dev)
     return cost, grads, um, {}
@@ -207,7 +205,7 @@ class _Reconstruction:
         self.src, self.signal = source_points(prop, mat, sources)
         self.rec = sensor_points(prop, sensors)
         self.sensors = sensors
-        self.nrec = self.rec.points.columns
+        self.nrec = self.rec.count
         self.bf16 = strip_dtype == "bfloat16"
         if strip_dtype not in (None, "bfloat16"):
             raise ValueError(f"strip_dtype {strip_dtype!r}: None or 'bfloat16'")
@@ -219,18 +217,18 @@ class _Reconstruction:
         if self.count:
             smap.ravel()[sim.flatten_indices(strip)] = np.arange(self.count, dtype=np.int32)
         self.strip_map = torch.as_tensor(smap, device=dev)
-        self.U = torch.zeros((2, self.B, *sim.Nx_padded), dtype=sim.dtype, device=dev)
+        self.U = prop.levels_memory(2)
         self.aux = prop.aux_memory()
         itemsize = 2 if self.bf16 else np.dtype(sim.np_dtype).itemsize
         snapshot = 2 * self.U[0].nbytes + sum(a.nbytes for a in self.aux)
         segment = sim.N
         if checkpoints is not None and self.count:
             # the strip of a segment costs about one snapshot (cuwave/sensitivity.py:575-583)
-            segment = min(sim.N, max(2, snapshot // (2 * self.count * itemsize * self.B)))
+            segment = min(sim.N, max(2, snapshot // (2 * self.count * itemsize * prop.lanes)))
         self.segment = segment
         self.checkpoints = checkpoints or 0
         self.starts = list(range(0, sim.N, segment)) + [sim.N]
-        self.record = torch.zeros((self.B, sim.N, self.nrec), dtype=sim.dtype, device=dev)
+        self.record = prop.record_memory(self.rec)
         self.recorded = 0
         self.steps = 0
         self.snapshots = {}
@@ -241,7 +239,7 @@ class _Reconstruction:
         first = -1 if s == 0 else s  # a zero row stands for step -1
         rows = e - first
         dtype = torch.int16 if self.bf16 else self.sim.dtype
-        buf = torch.zeros((self.B, rows, self.count), dtype=dtype, device=self.dev)
+        buf = torch.zeros((rows, self.count, self.prop.lanes), dtype=dtype, device=self.dev)
         return {"map": self.strip_map, "record": buf, "count": self.count, "first": first,
                 "rows": rows, "bf16": int(self.bf16)}
 
@@ -274,7 +272,7 @@ class _Reconstruction:
         accumulators and how the kernel adds into them.
         """
         sim = self.sim
-        self.lam = torch.zeros((2, self.B, *sim.Nx_padded), dtype=sim.dtype, device=self.dev)
+        self.lam = self.prop.levels_memory(2)
         self.eta = self.prop.eta_memory()
         self.costs = None
         self.seed = None
@@ -299,15 +297,14 @@ class _Reconstruction:
         strip = self.strip_window(s, e)
         self.advance(s, e, strip)
         if self.costs is None:
-            self.costs, dphi = self.objectives(self.record)
+            self.costs, dphi = self.objectives(prop.shots(self.record))
             w = torch.as_tensor(sim.adjoint_weights(self.sensors), device=self.dev)
-            self.adjoint = (torch.flip(torch.as_tensor(dphi, dtype=sim.dtype), (1,)) / w).contiguous()
+            dphi = torch.as_tensor(dphi, dtype=sim.dtype, device=self.dev)
+            self.adjoint = prop.lane_table(torch.flip(dphi, (1,)) / w)
             lin = sim.flatten_indices(self.sensors)
-            self.adj = sensor_points(prop, self.sensors, weight=sim.excitation_weights(self.mat, lin),
-                                     shot_stride=sim.N * self.nrec)
+            self.adj = sensor_points(prop, self.sensors, weight=sim.excitation_weights(self.mat, lin))
         if s == 0:
-            top = self.U[(e - 1) % 2]
-            self.seed = float(torch.linalg.vector_norm(top * torch.as_tensor(self.valid, device=self.dev)))
+            self.seed = self._valid_norm(self.U[(e - 1) % 2])
         common = dict(eta=self.eta, sensors=self.adj, adjoint=self.adjoint, sources=self.src,
                       signal=self.signal, strip=strip, **self.gradient_mode)
         # the top level reads u^(e - 1) as the forward left it; below it the
@@ -319,10 +316,13 @@ class _Reconstruction:
         if s > 0:
             prop.backward(self.lam, self.U, s, s + 1, reverse=False, **common)
 
+    def _valid_norm(self, level: torch.Tensor) -> float:
+        valid = torch.as_tensor(self.valid, device=self.dev)
+        return float(torch.linalg.vector_norm(self.prop.shots(level) * valid))
+
     def drift(self) -> float:
         """Norm of the rebuilt u^(-1), which is zero, over that of the top state."""
-        valid = torch.as_tensor(self.valid, device=self.dev)
-        rebuilt = float(torch.linalg.vector_norm(self.U[1] * valid))
+        rebuilt = self._valid_norm(self.U[1])
         return rebuilt / self.seed if self.seed else float("inf")
 
 
@@ -350,11 +350,11 @@ def reconstruction_sensitivity(sim: Simulation, source: Source, indicator, senso
         return [cost], torch.as_tensor(dphi, dtype=sim.dtype, device=dev)[None]
 
     costs = run.run(objectives, dict(g_mass=acc["mass"], g_stiff=acc["stiff"], skip_layer=True))
-    grads = _finalize(sim, {k: run.prop.reduce(a) for k, a in acc.items()}, dev)
+    grads = _finalize(sim, acc, dev)
     grads = _zero_outside(grads, ~run.valid, dev)
     info = {"strip": run.count, "drift": run.drift(), "segment": run.segment,
             "sweeps": run.steps / sim.N}
-    return costs[0], grads, run.record[0], info
+    return costs[0], grads, run.prop.shots(run.record)[0], info
 
 
 # ---------------------------------------------------------------- superposition

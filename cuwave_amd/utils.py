@@ -25,7 +25,7 @@ from .sensitivity import (
     reconstruction_sensitivity,
     sensitivity,
 )
-from .wave import Simulation, Source, as_numpy, simulate_shots
+from .wave import MAX_BATCH, Simulation, Source, as_numpy, simulate_shots
 
 TOL = 1e-6  # for `distribute`, so exact-boundary coordinates survive rounding
 
@@ -129,8 +129,9 @@ class Sensors:
 
 
 def _batches(items: list, batch: int | None):
-    step = len(items) if not batch else int(batch)
-    for i in range(0, len(items), step):
+    """Slices of at most `batch` (default: all) and at most MAX_BATCH items."""
+    step = min(len(items) if not batch else int(batch), MAX_BATCH)
+    for i in range(0, len(items), max(step, 1)):
         yield i, items[i : i + step]
 
 
@@ -207,8 +208,7 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
 
         costs = run.run(objectives, dict(g_mass=acc, combine=(jm, js), skip_layer=True))
         cost += sum(costs)
-        part = run.prop.reduce(acc)
-        gradient = part if gradient is None else gradient + part
+        gradient = acc if gradient is None else gradient + acc
     grads = _finalize(sim, {"g": gradient}, gradient.device)
     grads = _zero_outside(grads, ~valid, gradient.device)
     return cost, grads["g"]

@@ -22,6 +22,8 @@ from . import _abi
 from .boundary import Dirichlet, Neumann, canonical_boundary, face_mask, validate_pml
 from .stencils import STENCILS, flux_row, stable_dt  # noqa: F401  (re-exported)
 
+MAX_BATCH = 64  # shots one launch advances: one wave's lanes per node
+
 if TYPE_CHECKING:
     from .boundary import Pml
 
@@ -176,6 +178,7 @@ class Simulation:
         """The grid struct of the C ABI for a batch of `nbatch` shots."""
         g = _abi.Grid()
         g.ndim, g.order, g.nbatch = self.ndim, self.space_order, int(nbatch)
+        g.lanes = shot_lanes(nbatch)
         g.dtype = 0 if self.precision == "float32" else 1
         for d in range(self.ndim):
             g.n[d], g.np[d] = self.Nx[d], self.Nx_padded[d]
@@ -199,6 +202,14 @@ class Simulation:
         return 1.0 / self.dt**2
 
 
+def shot_lanes(nbatch: int) -> int:
+    """The innermost extent of a batch's per-shot arrays: a power of two >= nbatch, at most 64."""
+    nbatch = int(nbatch)
+    if not 1 <= nbatch <= MAX_BATCH:
+        raise ValueError(f"a launch batches 1 .. {MAX_BATCH} shots, not {nbatch}")
+    return 1 << (nbatch - 1).bit_length()
+
+
 # ------------------------------------------------------------------- forward
 def simulate_shots(sim: Simulation, sources, indicator, sensors=None, record_every=None):
     """Run every shot of `sources` forward in one batch under material `indicator`.
@@ -215,15 +226,13 @@ def simulate_shots(sim: Simulation, sources, indicator, sensors=None, record_eve
     mat = sim.build_materials(indicator)
     prop = Propagator(sim, mat, len(sources))
     src, signal = source_points(prop, mat, sources)
-    U = torch.zeros((2, len(sources), *sim.Nx_padded), dtype=sim.dtype, device=prop.device)
+    U = prop.levels_memory(2)
     aux = prop.aux_memory()
     rec = record = None
     if sensors is not None:
         sim.require_interior(sensors, "sensor")
         rec = sensor_points(prop, sensors)
-        record = torch.zeros((len(sources), sim.N, rec.points.columns), dtype=sim.dtype,
-                             device=prop.device)
-    interior = (slice(None), *sim.interior)
+        record = prop.record_memory(rec)
     snapshots = []
     if record_every is None:
         prop.forward(U, 0, sim.N, aux=aux, sources=src, signal=signal, sensors=rec, record=record)
@@ -232,10 +241,10 @@ def simulate_shots(sim: Simulation, sources, indicator, sensors=None, record_eve
             prop.forward(U, t, t + 1, aux=aux, sources=src, signal=signal, sensors=rec,
                          record=record)
             if t % record_every == 0:
-                snapshots.append(U[t % 2][interior].cpu().numpy())
-    out = (U[(sim.N - 1) % 2][interior].clone(),)
+                snapshots.append(prop.shots(U[t % 2])[(slice(None), *sim.interior)].cpu().numpy())
+    out = (prop.shots(U[(sim.N - 1) % 2])[(slice(None), *sim.interior)].clone(),)
     if sensors is not None:
-        out += (record,)
+        out += (prop.shots(record),)
     if record_every is not None:
         out += (np.stack(snapshots, axis=1),)
     return out if len(out) > 1 else out[0]

@@ -10,9 +10,15 @@
  *
  * Conventions:
  * - Every pointer is device memory owned by the caller; nothing allocates.
- * - Fields are C-contiguous over the padded grid, the fastest axis padded to
- *   a multiple of 32 (cuwave/wave.py:120-122); a batch of shots is a leading
- *   axis; a time-level slot holds the whole batch.
+ * - Per-node arrays (materials, cell means, gradients, maps) are
+ *   C-contiguous over the padded grid, the fastest axis padded to a multiple
+ *   of 32 (cuwave/wave.py:120-122).
+ * - Per-shot arrays carry the shots innermost: a field level is
+ *   (*np, lanes), `lanes` a power of two from nbatch to 64, the lanes past
+ *   nbatch padding the caller zeroes; the layer memories, the strip record,
+ *   the sensor record and the adjoint signal end in a lanes axis too. One
+ *   node's shots are then one contiguous run, which the kernels read and
+ *   write whole.
  * - `dtype` 0 is float32, 1 is float64; every field of a call has it.
  * - Launches are asynchronous on `stream` (a hipStream_t; 0 the null stream).
  * - A return of 0 is success; anything else is an error whose text
@@ -30,7 +36,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 1
+#define CW_ABI_VERSION 2
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -41,6 +47,8 @@ typedef struct cw_grid {
   int32_t n[3];      /* logical extents, ghosts included (Simulation.Nx) */
   int32_t np[3];     /* padded extents (Simulation.Nx_padded) */
   int32_t nbatch;    /* shots one call advances */
+  int32_t lanes;     /* innermost extent of the per-shot arrays: a power of
+                        two, nbatch <= lanes <= 64 */
   int32_t dtype;     /* 0 float32, 1 float64 */
   int32_t dirichlet; /* faces (bit 2 axis + side) held odd, Dirichlet */
   int32_t neumann;   /* faces refilled even, Neumann */
@@ -60,30 +68,31 @@ typedef struct cw_materials {
 typedef struct cw_pml {
   int32_t lo[3], hi[3];     /* layer rows behind the low / high face, 0 none */
   const void *profile[3];   /* (4, n[d]): a, b at nodes and at half points */
-  void *aux[3];             /* forward memory: (nbatch, 3, slab nodes) */
+  void *aux[3];             /* forward memory: (3, slab nodes, lanes) */
   void *eta[3][2];          /* adjoint memory, two buffers alternating, each
-                               (nbatch, 5, slab nodes): the three recursion
+                               (5, slab nodes, lanes): the three recursion
                                variables, then the two tap fields the
                                neighbours' transposed taps read; zeroed */
 } cw_pml;
 
 /* points a step injects at or records from, bucketed by the library's tiles
- * (cw_point_lists); `col` indexes the signal / record column */
+ * (cw_point_lists); entry e at step row r for shot b reads or writes element
+ * r row_stride + col[e] col_stride + b shot_stride of the signal / record */
 typedef struct cw_points {
   const int32_t *starts; /* (tiles + 1) CSR offsets, NULL for no points */
   const int32_t *lin;    /* flat node index within one shot's field */
   const int32_t *shot;   /* the shot of the entry, -1 for every shot */
   const int32_t *col;    /* column in the signal / record row */
   const void *weight;    /* injection weight, NULL for a record */
-  int32_t columns;       /* row length of the signal / record */
-  int64_t shot_stride;   /* elements between shots of the signal / record,
-                            0 when one table serves every shot */
+  int64_t row_stride;
+  int64_t col_stride;
+  int64_t shot_stride;   /* 0 when one table serves every shot */
 } cw_points;
 
 /* the boundary strip a reverse march replays (cuwave/sensitivity.py:160-195) */
 typedef struct cw_strip {
   const int32_t *map; /* (field) strip column of a node, -1 off the strip */
-  void *record;       /* (nbatch, rows, count): row t - first holds step t */
+  void *record;       /* (rows, count, lanes): row t - first holds step t */
   int32_t count;      /* strip nodes */
   int32_t first;      /* the step of record row 0 */
   int32_t rows;       /* record rows a shot holds */
@@ -91,11 +100,10 @@ typedef struct cw_strip {
 } cw_strip;
 
 /* time levels in slots: level k sits in slot (k + offset) mod slots, each
- * slot holding the batch; slot_stride and batch_stride in elements */
+ * slot a (*np, lanes) level; slot_stride in elements */
 typedef struct cw_levels {
   void *base;
   int64_t slot_stride;
-  int64_t batch_stride;
   int32_t slots;
   int32_t offset;
 } cw_levels;
@@ -111,9 +119,9 @@ typedef struct cw_forward_args {
   cw_materials mat;
   cw_pml pml;            /* lo = hi = 0 on every axis for no layer */
   cw_levels u;           /* the field levels */
-  cw_points sources;     /* signal (N, columns) per shot_stride */
+  cw_points sources;     /* signal rows are steps */
   const void *signal;
-  cw_points sensors;     /* record (nbatch, N, columns) */
+  cw_points sensors;     /* record rows are steps */
   void *record;
   cw_strip strip;        /* strip.map NULL for none */
   int32_t steps;         /* N, the rows of the signal and record tables */
@@ -137,14 +145,13 @@ typedef struct cw_backward_args {
   cw_levels lam;           /* adjoint levels, two slots */
   cw_levels u;             /* forward levels: two slots reversed, or history */
   int32_t reverse;         /* 1: rebuild u^(n - 1) from u^n, u^(n + 1) */
-  cw_points sensors;       /* adjoint load: signal (nbatch, N, columns) */
+  cw_points sensors;       /* adjoint load */
   const void *adjoint_signal; /* row N - 1 - n drives level n */
   cw_points sources;       /* re-injected by the reverse march */
   const void *signal;
   cw_strip strip;
-  void *g_mass, *g_stiff;  /* accumulators (cuwave/scalar.py:751-756) */
-  int64_t grad_batch_stride; /* per-shot fields; 0: the shots sum into the
-                                cw_scalar_grad_groups fields */
+  void *g_mass, *g_stiff;  /* per-node accumulators, the shots summed
+                              (cuwave/scalar.py:751-756) */
   int32_t combine;         /* 1: g_mass += jm gm + js gs (g_stiff unused) */
   double jac_mass, jac_stiff;
   int32_t skip_layer;      /* 1: no gradient inside the layer (zeroed after) */
@@ -153,7 +160,7 @@ typedef struct cw_backward_args {
 } cw_backward_args;
 
 /* the superposition step (cuwave/kernels/scalar_sensitivity.cu:599-668) and
- * the unfused Frechet densities (:352-385), one call a step */
+ * the unfused Frechet densities (:352-385), one call a step, one shot */
 typedef struct cw_superposed_args {
   cw_grid grid;
   cw_materials mat;
@@ -174,13 +181,6 @@ int cw_tile_count(const cw_grid *grid, int32_t *tiles);
  * (count) the point index in bucketed order, stable; host memory */
 int cw_point_lists(const cw_grid *grid, const int32_t *lin, int32_t count,
                    int32_t *starts, int32_t *order);
-
-/* the gradient accumulators a shot-summed adjoint call (grad_batch_stride 0)
- * adds into: `groups` fields one after the other, field k summing the shots
- * of group k (the kernels step a group's shots together), summed by the
- * caller once the sweep is done */
-int cw_scalar_grad_groups(const cw_grid *grid, const cw_pml *pml,
-                          int32_t *groups);
 
 /* the inverse inertia and the harmonic cell stiffness of every axis
  * (cuwave/kernels/scalar.cu:48-64 `sc sp / (sc + sp)`, :454) */

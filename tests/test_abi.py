@@ -90,6 +90,19 @@ def test_bad_arguments_are_refused_before_any_launch():
         _abi.call("cw_scalar_backward", ctypes.byref(b), 0, 1, None)
 
 
+def test_lanes_hold_the_batch_as_a_power_of_two():
+    sim = _sim()
+    assert [sim.abi_grid(b).lanes for b in (1, 2, 3, 5, 32, 33, 64)] == [1, 2, 4, 8, 32, 64, 64]
+    with pytest.raises(ValueError, match="1 .. 64"):
+        sim.abi_grid(65)
+    args = _abi.ForwardArgs()
+    args.grid = sim.abi_grid(3)
+    for lanes in (3, 2, 128):
+        args.grid.lanes = lanes
+        with pytest.raises(RuntimeError, match="lanes must be a power of two"):
+            _abi.call("cw_scalar_forward", ctypes.byref(args), 0, 1, None)
+
+
 def test_the_product_refuses_to_run_without_its_device(monkeypatch):
     import torch
 

@@ -113,7 +113,8 @@ def test_a_batch_steps_every_shot_as_alone():
 
 
 def test_one_adjoint_level_is_the_transpose_of_one_forward_step():
-    # <W m A^T p, u> = <W m p, A u> in float64, through the kernels themselves
+    # <W m A^T p, u> = <W m p, A u> in float64, through the kernels themselves,
+    # for every shot of a batch of three (four lanes, one of them padding)
     from cuwave_amd import boundary, scalar
     from cuwave_amd.engine import Propagator
 
@@ -123,26 +124,30 @@ def test_one_adjoint_level_is_the_transpose_of_one_forward_step():
                                   rho1=1.0, rho2=3.0, kappa1=1.0, kappa2=5.0)
         rng = np.random.default_rng(0)
         mat = sim.build_materials(0.2 + rng.random(sim.Nx_padded))
-        prop = Propagator(sim, mat, 1)
+        prop = Propagator(sim, mat, 3)
+        assert prop.lanes == 4
         dev = prop.device
         mask = np.zeros(sim.Nx_padded)
         mask[tuple(slice(1, n - 1) for n in sim.Nx)] = 1.0
-        U = torch.zeros((2, 1, *sim.Nx_padded), dtype=torch.float64, device=dev)
-        L = torch.zeros((2, 1, *sim.Nx_padded), dtype=torch.float64, device=dev)
+        U, L = prop.levels_memory(2), prop.levels_memory(2)
         # level -1 (slot 1) is u with its ghosts refilled, level -2 (slot 0) zero;
         # lambda^1 (slot 1) is p, lambda^2 zero
         for T in (U, L):
-            T[1, 0] = torch.as_tensor(rng.standard_normal(sim.Nx_padded) * mask, device=dev)
-            S._refill(sim, T[1, 0])
-        u, p = host(U[1, 0]) * mask, host(L[1, 0]) * mask
+            for b in range(3):
+                level = torch.as_tensor(rng.standard_normal(sim.Nx_padded) * mask, device=dev)
+                S._refill(sim, level)
+                T[1, ..., b] = level
+        u, p = host(prop.shots(U[1])) * mask, host(prop.shots(L[1])) * mask
         prop.forward(U, 0, 1)
-        Au = host(U[0, 0]) * mask - 2 * u
-        g = torch.zeros(sim.Nx_padded, dtype=torch.float64, device=dev)
+        Au = host(prop.shots(U[0])) * mask - 2 * u
+        g = prop.accumulator()
         prop.backward(L, U, 0, 1, eta=prop.eta_memory(), g_mass=g, g_stiff=g.clone())
-        Ap = host(L[0, 0]) * mask - 2 * p
+        Ap = host(prop.shots(L[0])) * mask - 2 * p
         Wm = sim.cell_weights() * mask / mat["minv_host"]
-        scale = np.sqrt(np.sum(Wm * Ap**2) * np.sum(Wm * u**2))
-        assert abs(np.sum(Wm * Ap * u) - np.sum(Wm * p * Au)) / scale < 1e-12, order
+        for b in range(3):
+            scale = np.sqrt(np.sum(Wm * Ap[b] ** 2) * np.sum(Wm * u[b] ** 2))
+            assert abs(np.sum(Wm * Ap[b] * u[b]) - np.sum(Wm * p[b] * Au[b])) / scale < 1e-12, (order, b)
+        assert not host(U[..., 3]).any() and not host(L[..., 3]).any()  # the padding lane stays zero
 
 
 def test_the_gradient_matches_finite_differences_in_float64():
