@@ -135,6 +135,48 @@ __device__ __forceinline__ int64_t slab_size(const Geom<T> &g, int d, int lo,
   return plane * (int64_t)(lo + hi);
 }
 
+// A buffer resource over `bytes` bytes from a base: a load or store takes a
+// byte offset from a vector register plus one from a scalar register (or
+// the instruction), with no 64-bit address arithmetic per access; the
+// hardware drops an access whose vector offset lies past the end (a read
+// gives 0). The word-3 value is the gfx9-family raw-buffer format.
+struct Buf {
+  __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ Buf make_buf(const void *base, uint32_t bytes) {
+  return {__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0,
+                                            (int)bytes, 0x00020000)};
+}
+template <typename T>
+__device__ __forceinline__ T bload(const Buf &b, uint32_t vo, uint32_t so);
+template <>
+__device__ __forceinline__ float bload<float>(const Buf &b, uint32_t vo,
+                                              uint32_t so) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_raw_buffer_load_b32(b.r, vo, so, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(const Buf &b, uint32_t vo,
+                                                uint32_t so) {
+  return __builtin_bit_cast(double,
+                            __builtin_amdgcn_raw_buffer_load_b64(b.r, vo, so, 0));
+}
+__device__ __forceinline__ int bload_int(const Buf &b, uint32_t vo,
+                                         uint32_t so) {
+  return (int)__builtin_amdgcn_raw_buffer_load_b32(b.r, vo, so, 0);
+}
+__device__ __forceinline__ void bstore(const Buf &b, float v, uint32_t vo,
+                                       uint32_t so) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), b.r,
+                                        vo, so, 0);
+}
+__device__ __forceinline__ void bstore(const Buf &b, double v, uint32_t vo,
+                                       uint32_t so) {
+  using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), b.r, vo, so,
+                                        0);
+}
+
 // the sum over a node's shot lanes (a butterfly: every lane of the node ends
 // with the same bits); every lane of the node must take part
 template <typename T>

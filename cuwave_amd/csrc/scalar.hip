@@ -224,12 +224,16 @@ constexpr int kChunk = 32; // rows a streaming thread marches
 
 // the stream box of a 2D grid: every node order / 2 + 2 off the walls, out
 // of reach of the wall mirror, the refill, the held nodes and the adjoint's
-// reflected taps; empty on other grids and on grids too small for one
+// reflected taps; empty on other grids, on grids too small for one and on
+// levels past the 4 GiB a buffer resource addresses
 cw::Box stream_box(const cw_grid &g) {
   cw::Box box{};
   const int margin = g.order / 2 + 2;
   box.chunk = kChunk;
   if (g.ndim != 2 || g.n[0] - margin <= margin || g.n[1] - margin <= margin)
+    return box;
+  const int64_t bytes = (int64_t)g.np[0] * g.np[1] * g.lanes * (g.dtype ? 8 : 4);
+  if (bytes >= ((int64_t)1 << 32))
     return box;
   box.y0 = margin;
   box.y1 = g.n[0] - margin;

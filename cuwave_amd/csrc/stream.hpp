@@ -7,15 +7,23 @@
 // the slow axis, keeping the 2R + 1 values of its column that the slow-axis
 // taps read in a register queue: each row costs one new load per queued
 // field. The fast-axis taps are the same lane of the neighbouring columns,
-// each a contiguous run of the wave (common.hpp), served by L1 and L2; the
-// next row's loads are all issued before the current row is computed. A
+// each a contiguous run of the wave (common.hpp), served by L1 and L2. A
 // block is 256 / lanes neighbouring columns, and blocks are dealt to the
 // XCDs in contiguous runs, rows of blocks across the box, so the columns
 // whose taps overlap share one L2.
 //
+// Every load of a row is issued a row ahead into one of two row buffers
+// that alternate (the loop is unrolled by two, so no buffer is copied while
+// its loads are in flight), and all of them are issued on every row, the
+// last row's repeated at the end of a chunk: the compiler then waits for
+// exactly the older buffer and each wave keeps a whole row of loads in
+// flight while it computes. Fields and materials go through buffer
+// resources: one vector offset per thread advanced a row at a time, the tap
+// offsets uniform.
+//
 // A row of the slow axis is either in that axis' slab or not, so its layer
-// terms are uniform over the block and its profile values scalar loads; the
-// fast axis' slab columns are fixed per thread for the whole chunk.
+// terms are uniform over the block; the fast axis' slab columns are fixed
+// per thread for the whole chunk.
 //
 // The arithmetic is the general kernels' term by term, in the same order,
 // so both paths give the same bits; the cell weight of every node here is 1.
@@ -54,18 +62,23 @@ __device__ __forceinline__ Prof<T> profile_at(const T *pr, int N, int a) {
           pr[3 * N + a - 1]};
 }
 
-// the fast-axis taps k = -R .. -1, 1 .. R of element e into x[0 .. 2R)
-template <typename T, int R>
-__device__ __forceinline__ void load_row_taps(T *x, const T *__restrict__ u,
-                                              uint32_t e, uint32_t el) {
-#pragma unroll
-  for (int k = 1; k <= R; ++k) {
-    x[R - k] = u[e - k * el];
-    x[R - 1 + k] = u[e + k * el];
-  }
+// whether a point list holds a point in the tiles of rows [ya, yb) at tile
+// column tx: looked up once a chunk, so rows without points (nearly all)
+// skip the lists without waiting on a load
+template <typename T>
+__device__ __forceinline__ bool chunk_has(const Points<T> &pts,
+                                          const Geom<T> &g, int ya, int yb,
+                                          int tx) {
+  if (!pts.starts)
+    return false;
+  int any = 0;
+  for (int ty = ya / TileShape<2>::TY; ty <= (yb - 1) / TileShape<2>::TY; ++ty)
+    any |= pts.starts[ty * g.tf + tx + 1] - pts.starts[ty * g.tf + tx];
+  return __builtin_amdgcn_readfirstlane(any) != 0;
 }
 
-// taps[0 .. 2R] from the fast-axis taps and the centre
+// taps[0 .. 2R] from the fast-axis taps x[0 .. 2R) (k = -R .. -1, 1 .. R)
+// and the centre
 template <typename T, int R>
 __device__ __forceinline__ void row_line(T *t, const T *x, T c) {
 #pragma unroll
@@ -76,7 +89,45 @@ __device__ __forceinline__ void row_line(T *t, const T *x, T c) {
   t[R] = c;
 }
 
+// Offsets of a thread's march. `f` and `n` are the byte offsets of the
+// field element and of the node at (y - R, x - R) for the current row y;
+// F(j, k) and N(j, k) the uniform byte offsets of (y + j, x + k) from them.
+template <typename T, int R> struct March {
+  uint32_t f, n;         // at row ya
+  uint32_t frow, nrow;   // bytes a row
+  uint32_t es0, el, s0;  // element strides of a row and a column, node row
+  __device__ __forceinline__ uint32_t F(int j, int k) const {
+    return ((uint32_t)(j + R) * es0 + (uint32_t)(k + R) * el) * sizeof(T);
+  }
+  __device__ __forceinline__ uint32_t N(int j, int k) const {
+    return ((uint32_t)(j + R) * s0 + (uint32_t)(k + R)) * sizeof(T);
+  }
+};
+
+template <typename T, int R>
+__device__ __forceinline__ March<T, R> march_at(const Geom<T> &g, int ya,
+                                                int xl, int b) {
+  March<T, R> m;
+  const int sh = g.lshift;
+  m.s0 = (uint32_t)g.s[0];
+  m.es0 = m.s0 << sh;
+  m.el = 1u << sh;
+  const uint32_t node = (uint32_t)(ya - R) * m.s0 + (uint32_t)(xl - R);
+  m.f = ((node << sh) + (uint32_t)b) * sizeof(T);
+  m.n = node * sizeof(T);
+  m.frow = m.es0 * sizeof(T);
+  m.nrow = m.s0 * sizeof(T);
+  return m;
+}
+
 // --------------------------------------------------------------- forward
+template <typename T, int R> struct FwdRow {
+  T feed;       // u^(t-1) at (y + R, x): the queue's new entry
+  T x[2 * R];   // u^(t-1) at (y, x -+ k)
+  T u0, mi, gp1, gm1, gp0;
+  int col;      // the node's strip column (or -1)
+};
+
 template <typename T, int R>
 __global__ __launch_bounds__(kThreads) void forward_stream(
     const FwdParams<T> p, const Box box, const int t) {
@@ -88,8 +139,7 @@ __global__ __launch_bounds__(kThreads) void forward_stream(
   const int x = xb + (int)(threadIdx.x >> sh);
   const bool active = x >= box.x0 && x < box.x1;
   const int xl = min(max(x, box.x0), box.x1 - 1); // every thread loads
-  const int s0 = g.s[0];
-  const uint32_t es0 = (uint32_t)s0 << sh, el = 1u << sh;
+  const int tx = __builtin_amdgcn_readfirstlane(xl) / TileShape<2>::TX;
   const int N0 = g.n[0], N1 = g.n[1];
   const int L0 = p.pml.lo[0], H0 = p.pml.hi[0];
   const int L1 = p.pml.lo[1], H1 = p.pml.hi[1];
@@ -99,49 +149,54 @@ __global__ __launch_bounds__(kThreads) void forward_stream(
   Prof<T> pf1{};
   if (prow1 >= 0)
     pf1 = profile_at(p.pml.profile[1], N1, xl);
-  const int tx = xl / TileShape<2>::TX;
-  const T *__restrict__ u1 = p.u.at(t - 1);
-  const T *__restrict__ u0 = p.u.at(t - 2);
-  T *__restrict__ u2 = p.u.at(t);
-  const T *__restrict__ cx = p.cell + g.nodes; // cells along the fast axis
-  const T *__restrict__ cyc = p.cell;          // cells along the slow axis
-  uint32_t node = (uint32_t)(ya * s0 + xl);
-  uint32_t e = (node << sh) + b;
+  const uint32_t fbytes = (uint32_t)((g.nodes << sh) * sizeof(T));
+  const uint32_t nbytes = (uint32_t)(g.nodes * sizeof(T));
+  const Buf U1 = make_buf(p.u.at(t - 1), fbytes);
+  const Buf U0 = make_buf(p.u.at(t - 2), fbytes);
+  const Buf U2 = make_buf(p.u.at(t), fbytes);
+  const Buf INV = make_buf(p.inv, nbytes);
+  const Buf CY = make_buf(p.cell, nbytes);           // cells along the slow axis
+  const Buf CX = make_buf(p.cell + g.nodes, nbytes); // along the fast axis
+  const Buf MAP = make_buf(p.strip.map, p.strip.map ? nbytes / sizeof(T) * 4 : 0u);
+  const March<T, R> M = march_at<T, R>(g, ya, xl, b);
+  const uint32_t mrow = M.nrow / sizeof(T) * 4, m0 = M.n / sizeof(T) * 4;
+  const bool has_src = chunk_has(p.src, g, ya, yb, tx);
+  const bool has_rec = chunk_has(p.rec, g, ya, yb, tx);
+
+  auto load = [&](FwdRow<T, R> &r, int y) {
+    const uint32_t f = M.f + (uint32_t)(y - ya) * M.frow;
+    const uint32_t n = M.n + (uint32_t)(y - ya) * M.nrow;
+    r.feed = bload<T>(U1, f, M.F(R, 0));
+#pragma unroll
+    for (int k = 1; k <= R; ++k) {
+      r.x[R - k] = bload<T>(U1, f, M.F(0, -k));
+      r.x[R - 1 + k] = bload<T>(U1, f, M.F(0, k));
+    }
+    r.u0 = bload<T>(U0, f, M.F(0, 0));
+    r.mi = bload<T>(INV, n, M.N(0, 0));
+    r.gp1 = bload<T>(CX, n, M.N(0, 0));
+    r.gm1 = bload<T>(CX, n, M.N(0, -1));
+    r.gp0 = bload<T>(CY, n, M.N(0, 0));
+    r.col = bload_int(MAP, m0 + (uint32_t)(y - ya) * mrow, M.N(0, 0) / sizeof(T) * 4);
+  };
 
   T q[2 * R + 1];
 #pragma unroll
   for (int j = 0; j < 2 * R; ++j)
-    q[j] = u1[e + (j - R) * es0];
-  T gm0 = cyc[node - s0];
-  // the next row's loads, issued a row ahead
-  T nq = u1[e + R * es0], nx[2 * R];
-  load_row_taps<T, R>(nx, u1, e, el);
-  T nu0 = u0[e], nmi = p.inv[node];
-  T ngp1 = cx[node], ngm1 = cx[node - 1], ngp0 = cyc[node];
+    q[j] = bload<T>(U1, M.f, M.F(j - R, 0));
+  T gm0 = bload<T>(CY, M.n, M.N(-1, 0));
 
-  for (int y = ya; y < yb; ++y) {
-    q[2 * R] = nq;
-    T tf[2 * R + 1];
-    row_line<T, R>(tf, nx, q[R]);
-    const T u0c = nu0, mi = nmi, gp1 = ngp1, gm1 = ngm1, gp0 = ngp0;
-    const uint32_t ec = e, nc = node;
-    e += es0;
-    node += s0;
-    if (y + 1 < yb) {
-      nq = u1[e + R * es0];
-      load_row_taps<T, R>(nx, u1, e, el);
-      nu0 = u0[e];
-      nmi = p.inv[node];
-      ngp1 = cx[node];
-      ngm1 = cx[node - 1];
-      ngp0 = cyc[node];
-    }
+  auto step = [&](const FwdRow<T, R> &r, int y) {
+    q[2 * R] = r.feed;
     if (active) {
+      T tf[2 * R + 1];
+      row_line<T, R>(tf, r.x, q[R]);
+      const T mi = r.mi;
       T div[2];
-      div[0] = flux<T, R>(q, gp0, gm0, g.op);
-      div[1] = flux<T, R>(tf, gp1, gm1, g.op);
+      div[0] = flux<T, R>(q, r.gp0, gm0, g.op);
+      div[1] = flux<T, R>(tf, r.gp1, r.gm1, g.op);
       const T lap = combine_axes<T, 2, R>(div, g.f);
-      T v = -u0c + (T)2 * q[R] + mi * lap;
+      T v = -r.u0 + (T)2 * q[R] + mi * lap;
       const int prow0 = L0 + H0 ? slab_row(y, N0, L0, H0) : -1;
       if (prow0 >= 0) {
         const Prof<T> pf = profile_at(p.pml.profile[0], N0, y);
@@ -149,7 +204,7 @@ __global__ __launch_bounds__(kThreads) void forward_stream(
         cell_diffs<T, R>(q, g.op, Dp, Dm);
         v += pml_forward(p.pml.aux[0] +
                              (((int64_t)prow0 * g.np[1] + x) << sh) + b,
-                         size0, Dp, Dm, gp0, gm0, g.f[0], mi, pf.an, pf.bn,
+                         size0, Dp, Dm, r.gp0, gm0, g.f[0], mi, pf.an, pf.bn,
                          pf.ah, pf.bh, pf.ah1, pf.bh1);
       }
       if (prow1 >= 0) {
@@ -157,21 +212,22 @@ __global__ __launch_bounds__(kThreads) void forward_stream(
         cell_diffs<T, R>(tf, g.op, Dp, Dm);
         v += pml_forward(p.pml.aux[1] +
                              (((int64_t)y * (L1 + H1) + prow1) << sh) + b,
-                         size1, Dp, Dm, gp1, gm1, g.f[1], mi, pf1.an, pf1.bn,
-                         pf1.ah, pf1.bh, pf1.ah1, pf1.bh1);
+                         size1, Dp, Dm, r.gp1, r.gm1, g.f[1], mi, pf1.an,
+                         pf1.bn, pf1.ah, pf1.bh, pf1.ah1, pf1.bh1);
       }
+      const int nc = y * g.s[0] + x;
       const int tile = (y / TileShape<2>::TY) * g.tf + tx;
-      if (p.src.starts)
+      if (has_src)
         for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
           if (p.src.takes(k, nc, b))
             v += p.src.weight[k] * p.signal[p.src.at(t, k, b)];
-      u2[ec] = v;
-      if (p.rec.starts)
+      bstore(U2, v, M.f + (uint32_t)(y - ya) * M.frow, M.F(0, 0));
+      if (has_rec)
         for (int k = p.rec.starts[tile]; k < p.rec.starts[tile + 1]; ++k)
           if (p.rec.takes(k, nc, b))
             p.record[p.rec.at(t, k, b)] = v;
       if ((prow0 >= 0 || prow1 >= 0) && p.strip.map) {
-        const int col = p.strip.map[nc];
+        const int col = r.col;
         if (col >= 0) {
           const int64_t at =
               (((int64_t)(t - p.strip.first) * p.strip.count + col) << sh) + b;
@@ -182,14 +238,35 @@ __global__ __launch_bounds__(kThreads) void forward_stream(
         }
       }
     }
-    gm0 = gp0;
+    gm0 = r.gp0;
 #pragma unroll
     for (int j = 0; j < 2 * R; ++j)
       q[j] = q[j + 1];
+  };
+
+  FwdRow<T, R> A, B;
+  load(A, ya);
+  for (int y = ya; y < yb; y += 2) {
+    load(B, min(y + 1, yb - 1));
+    step(A, y);
+    load(A, min(y + 2, yb - 1));
+    if (y + 1 < yb)
+      step(B, y + 1);
   }
 }
 
 // --------------------------------------------------------------- adjoint
+template <typename T, int R> struct BwdRow {
+  T lfeed, ufeed;   // lambda^(n+1), u^n at (y + R, x)
+  T lx[2 * R], ux[2 * R]; // their fast-axis taps at row y
+  T cx[2 * R];      // fast-axis cell means at x - R .. x + R - 1
+  T cfeed;          // slow-axis cell mean of row y + R - 1
+  T lo, uo, mi;     // lambda^(n+2), u^(n+1), 1 / m at (y, x)
+  T accm, accs;     // the gradient accumulators at the node
+  T sk[5];          // stiffness at the node and its four neighbours
+  int col;          // the node's strip column (or -1)
+};
+
 template <typename T, int R, bool STIFF>
 __global__ __launch_bounds__(kThreads) void backward_stream(
     const BwdParams<T> p, const Box box, const int n) {
@@ -201,8 +278,7 @@ __global__ __launch_bounds__(kThreads) void backward_stream(
   const int x = xb + (int)(threadIdx.x >> sh);
   const bool active = x >= box.x0 && x < box.x1;
   const int xl = min(max(x, box.x0), box.x1 - 1);
-  const int s0 = g.s[0];
-  const uint32_t es0 = (uint32_t)s0 << sh, el = 1u << sh;
+  const int tx = __builtin_amdgcn_readfirstlane(xl) / TileShape<2>::TX;
   const int N0 = g.n[0], N1 = g.n[1];
   const int L0 = p.pml.lo[0], H0 = p.pml.hi[0];
   const int L1 = p.pml.lo[1], H1 = p.pml.hi[1];
@@ -218,80 +294,92 @@ __global__ __launch_bounds__(kThreads) void backward_stream(
   Prof<T> pf1{};
   if (prow1 >= 0)
     pf1 = profile_at(p.pml.profile[1], N1, xl);
-  const int tx = xl / TileShape<2>::TX;
-  const T *__restrict__ l1 = p.lam.at(n + 1);
-  const T *__restrict__ u1 = p.u.at(n);
-  T *__restrict__ l0 = p.lam.at(n);
-  T *__restrict__ uo = p.u.at(n + 1);
-  const T *__restrict__ cx = p.cell + g.nodes;
-  const T *__restrict__ cyc = p.cell;
-  uint32_t node = (uint32_t)(ya * s0 + xl);
-  uint32_t e = (node << sh) + b;
+  const uint32_t fbytes = (uint32_t)((g.nodes << sh) * sizeof(T));
+  const uint32_t nbytes = (uint32_t)(g.nodes * sizeof(T));
+  const Buf L1b = make_buf(p.lam.at(n + 1), fbytes);
+  const Buf U1b = make_buf(p.u.at(n), fbytes);
+  const Buf L0b = make_buf(p.lam.at(n), fbytes);
+  // without the reverse march u^(n + 1) is neither read nor written: an
+  // empty resource turns its loads into zeros that touch no memory
+  const Buf UOb = make_buf(p.u.at(n + 1), reverse ? fbytes : 0u);
+  const Buf INV = make_buf(p.inv, nbytes);
+  const Buf CY = make_buf(p.cell, nbytes);
+  const Buf CX = make_buf(p.cell + g.nodes, nbytes);
+  const Buf GM = make_buf(p.gmass, nbytes);
+  const Buf GS = make_buf(p.gstiff, p.combine ? 0u : nbytes);
+  const Buf SK = make_buf(p.stiff, STIFF ? nbytes : 0u);
+  const Buf MAP = make_buf(p.strip.map, reverse && p.strip.map ? nbytes / sizeof(T) * 4 : 0u);
+  const March<T, R> M = march_at<T, R>(g, ya, xl, b);
+  const uint32_t mrow = M.nrow / sizeof(T) * 4, m0 = M.n / sizeof(T) * 4;
+  const bool has_rec = chunk_has(p.rec, g, ya, yb, tx);
+  const bool has_src = reverse && chunk_has(p.src, g, ya, yb, tx);
+
+  auto load = [&](BwdRow<T, R> &r, int y) {
+    const uint32_t f = M.f + (uint32_t)(y - ya) * M.frow;
+    const uint32_t nn = M.n + (uint32_t)(y - ya) * M.nrow;
+    r.lfeed = bload<T>(L1b, f, M.F(R, 0));
+    r.ufeed = bload<T>(U1b, f, M.F(R, 0));
+#pragma unroll
+    for (int k = 1; k <= R; ++k) {
+      r.lx[R - k] = bload<T>(L1b, f, M.F(0, -k));
+      r.lx[R - 1 + k] = bload<T>(L1b, f, M.F(0, k));
+      r.ux[R - k] = bload<T>(U1b, f, M.F(0, -k));
+      r.ux[R - 1 + k] = bload<T>(U1b, f, M.F(0, k));
+    }
+#pragma unroll
+    for (int k = 0; k < 2 * R; ++k)
+      r.cx[k] = bload<T>(CX, nn, M.N(0, k - R));
+    r.cfeed = bload<T>(CY, nn, M.N(R - 1, 0));
+    r.lo = bload<T>(L0b, f, M.F(0, 0));
+    r.uo = bload<T>(UOb, f, M.F(0, 0));
+    r.mi = bload<T>(INV, nn, M.N(0, 0));
+    r.accm = bload<T>(GM, nn, M.N(0, 0));
+    r.accs = bload<T>(GS, nn, M.N(0, 0));
+    if (STIFF) {
+      r.sk[0] = bload<T>(SK, nn, M.N(0, 0));
+      r.sk[1] = bload<T>(SK, nn, M.N(1, 0));
+      r.sk[2] = bload<T>(SK, nn, M.N(-1, 0));
+      r.sk[3] = bload<T>(SK, nn, M.N(0, 1));
+      r.sk[4] = bload<T>(SK, nn, M.N(0, -1));
+    }
+    r.col = bload_int(MAP, m0 + (uint32_t)(y - ya) * mrow, M.N(0, 0) / sizeof(T) * 4);
+  };
 
   // queues along the slow axis: lambda^(n + 1) and u^n rows y - R .. y + R,
   // the cell means of rows y - R .. y + R - 1
   T ql[2 * R + 1], qu[2 * R + 1], qc[2 * R];
 #pragma unroll
   for (int j = 0; j < 2 * R; ++j) {
-    ql[j] = l1[e + (j - R) * es0];
-    qu[j] = u1[e + (j - R) * es0];
+    ql[j] = bload<T>(L1b, M.f, M.F(j - R, 0));
+    qu[j] = bload<T>(U1b, M.f, M.F(j - R, 0));
     if (j < 2 * R - 1)
-      qc[j] = cyc[node + (j - R) * s0];
+      qc[j] = bload<T>(CY, M.n, M.N(j - R, 0));
   }
-  // the next row's loads: the queue feeds, the fast-axis taps of lambda, u
-  // and the fast-axis cell means (x - R .. x + R - 1), the slots overwritten
-  T nl, nu, nlx[2 * R], nux[2 * R], ncx[2 * R], nlo, nuo, nmi, ncy;
-  auto fetch = [&]() {
-    nl = l1[e + R * es0];
-    nu = u1[e + R * es0];
-    load_row_taps<T, R>(nlx, l1, e, el);
-    load_row_taps<T, R>(nux, u1, e, el);
-#pragma unroll
-    for (int k = 0; k < 2 * R; ++k)
-      ncx[k] = cx[node + k - R];
-    nlo = l0[e];
-    nuo = reverse ? uo[e] : (T)0;
-    nmi = p.inv[node];
-    ncy = cyc[node + (R - 1) * s0];
-  };
-  fetch();
 
-  for (int y = ya; y < yb; ++y) {
-    ql[2 * R] = nl;
-    qu[2 * R] = nu;
-    qc[2 * R - 1] = ncy;
-    T lt1[2 * R + 1], ut1[2 * R + 1], chi1[R], clo1[R];
-    row_line<T, R>(lt1, nlx, ql[R]);
-    row_line<T, R>(ut1, nux, qu[R]);
-#pragma unroll
-    for (int j = 1; j <= R; ++j) {
-      chi1[j - 1] = ncx[R + j - 1];
-      clo1[j - 1] = ncx[R - j];
-    }
-    const T lo_ = nlo, uo_ = nuo, mi = nmi;
-    const uint32_t ec = e, nc = node;
-    e += es0;
-    node += s0;
-    if (y + 1 < yb)
-      fetch();
+  auto step = [&](const BwdRow<T, R> &r, int y) {
+    ql[2 * R] = r.lfeed;
+    qu[2 * R] = r.ufeed;
+    qc[2 * R - 1] = r.cfeed;
     if (active) {
+      const uint32_t f = M.f + (uint32_t)(y - ya) * M.frow;
+      const int nc = y * g.s[0] + x;
+      T lt1[2 * R + 1], ut1[2 * R + 1], chi1[R], clo1[R], chi0[R], clo0[R];
+      row_line<T, R>(lt1, r.lx, ql[R]);
+      row_line<T, R>(ut1, r.ux, qu[R]);
+#pragma unroll
+      for (int j = 1; j <= R; ++j) {
+        chi1[j - 1] = r.cx[R + j - 1];
+        clo1[j - 1] = r.cx[R - j];
+        chi0[j - 1] = qc[R + j - 1];
+        clo0[j - 1] = qc[R - j];
+      }
+      const T mi = r.mi, lo_ = r.lo;
       // this row along the slow axis: slab row, reach of the transposed taps
       const int prow0 = L0 + H0 ? slab_row(y, N0, L0, H0) : -1;
       const bool ext0 = (L0 && y - 1 < L0 + R) || (H0 && N0 - 2 - y < H0 + R);
       const bool layer = prow0 >= 0 || prow1 >= 0;
       const bool grad = !(p.skip_layer && layer);
-      T chi0[R], clo0[R];
-#pragma unroll
-      for (int j = 1; j <= R; ++j) {
-        chi0[j - 1] = qc[R + j - 1];
-        clo0[j - 1] = qc[R - j];
-      }
-      T accm = 0, accs = 0;
-      if (grad) {
-        accm = p.gmass[nc];
-        if (!p.combine)
-          accs = p.gstiff[nc];
-      }
+      const T accm = r.accm, accs = r.accs;
       const T lc = ql[R], uc = qu[R];
       T div_l = 0;
       div_l += g.f[0] * transposed<T, R>(ql, chi0, clo0, g.op);
@@ -302,12 +390,11 @@ __global__ __launch_bounds__(kThreads) void backward_stream(
       cm = cm - p.mf * uc * (ln - (T)2 * lc + lo_);
       T cs = chain ? accs : (T)0;
       if (STIFF && grad) {
-        const T sc = p.stiff[nc];
         T gsum = 0;
-        gsum += stiffness_density<T, R>(qu, ql, lc, sc, p.stiff[nc + s0],
-                                        p.stiff[nc - s0], g.f[0], g.op);
-        gsum += stiffness_density<T, R>(ut1, lt1, lc, sc, p.stiff[nc + 1],
-                                        p.stiff[nc - 1], g.f[1], g.op);
+        gsum += stiffness_density<T, R>(qu, ql, lc, r.sk[0], r.sk[1], r.sk[2],
+                                        g.f[0], g.op);
+        gsum += stiffness_density<T, R>(ut1, lt1, lc, r.sk[0], r.sk[3],
+                                        r.sk[4], g.f[1], g.op);
         cs = cs - p.mf * gsum;
       }
       // the transposed layer: the slab nodes' tap fields within reach
@@ -334,14 +421,14 @@ __global__ __launch_bounds__(kThreads) void backward_stream(
         cm -= p.mf * uc * delta;
       }
       const int tile = (y / TileShape<2>::TY) * g.tf + tx;
-      if (p.rec.starts)
+      if (has_rec)
         for (int k = p.rec.starts[tile]; k < p.rec.starts[tile + 1]; ++k)
           if (p.rec.takes(k, nc, b)) {
             const T load = p.rec.weight[k] * p.adjoint[p.rec.at(m, k, b)];
             ln += load;
             cm += -p.mf * uc * load;
           }
-      l0[ec] = ln;
+      bstore(L0b, ln, f, M.F(0, 0));
       // the memory the next level gathers, from this level's final lambda
       if (prow0 >= 0) {
         const Prof<T> pf0 = profile_at(p.pml.profile[0], N0, y);
@@ -361,9 +448,7 @@ __global__ __launch_bounds__(kThreads) void backward_stream(
       if (grad)
         add_gradient(p, nc, b, accm, accs, cm, cs);
       if (reverse) {
-        int strip_col = -1;
-        if (layer && p.strip.map)
-          strip_col = p.strip.map[nc];
+        const int strip_col = layer && p.strip.map ? r.col : -1;
         T v;
         if (strip_col >= 0) {
           // a layer node is replayed from the forward sweep's record
@@ -379,13 +464,13 @@ __global__ __launch_bounds__(kThreads) void backward_stream(
           div[0] = flux<T, R>(qu, chi0[0], clo0[0], g.op);
           div[1] = flux<T, R>(ut1, chi1[0], clo1[0], g.op);
           const T lap = combine_axes<T, 2, R>(div, g.f);
-          v = -uo_ + (T)2 * uc + mi * lap;
-          if (p.src.starts)
+          v = -r.uo + (T)2 * uc + mi * lap;
+          if (has_src)
             for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
               if (p.src.takes(k, nc, b))
                 v += p.src.weight[k] * p.signal[p.src.at(n + 1, k, b)];
         }
-        uo[ec] = v;
+        bstore(UOb, v, f, M.F(0, 0));
       }
     }
 #pragma unroll
@@ -396,6 +481,16 @@ __global__ __launch_bounds__(kThreads) void backward_stream(
 #pragma unroll
     for (int j = 0; j < 2 * R - 1; ++j)
       qc[j] = qc[j + 1];
+  };
+
+  BwdRow<T, R> A, B;
+  load(A, ya);
+  for (int y = ya; y < yb; y += 2) {
+    load(B, min(y + 1, yb - 1));
+    step(A, y);
+    load(A, min(y + 2, yb - 1));
+    if (y + 1 < yb)
+      step(B, y + 1);
   }
 }
 
