@@ -242,13 +242,16 @@ cw::Box stream_box(const cw_grid &g) {
   return box;
 }
 
+// the stream blocks of a sweep, padded to a multiple of the XCD count so
+// the general blocks after them start on XCD 0
 int box_blocks(const cw::Box &b, int lanes) {
   if (b.y0 >= b.y1)
     return 0;
   const int cols = cw::kThreads / lanes;
   const int xa = b.x0 / cols * cols;
   const int nchunk = (b.y1 - b.y0 + b.chunk - 1) / b.chunk;
-  return nchunk * ((b.x1 - xa + cols - 1) / cols);
+  const int blocks = nchunk * ((b.x1 - xa + cols - 1) / cols);
+  return (blocks + cw::kXcds - 1) / cw::kXcds * cw::kXcds;
 }
 
 // where the general kernels run: the band around the box, or every node
@@ -292,16 +295,11 @@ template <typename T, int NDIM, int R> struct Forward {
     p.strip = make_strip<T>(a.strip);
     const cw::Box box = stream_box(a.grid);
     p.region = band_of(a.grid, box);
-    const int general = general_blocks(a.grid, p.region);
     const int streams = NDIM == 2 ? box_blocks(box, a.grid.lanes) : 0;
-    for (int t = t0; t < t1; ++t) {
-      if (general)
-        hipLaunchKernelGGL((cw::forward_kernel<T, NDIM, R>), dim3(general),
-                           dim3(cw::kThreads), 0, stream, p, t);
-      if (streams)
-        hipLaunchKernelGGL((cw::forward_stream<T, R>), dim3(streams),
-                           dim3(cw::kThreads), 0, stream, p, box, t);
-    }
+    const int blocks = streams + general_blocks(a.grid, p.region);
+    for (int t = t0; t < t1; ++t)
+      hipLaunchKernelGGL((cw::forward_step<T, NDIM, R>), dim3(blocks),
+                         dim3(cw::kThreads), 0, stream, p, box, t, streams);
     return hip_status("forward");
   }
 };
@@ -334,21 +332,16 @@ template <typename T, int NDIM, int R> struct Backward {
     p.steps = a.steps;
     const cw::Box box = stream_box(a.grid);
     p.region = band_of(a.grid, box);
-    const int general = general_blocks(a.grid, p.region);
     const int streams = NDIM == 2 ? box_blocks(box, a.grid.lanes) : 0;
+    const int blocks = streams + general_blocks(a.grid, p.region);
     for (int n = n1 - 1; n >= n0; --n) {
-      if (general)
-        hipLaunchKernelGGL((cw::backward_kernel<T, NDIM, R>), dim3(general),
-                           dim3(cw::kThreads), 0, stream, p, n);
-      if (streams) {
-        if (p.stiff_grad)
-          hipLaunchKernelGGL((cw::backward_stream<T, R, true>), dim3(streams),
-                             dim3(cw::kThreads), 0, stream, p, box, n);
-        else
-          hipLaunchKernelGGL((cw::backward_stream<T, R, false>),
-                             dim3(streams), dim3(cw::kThreads), 0, stream, p,
-                             box, n);
-      }
+      if (p.stiff_grad)
+        hipLaunchKernelGGL((cw::backward_step<T, NDIM, R, true>), dim3(blocks),
+                           dim3(cw::kThreads), 0, stream, p, box, n, streams);
+      else
+        hipLaunchKernelGGL((cw::backward_step<T, NDIM, R, false>),
+                           dim3(blocks), dim3(cw::kThreads), 0, stream, p, box,
+                           n, streams);
     }
     return hip_status("backward");
   }

@@ -312,13 +312,12 @@ __device__ __forceinline__ bool band_node(const Region &r, const Geom<T> &g,
 }
 
 // this thread's node, every lane of a node the same: the band's, or the
-// interior's in row-major order
+// interior's in row-major order; block `bid` of `nb` general blocks
 template <int NDIM, typename T>
 __device__ __forceinline__ bool general_node(const Region &r, const Geom<T> &g,
-                                             int *a) {
+                                             int *a, int bid, int nb) {
   const int per = kThreads >> g.lshift;
-  const int t = xcd_tile(blockIdx.x, gridDim.x) * per +
-                (int)(threadIdx.x >> g.lshift);
+  const int t = xcd_tile(bid, nb) * per + (int)(threadIdx.x >> g.lshift);
   if (NDIM == 2 && r.band)
     return t < r.count && band_node(r, g, t, a);
   if (t >= g.interior)
@@ -364,12 +363,13 @@ __global__ __launch_bounds__(kThreads) void prepare_kernel(
 }
 
 // ------------------------------------------------------------ forward step
+// step t at the nodes of general block `bid` of `nb`
 template <typename T, int NDIM, int R>
-__global__ __launch_bounds__(kThreads) void forward_kernel(const FwdParams<T> p,
-                                                           const int t) {
+__device__ __forceinline__ void forward_general(const FwdParams<T> &p,
+                                                const int t, int bid, int nb) {
   const Geom<T> &g = p.g;
   int a[3];
-  if (!general_node<NDIM>(p.region, g, a))
+  if (!general_node<NDIM>(p.region, g, a, bid, nb))
     return;
   const int sh = g.lshift;
   const int b = threadIdx.x & (g.lanes - 1);
@@ -474,16 +474,18 @@ __global__ __launch_bounds__(kThreads) void forward_kernel(const FwdParams<T> p,
 }
 
 // --------------------------------------------------------- adjoint step
-// level n of the adjoint (see cw_backward_args): lambda^n over the slot of
-// lambda^(n + 2), its gradients against u^n, the transposed layer, and the
-// reverse march u^(n - 1) over the slot of u^(n + 1)
+// level n of the adjoint (see cw_backward_args) at the nodes of general
+// block `bid` of `nb`: lambda^n over the slot of lambda^(n + 2), its
+// gradients against u^n, the transposed layer, and the reverse march
+// u^(n - 1) over the slot of u^(n + 1)
 template <typename T, int NDIM, int R>
-__global__ __launch_bounds__(kThreads) void backward_kernel(
-    const BwdParams<T> p, const int n) {
+__device__ __forceinline__ void backward_general(const BwdParams<T> &p,
+                                                 const int n, int bid,
+                                                 int nb) {
   constexpr int REACH = R + 2; // scalar_sensitivity.cu:44 WALL_REACH
   const Geom<T> &g = p.g;
   int a[3];
-  if (!general_node<NDIM>(p.region, g, a))
+  if (!general_node<NDIM>(p.region, g, a, bid, nb))
     return;
   const int sh = g.lshift;
   const int b = threadIdx.x & (g.lanes - 1);
@@ -736,7 +738,7 @@ __global__ __launch_bounds__(kThreads) void superposed_kernel(
     int step) {
   const Region full{};
   int a[3];
-  if (!general_node<NDIM>(full, g, a))
+  if (!general_node<NDIM>(full, g, a, blockIdx.x, gridDim.x))
     return;
   int64_t idx = 0;
   bool held = false;

@@ -39,17 +39,22 @@ struct Box {
   int chunk; // rows a thread marches
 };
 
-// this block's rows [ya, yb) and first column xb; blocks start at multiples
-// of their width, so no wave straddles a point tile
-__device__ __forceinline__ void box_item(const Box &box, int cols, int &ya,
-                                         int &yb, int &xb) {
+// stream block `bid` of `nb` (a multiple of the XCD count, at least the
+// box's blocks): its rows [ya, yb) and first column xb, or false for a
+// padding block. Blocks start at multiples of their width, so no wave
+// straddles a point tile.
+__device__ __forceinline__ bool box_item(const Box &box, int cols, int bid,
+                                         int nb, int &ya, int &yb, int &xb) {
   const int xa = box.x0 / cols * cols;
   const int ncx = (box.x1 - xa + cols - 1) / cols;
   const int nchunk = (box.y1 - box.y0 + box.chunk - 1) / box.chunk;
-  const int w = xcd_tile(blockIdx.x, nchunk * ncx);
+  const int w = xcd_tile(bid, nb);
+  if (w >= nchunk * ncx)
+    return false;
   ya = box.y0 + (w / ncx) * box.chunk;
   yb = min(ya + box.chunk, box.y1);
   xb = xa + (w % ncx) * cols;
+  return true;
 }
 
 // the profile of axis d at node a and at the half point below it
@@ -129,12 +134,14 @@ template <typename T, int R> struct FwdRow {
 };
 
 template <typename T, int R>
-__global__ __launch_bounds__(kThreads) void forward_stream(
-    const FwdParams<T> p, const Box box, const int t) {
+__device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
+                                               const Box &box, const int t,
+                                               int bid, int nb) {
   const Geom<T> &g = p.g;
   const int sh = g.lshift;
   int ya, yb, xb;
-  box_item(box, kThreads >> sh, ya, yb, xb);
+  if (!box_item(box, kThreads >> sh, bid, nb, ya, yb, xb))
+    return;
   const int b = threadIdx.x & (g.lanes - 1);
   const int x = xb + (int)(threadIdx.x >> sh);
   const bool active = x >= box.x0 && x < box.x1;
@@ -268,12 +275,14 @@ template <typename T, int R> struct BwdRow {
 };
 
 template <typename T, int R, bool STIFF>
-__global__ __launch_bounds__(kThreads) void backward_stream(
-    const BwdParams<T> p, const Box box, const int n) {
+__device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
+                                                const Box &box, const int n,
+                                                int bid, int nb) {
   const Geom<T> &g = p.g;
   const int sh = g.lshift;
   int ya, yb, xb;
-  box_item(box, kThreads >> sh, ya, yb, xb);
+  if (!box_item(box, kThreads >> sh, bid, nb, ya, yb, xb))
+    return;
   const int b = threadIdx.x & (g.lanes - 1);
   const int x = xb + (int)(threadIdx.x >> sh);
   const bool active = x >= box.x0 && x < box.x1;
@@ -492,6 +501,40 @@ __global__ __launch_bounds__(kThreads) void backward_stream(
     if (y + 1 < yb)
       step(B, y + 1);
   }
+}
+
+// ------------------------------------------------------------ the steps
+// One launch a step: the first `streams` blocks (a multiple of the XCD
+// count) march the stream box, the rest take the general nodes (the 2D wall
+// band, or every node in 1D and 3D).
+template <typename T, int NDIM, int R>
+__global__ __launch_bounds__(kThreads) void forward_step(const FwdParams<T> p,
+                                                         const Box box,
+                                                         const int t,
+                                                         const int streams) {
+  if constexpr (NDIM == 2) {
+    if ((int)blockIdx.x < streams) {
+      forward_stream<T, R>(p, box, t, blockIdx.x, streams);
+      return;
+    }
+  }
+  forward_general<T, NDIM, R>(p, t, blockIdx.x - streams,
+                              gridDim.x - streams);
+}
+
+template <typename T, int NDIM, int R, bool STIFF>
+__global__ __launch_bounds__(kThreads) void backward_step(const BwdParams<T> p,
+                                                          const Box box,
+                                                          const int n,
+                                                          const int streams) {
+  if constexpr (NDIM == 2) {
+    if ((int)blockIdx.x < streams) {
+      backward_stream<T, R, STIFF>(p, box, n, blockIdx.x, streams);
+      return;
+    }
+  }
+  backward_general<T, NDIM, R>(p, n, blockIdx.x - streams,
+                               gridDim.x - streams);
 }
 
 } // namespace cw
