@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -41,7 +41,7 @@ class Grid(ctypes.Structure):
 
 
 class Materials(ctypes.Structure):
-    _fields_ = [("stiff", VP), ("minv", VP), ("cell", VP), ("inv", VP)]
+    _fields_ = [("stiff", VP), ("minv", VP), ("prep", VP)]
 
 
 class Pml(ctypes.Structure):

@@ -1,0 +1,189 @@
+// Host-side pieces shared by the translation units of the library: the C
+// ABI structs turned into the kernels' parameter blocks, the split of a 2D
+// grid into the stream box and the wall band, and the step launchers, one
+// translation unit per (precision, radius) so the library builds in
+// parallel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/cuwave_amd.h"
+#include "scalar_kernels.hpp"
+#include "stream.hpp"
+
+namespace cwh {
+
+// records the error text cw_last_error() returns; always returns 1
+int fail(const std::string &what);
+
+inline int hip_status(const char *where) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    fail(std::string(where) + ": " + hipGetErrorString(e));
+    return 2;
+  }
+  return 0;
+}
+
+inline int lanes_shift(int lanes) {
+  int k = 0;
+  while ((1 << k) < lanes)
+    ++k;
+  return k;
+}
+
+template <typename T> cw::Geom<T> make_geom(const cw_grid &gr) {
+  cw::Geom<T> g{};
+  const int nd = gr.ndim;
+  for (int d = 0; d < 3; ++d) {
+    g.n[d] = d < nd ? gr.n[d] : 1;
+    g.np[d] = d < nd ? gr.np[d] : 1;
+  }
+  int64_t stride = 1;
+  for (int d = nd - 1; d >= 0; --d) {
+    g.s[d] = (int)stride;
+    stride *= g.np[d];
+  }
+  for (int d = nd; d < 3; ++d)
+    g.s[d] = 0;
+  g.nodes = stride;
+  g.lanes = gr.lanes;
+  g.lshift = lanes_shift(gr.lanes);
+  g.nbatch = gr.nbatch;
+  g.dirichlet = gr.dirichlet;
+  g.neumann = gr.neumann;
+  const int TX = nd == 1 ? cw::TileShape<1>::TX : cw::TileShape<2>::TX;
+  const int TY = cw::kThreads / TX;
+  g.tf = (g.np[nd - 1] + TX - 1) / TX;
+  g.tm = nd >= 2 ? (g.n[nd - 2] + TY - 1) / TY : 1;
+  g.ts = nd == 3 ? g.n[0] : 1;
+  g.ntiles = g.tf * g.tm * g.ts;
+  g.interior = 1;
+  for (int d = 0; d < nd; ++d)
+    g.interior *= g.n[d] - 2;
+  for (int k = 0; k < 4; ++k)
+    g.op[k] = (T)gr.op[k];
+  for (int d = 0; d < 3; ++d)
+    g.f[d] = (T)gr.f[d];
+  return g;
+}
+
+template <typename T> cw::Points<T> make_points(const cw_points &p) {
+  cw::Points<T> q{};
+  q.starts = p.starts;
+  q.lin = p.lin;
+  q.shot = p.shot;
+  q.col = p.col;
+  q.weight = (const T *)p.weight;
+  q.rs = p.row_stride;
+  q.cs = p.col_stride;
+  q.ss = p.shot_stride;
+  return q;
+}
+
+template <typename T> cw::Levels<T> make_levels(const cw_levels &l) {
+  cw::Levels<T> q{};
+  q.base = (T *)l.base;
+  q.slot = l.slot_stride;
+  q.slots = l.slots;
+  q.offset = l.offset;
+  return q;
+}
+
+template <typename T> cw::Layer<T> make_layer(const cw_pml &p, int ndim) {
+  cw::Layer<T> q{};
+  for (int d = 0; d < 3; ++d) {
+    q.lo[d] = d < ndim ? p.lo[d] : 0;
+    q.hi[d] = d < ndim ? p.hi[d] : 0;
+    q.profile[d] = (const T *)p.profile[d];
+    q.aux[d] = (T *)p.aux[d];
+    q.eta[d][0] = (T *)p.eta[d][0];
+    q.eta[d][1] = (T *)p.eta[d][1];
+  }
+  return q;
+}
+
+template <typename T> cw::Strip<T> make_strip(const cw_strip &s) {
+  cw::Strip<T> q{};
+  q.map = s.map;
+  q.record = s.record;
+  q.count = s.count;
+  q.first = s.first;
+  q.rows = s.rows;
+  q.bf16 = s.bf16;
+  return q;
+}
+
+// ---------------------------------------------------------- stream box
+constexpr int kChunk = 32; // rows a streaming thread marches
+
+// the stream box of a 2D grid: every node order / 2 + 2 off the walls, out
+// of reach of the wall mirror, the refill, the held nodes and the adjoint's
+// reflected taps; empty on other grids, on grids too small for one and
+// where two levels pass the 4 GiB a buffer resource addresses
+inline cw::Box stream_box(const cw_grid &g) {
+  cw::Box box{};
+  const int margin = g.order / 2 + 2;
+  box.chunk = kChunk;
+  if (g.ndim != 2 || g.n[0] - margin <= margin || g.n[1] - margin <= margin)
+    return box;
+  const int64_t level = (int64_t)g.np[0] * g.np[1] * g.lanes * (g.dtype ? 8 : 4);
+  if (2 * level >= ((int64_t)1 << 32))
+    return box;
+  box.y0 = margin;
+  box.y1 = g.n[0] - margin;
+  box.x0 = margin;
+  box.x1 = g.n[1] - margin;
+  return box;
+}
+
+// the stream blocks of a sweep, padded to a multiple of the XCD count so
+// the general blocks after them start on XCD 0
+inline int box_blocks(const cw::Box &b, int lanes) {
+  if (b.y0 >= b.y1)
+    return 0;
+  const int cols = cw::kThreads / lanes;
+  const int xa = b.x0 / cols * cols;
+  const int nchunk = (b.y1 - b.y0 + b.chunk - 1) / b.chunk;
+  const int blocks = nchunk * ((b.x1 - xa + cols - 1) / cols);
+  return (blocks + cw::kXcds - 1) / cw::kXcds * cw::kXcds;
+}
+
+// where the general kernels run: the band around the box, or every node
+inline cw::Region band_of(const cw_grid &g, const cw::Box &b) {
+  cw::Region r{};
+  if (b.y0 >= b.y1)
+    return r;
+  r.band = 1;
+  r.y0 = b.y0;
+  r.y1 = b.y1;
+  r.x0 = b.x0;
+  r.x1 = b.x1;
+  const int W = g.n[1] - 2;
+  r.count = (b.y0 - 1) * W + (g.n[0] - 1 - b.y1) * W +
+            (b.y1 - b.y0) * ((b.x0 - 1) + (g.n[1] - 1 - b.x1));
+  return r;
+}
+
+inline int general_blocks(const cw_grid &gr, const cw::Region &r) {
+  const int per = cw::kThreads / gr.lanes; // nodes a block takes
+  int64_t nodes = r.band ? r.count : 1;
+  if (!r.band)
+    for (int d = 0; d < gr.ndim; ++d)
+      nodes *= gr.n[d] - 2;
+  return (int)((nodes + per - 1) / per);
+}
+
+// ------------------------------------------------------------ launchers
+// forward steps t0 .. t1 - 1 and adjoint levels n1 - 1 .. n0, one launch
+// each; defined in steps.inc, one translation unit per (T, R)
+template <typename T, int NDIM, int R>
+int forward_steps(const cw_forward_args &a, int t0, int t1,
+                  hipStream_t stream);
+template <typename T, int NDIM, int R>
+int backward_steps(const cw_backward_args &a, int n0, int n1,
+                   hipStream_t stream);
+
+} // namespace cwh

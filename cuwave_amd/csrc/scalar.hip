@@ -1,6 +1,7 @@
 // The C ABI of include/cuwave_amd.h for the pressure family: argument
-// checks, template dispatch over (precision, dimension, radius), and the
-// time loops that launch the fused kernels of a step.
+// checks, template dispatch over (precision, dimension, radius), the
+// material preparation and the superposition step. The step launchers live
+// in steps_*.hip, one translation unit per (precision, radius).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -9,11 +10,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/cuwave_amd.h"
-#include "scalar_kernels.hpp"
-#include "stream.hpp"
+#include "host.hpp"
 
-namespace {
+namespace cwh {
 
 thread_local std::string g_error;
 
@@ -22,21 +21,13 @@ int fail(const std::string &what) {
   return 1;
 }
 
-int hip_status(const char *where) {
-  const hipError_t e = hipGetLastError();
-  if (e != hipSuccess) {
-    g_error = std::string(where) + ": " + hipGetErrorString(e);
-    return 2;
-  }
-  return 0;
-}
+} // namespace cwh
 
-int log2_of(int v) {
-  int k = 0;
-  while ((1 << k) < v)
-    ++k;
-  return k;
-}
+namespace {
+
+using cwh::fail;
+using cwh::hip_status;
+using cwh::make_geom;
 
 int check_grid(const cw_grid *g) {
   if (!g)
@@ -65,88 +56,6 @@ int check_grid(const cw_grid *g) {
   if (nodes * g->lanes >= (int64_t)1 << 31)
     return fail("one level of the batch overflows the int32 element index");
   return 0;
-}
-
-template <typename T> cw::Geom<T> make_geom(const cw_grid &gr) {
-  cw::Geom<T> g{};
-  const int nd = gr.ndim;
-  for (int d = 0; d < 3; ++d) {
-    g.n[d] = d < nd ? gr.n[d] : 1;
-    g.np[d] = d < nd ? gr.np[d] : 1;
-  }
-  int64_t stride = 1;
-  for (int d = nd - 1; d >= 0; --d) {
-    g.s[d] = (int)stride;
-    stride *= g.np[d];
-  }
-  for (int d = nd; d < 3; ++d)
-    g.s[d] = 0;
-  g.nodes = stride;
-  g.lanes = gr.lanes;
-  g.lshift = log2_of(gr.lanes);
-  g.nbatch = gr.nbatch;
-  g.dirichlet = gr.dirichlet;
-  g.neumann = gr.neumann;
-  const int TX = nd == 1 ? cw::TileShape<1>::TX : cw::TileShape<2>::TX;
-  const int TY = cw::kThreads / TX;
-  g.tf = (g.np[nd - 1] + TX - 1) / TX;
-  g.tm = nd >= 2 ? (g.n[nd - 2] + TY - 1) / TY : 1;
-  g.ts = nd == 3 ? g.n[0] : 1;
-  g.ntiles = g.tf * g.tm * g.ts;
-  g.interior = 1;
-  for (int d = 0; d < nd; ++d)
-    g.interior *= g.n[d] - 2;
-  for (int k = 0; k < 4; ++k)
-    g.op[k] = (T)gr.op[k];
-  for (int d = 0; d < 3; ++d)
-    g.f[d] = (T)gr.f[d];
-  return g;
-}
-
-template <typename T> cw::Points<T> make_points(const cw_points &p) {
-  cw::Points<T> q{};
-  q.starts = p.starts;
-  q.lin = p.lin;
-  q.shot = p.shot;
-  q.col = p.col;
-  q.weight = (const T *)p.weight;
-  q.rs = p.row_stride;
-  q.cs = p.col_stride;
-  q.ss = p.shot_stride;
-  return q;
-}
-
-template <typename T> cw::Levels<T> make_levels(const cw_levels &l) {
-  cw::Levels<T> q{};
-  q.base = (T *)l.base;
-  q.slot = l.slot_stride;
-  q.slots = l.slots;
-  q.offset = l.offset;
-  return q;
-}
-
-template <typename T> cw::Layer<T> make_layer(const cw_pml &p, int ndim) {
-  cw::Layer<T> q{};
-  for (int d = 0; d < 3; ++d) {
-    q.lo[d] = d < ndim ? p.lo[d] : 0;
-    q.hi[d] = d < ndim ? p.hi[d] : 0;
-    q.profile[d] = (const T *)p.profile[d];
-    q.aux[d] = (T *)p.aux[d];
-    q.eta[d][0] = (T *)p.eta[d][0];
-    q.eta[d][1] = (T *)p.eta[d][1];
-  }
-  return q;
-}
-
-template <typename T> cw::Strip<T> make_strip(const cw_strip &s) {
-  cw::Strip<T> q{};
-  q.map = s.map;
-  q.record = s.record;
-  q.count = s.count;
-  q.first = s.first;
-  q.rows = s.rows;
-  q.bf16 = s.bf16;
-  return q;
 }
 
 int check_levels(const cw_levels &l, const char *what) {
@@ -212,148 +121,36 @@ template <typename T, int NDIM, int R> struct Prepare {
     if (R == 1) // one preparation serves every radius
       hipLaunchKernelGGL((cw::prepare_kernel<T, NDIM>), dim3(blocks),
                          dim3(cw::kThreads), 0, stream, g, (const T *)m.stiff,
-                         (const T *)m.minv, (T *)m.inv, (T *)m.cell);
+                         (const T *)m.minv, (T *)m.prep + NDIM * g.nodes,
+                         (T *)m.prep);
     else
       return Prepare<T, NDIM, 1>::run(gr, m, stream);
     return hip_status("prepare_kernel");
   }
 };
 
-// ---------------------------------------------------------- stream box
-constexpr int kChunk = 32; // rows a streaming thread marches
-
-// the stream box of a 2D grid: every node order / 2 + 2 off the walls, out
-// of reach of the wall mirror, the refill, the held nodes and the adjoint's
-// reflected taps; empty on other grids, on grids too small for one and on
-// levels past the 4 GiB a buffer resource addresses
-cw::Box stream_box(const cw_grid &g) {
-  cw::Box box{};
-  const int margin = g.order / 2 + 2;
-  box.chunk = kChunk;
-  if (g.ndim != 2 || g.n[0] - margin <= margin || g.n[1] - margin <= margin)
-    return box;
-  const int64_t bytes = (int64_t)g.np[0] * g.np[1] * g.lanes * (g.dtype ? 8 : 4);
-  if (bytes >= ((int64_t)1 << 32))
-    return box;
-  box.y0 = margin;
-  box.y1 = g.n[0] - margin;
-  box.x0 = margin;
-  box.x1 = g.n[1] - margin;
-  return box;
-}
-
-// the stream blocks of a sweep, padded to a multiple of the XCD count so
-// the general blocks after them start on XCD 0
-int box_blocks(const cw::Box &b, int lanes) {
-  if (b.y0 >= b.y1)
-    return 0;
-  const int cols = cw::kThreads / lanes;
-  const int xa = b.x0 / cols * cols;
-  const int nchunk = (b.y1 - b.y0 + b.chunk - 1) / b.chunk;
-  const int blocks = nchunk * ((b.x1 - xa + cols - 1) / cols);
-  return (blocks + cw::kXcds - 1) / cw::kXcds * cw::kXcds;
-}
-
-// where the general kernels run: the band around the box, or every node
-cw::Region band_of(const cw_grid &g, const cw::Box &b) {
-  cw::Region r{};
-  if (b.y0 >= b.y1)
-    return r;
-  r.band = 1;
-  r.y0 = b.y0;
-  r.y1 = b.y1;
-  r.x0 = b.x0;
-  r.x1 = b.x1;
-  const int W = g.n[1] - 2;
-  r.count = (b.y0 - 1) * W + (g.n[0] - 1 - b.y1) * W +
-            (b.y1 - b.y0) * ((b.x0 - 1) + (g.n[1] - 1 - b.x1));
-  return r;
-}
-
-int general_blocks(const cw_grid &gr, const cw::Region &r) {
-  const int per = cw::kThreads / gr.lanes; // nodes a block takes
-  int64_t nodes = r.band ? r.count : 1;
-  if (!r.band)
-    for (int d = 0; d < gr.ndim; ++d)
-      nodes *= gr.n[d] - 2;
-  return (int)((nodes + per - 1) / per);
-}
-
 template <typename T, int NDIM, int R> struct Forward {
   static int run(const cw_forward_args &a, int t0, int t1,
                  hipStream_t stream) {
-    cw::FwdParams<T> p{};
-    p.g = make_geom<T>(a.grid);
-    p.inv = (const T *)a.mat.inv;
-    p.cell = (const T *)a.mat.cell;
-    p.pml = make_layer<T>(a.pml, a.grid.ndim);
-    p.u = make_levels<T>(a.u);
-    p.src = make_points<T>(a.sources);
-    p.signal = (const T *)a.signal;
-    p.rec = make_points<T>(a.sensors);
-    p.record = (T *)a.record;
-    p.strip = make_strip<T>(a.strip);
-    const cw::Box box = stream_box(a.grid);
-    p.region = band_of(a.grid, box);
-    const int streams = NDIM == 2 ? box_blocks(box, a.grid.lanes) : 0;
-    const int blocks = streams + general_blocks(a.grid, p.region);
-    for (int t = t0; t < t1; ++t)
-      hipLaunchKernelGGL((cw::forward_step<T, NDIM, R>), dim3(blocks),
-                         dim3(cw::kThreads), 0, stream, p, box, t, streams);
-    return hip_status("forward");
+    return cwh::forward_steps<T, NDIM, R>(a, t0, t1, stream);
   }
 };
 
 template <typename T, int NDIM, int R> struct Backward {
   static int run(const cw_backward_args &a, int n0, int n1,
                  hipStream_t stream) {
-    cw::BwdParams<T> p{};
-    p.g = make_geom<T>(a.grid);
-    p.inv = (const T *)a.mat.inv;
-    p.cell = (const T *)a.mat.cell;
-    p.stiff = (const T *)a.mat.stiff;
-    p.pml = make_layer<T>(a.pml, a.grid.ndim);
-    p.lam = make_levels<T>(a.lam);
-    p.u = make_levels<T>(a.u);
-    p.reverse = a.reverse;
-    p.rec = make_points<T>(a.sensors);
-    p.adjoint = (const T *)a.adjoint_signal;
-    p.src = make_points<T>(a.sources);
-    p.signal = (const T *)a.signal;
-    p.strip = make_strip<T>(a.strip);
-    p.gmass = (T *)a.g_mass;
-    p.gstiff = (T *)a.g_stiff;
-    p.combine = a.combine;
-    p.jm = (T)a.jac_mass;
-    p.js = (T)a.jac_stiff;
-    p.stiff_grad = !a.combine || a.jac_stiff != 0.0;
-    p.skip_layer = a.skip_layer;
-    p.mf = (T)a.mass_factor;
-    p.steps = a.steps;
-    const cw::Box box = stream_box(a.grid);
-    p.region = band_of(a.grid, box);
-    const int streams = NDIM == 2 ? box_blocks(box, a.grid.lanes) : 0;
-    const int blocks = streams + general_blocks(a.grid, p.region);
-    for (int n = n1 - 1; n >= n0; --n) {
-      if (p.stiff_grad)
-        hipLaunchKernelGGL((cw::backward_step<T, NDIM, R, true>), dim3(blocks),
-                           dim3(cw::kThreads), 0, stream, p, box, n, streams);
-      else
-        hipLaunchKernelGGL((cw::backward_step<T, NDIM, R, false>),
-                           dim3(blocks), dim3(cw::kThreads), 0, stream, p, box,
-                           n, streams);
-    }
-    return hip_status("backward");
+    return cwh::backward_steps<T, NDIM, R>(a, n0, n1, stream);
   }
 };
 
 template <typename T, int NDIM, int R> struct Superposed {
   static int run(const cw_superposed_args &a, hipStream_t stream) {
     const cw::Geom<T> g = make_geom<T>(a.grid);
-    const int blocks = general_blocks(a.grid, cw::Region{});
+    const int blocks = cwh::general_blocks(a.grid, cw::Region{});
+    const T *cell = (const T *)a.mat.prep;
     hipLaunchKernelGGL((cw::superposed_kernel<T, NDIM, R>), dim3(blocks),
-                       dim3(cw::kThreads), 0, stream, g, (const T *)a.mat.inv,
-                       (const T *)a.mat.cell, (const T *)a.u0, (const T *)a.u1,
+                       dim3(cw::kThreads), 0, stream, g, cell + NDIM * g.nodes,
+                       cell, (const T *)a.u0, (const T *)a.u1,
                        (T *)a.u2, (T *)a.acc_mass, (T *)a.acc_stiff, (T)a.ft,
                        (T)a.fs, a.step);
     return hip_status("superposed_kernel");
@@ -366,7 +163,7 @@ extern "C" {
 
 int cw_version(void) { return CW_ABI_VERSION; }
 
-const char *cw_last_error(void) { return g_error.c_str(); }
+const char *cw_last_error(void) { return cwh::g_error.c_str(); }
 
 int cw_tile_count(const cw_grid *grid, int32_t *tiles) {
   if (int e = check_grid(grid))
@@ -418,8 +215,8 @@ int cw_scalar_prepare(const cw_grid *grid, const cw_materials *mat,
                       void *stream) {
   if (int e = check_grid(grid))
     return e;
-  if (!mat || !mat->stiff || !mat->cell || !mat->inv)
-    return fail("prepare needs stiff, cell and inv");
+  if (!mat || !mat->stiff || !mat->prep)
+    return fail("prepare needs stiff and prep");
   return dispatch<Prepare>(*grid, *grid, *mat, (hipStream_t)stream);
 }
 
@@ -429,8 +226,8 @@ int cw_scalar_forward(const cw_forward_args *args, int32_t t0, int32_t t1,
     return fail("args is NULL");
   if (int e = check_grid(&args->grid))
     return e;
-  if (!args->mat.inv || !args->mat.cell)
-    return fail("forward needs the prepared inv and cell");
+  if (!args->mat.prep)
+    return fail("forward needs the prepared materials");
   if (int e = check_levels(args->u, "u"))
     return e;
   if (int e = check_layer(args->grid, args->pml, true))
@@ -463,8 +260,8 @@ int cw_scalar_backward(const cw_backward_args *args, int32_t n0, int32_t n1,
     return fail("args is NULL");
   if (int e = check_grid(&args->grid))
     return e;
-  if (!args->mat.inv || !args->mat.cell || !args->mat.stiff)
-    return fail("backward needs stiff and the prepared inv and cell");
+  if (!args->mat.prep || !args->mat.stiff)
+    return fail("backward needs stiff and the prepared materials");
   if (int e = check_levels(args->lam, "lambda"))
     return e;
   if (int e = check_levels(args->u, "u"))
@@ -511,9 +308,9 @@ int cw_scalar_superposed(const cw_superposed_args *args, void *stream) {
     return e;
   if (args->grid.nbatch != 1 || args->grid.lanes != 1)
     return fail("the superposition step takes one shot a call");
-  if (!args->mat.inv || !args->mat.cell || !args->u0 || !args->u1 ||
+  if (!args->mat.prep || !args->u0 || !args->u1 ||
       !args->u2 || !args->acc_mass || !args->acc_stiff)
-    return fail("superposed needs inv, cell, u0, u1, u2 and both accumulators");
+    return fail("superposed needs prep, u0, u1, u2 and both accumulators");
   return dispatch<Superposed>(args->grid, *args, (hipStream_t)stream);
 }
 

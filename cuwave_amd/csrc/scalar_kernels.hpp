@@ -39,10 +39,12 @@ template <typename T> struct Levels {
   T *base;
   int64_t slot;
   int slots, offset;
-  __device__ __forceinline__ T *at(int level) const {
+  __device__ __forceinline__ uint32_t slot_of(int level) const {
     int k = (level + offset) % slots;
-    k += k < 0 ? slots : 0;
-    return base + (int64_t)k * slot;
+    return (uint32_t)(k + (k < 0 ? slots : 0));
+  }
+  __device__ __forceinline__ T *at(int level) const {
+    return base + (int64_t)slot_of(level) * slot;
   }
 };
 

@@ -48,11 +48,11 @@ class Propagator:
         dtype = sim.dtype
         self.stiff = mat["stiff"]
         self.minv = mat.get("minv")
-        self.cell = torch.empty((sim.ndim, *sim.Nx_padded), dtype=dtype, device=self.device)
-        self.inv = torch.empty(sim.Nx_padded, dtype=dtype, device=self.device)
-        self.mat = _abi.Materials(
-            _abi.ptr(self.stiff), _abi.ptr(self.minv), _abi.ptr(self.cell), _abi.ptr(self.inv)
-        )
+        # the prepared materials: the harmonic cell stiffness of every axis,
+        # then the inverse inertia, one block the kernels read through one
+        # buffer resource
+        self.prep = torch.empty((sim.ndim + 1, *sim.Nx_padded), dtype=dtype, device=self.device)
+        self.mat = _abi.Materials(_abi.ptr(self.stiff), _abi.ptr(self.minv), _abi.ptr(self.prep))
         _abi.call("cw_scalar_prepare", ctypes.byref(self.grid), ctypes.byref(self.mat), _abi.stream())
         tiles = _abi.I32()
         _abi.call("cw_tile_count", ctypes.byref(self.grid), ctypes.byref(tiles))

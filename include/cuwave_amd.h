@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 2
+#define CW_ABI_VERSION 3
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -60,8 +60,8 @@ typedef struct cw_grid {
 typedef struct cw_materials {
   const void *stiff; /* nodal stiffness, its ghosts mirrored */
   const void *minv;  /* nodal inverse inertia, or NULL to derive 1 / stiff */
-  void *cell;        /* (ndim, field) harmonic cell stiffness, from prepare */
-  void *inv;         /* (field) the inverse inertia the kernels read */
+  void *prep;        /* (ndim + 1, field), from prepare: the harmonic cell
+                        stiffness of every axis, then the inverse inertia */
 } cw_materials;
 
 /* the convolutional PML behind the faces (cuwave/boundary.py:272-376) */

@@ -75,7 +75,7 @@ def test_bad_arguments_are_refused_before_any_launch():
     grid = sim.abi_grid(1)
     args = _abi.ForwardArgs()
     args.grid = grid
-    with pytest.raises(RuntimeError, match="prepared inv and cell"):
+    with pytest.raises(RuntimeError, match="prepared materials"):
         _abi.call("cw_scalar_forward", ctypes.byref(args), 0, 1, None)
     args.grid.order = 10
     with pytest.raises(RuntimeError, match="space order"):
