@@ -147,6 +147,10 @@ __device__ __forceinline__ Buf make_buf(const void *base, uint32_t bytes) {
   return {__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0,
                                             (int)bytes, 0x00020000)};
 }
+// a vector offset past every resource: the access is dropped (a read gives
+// 0), the way a lane takes part in an instruction without touching memory
+constexpr uint32_t kDrop = 0xFFFFFFF0u;
+
 template <typename T>
 __device__ __forceinline__ T bload(const Buf &b, uint32_t vo, uint32_t so);
 template <>
@@ -175,6 +179,30 @@ __device__ __forceinline__ void bstore(const Buf &b, double v, uint32_t vo,
   using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(b.r, 0, 0, 0));
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), b.r, vo, so,
                                         0);
+}
+
+// The LDS DMA of gfx950: each active lane of the wave copies SIZE (4 or 16)
+// bytes from its global address `src` to LDS at the wave-uniform `dst` plus
+// lane * SIZE, without passing through registers; completion is counted by
+// vmcnt like any vector load.
+#ifndef CW_LDS_CAST
+#define CW_LDS_CAST(p) ((__attribute__((address_space(3))) void *)(p))
+#endif
+template <int SIZE>
+__device__ __forceinline__ void lds_dma(const void *src, void *dst) {
+  static_assert(SIZE == 4 || SIZE == 16, "a lane moves 4 or 16 bytes");
+  if constexpr (SIZE == 16)
+    __builtin_amdgcn_global_load_lds(src, CW_LDS_CAST(dst), 16, 0, 0);
+  else
+    __builtin_amdgcn_global_load_lds(src, CW_LDS_CAST(dst), 4, 0, 0);
+}
+
+// s_waitcnt on vmcnt only: wait until at most n vector memory operations of
+// this wave are outstanding (gfx9 encoding: vmcnt bits 3:0 and 15:14, expcnt
+// and lgkmcnt left at their no-wait maxima)
+template <int N> __device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt is six bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
 // the sum over a node's shot lanes (a butterfly: every lane of the node ends

@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 
 #include "../../include/cuwave_amd.h"
@@ -122,10 +123,12 @@ constexpr int kChunk = 32; // rows a streaming thread marches
 // the stream box of a 2D grid: every node order / 2 + 2 off the walls, out
 // of reach of the wall mirror, the refill, the held nodes and the adjoint's
 // reflected taps; empty on other grids, on grids too small for one and
-// where two levels pass the 4 GiB a buffer resource addresses
-inline cw::Box stream_box(const cw_grid &g) {
+// where two levels pass the 4 GiB a buffer resource addresses. Its core is
+// the rectangle of blocks whose nodes neither lie in a layer slab nor reach
+// one with their transposed taps (order / 2 past the slab).
+inline cw::Box stream_box(const cw_grid &g, const cw_pml &pml) {
   cw::Box box{};
-  const int margin = g.order / 2 + 2;
+  const int R = g.order / 2, margin = R + 2;
   box.chunk = kChunk;
   if (g.ndim != 2 || g.n[0] - margin <= margin || g.n[1] - margin <= margin)
     return box;
@@ -136,19 +139,60 @@ inline cw::Box stream_box(const cw_grid &g) {
   box.y1 = g.n[0] - margin;
   box.x0 = margin;
   box.x1 = g.n[1] - margin;
+  // the rows and columns no layer term reaches: [lo, hi] per axis
+  int lo[2], hi[2];
+  for (int d = 0; d < 2; ++d) {
+    lo[d] = pml.lo[d] ? pml.lo[d] + R + 1 : 0;
+    hi[d] = pml.hi[d] ? g.n[d] - 3 - pml.hi[d] - R : g.n[d];
+  }
+  const int C = cw::kThreads / g.lanes;
+  const int xa = box.x0 / C * C;
+  const int nchunk = (box.y1 - box.y0 + box.chunk - 1) / box.chunk;
+  const int ncx = (box.x1 - xa + C - 1) / C;
+  box.c0 = box.c1 = box.k0 = box.k1 = 0;
+  int c0 = -1, c1 = -1, k0 = -1, k1 = -1;
+  for (int c = 0; c < nchunk; ++c) {
+    const int ya = box.y0 + c * box.chunk;
+    const int yb = std::min(ya + box.chunk, box.y1);
+    if (ya >= lo[0] && yb - 1 <= hi[0]) {
+      if (c0 < 0)
+        c0 = c;
+      c1 = c + 1;
+    }
+  }
+  for (int k = 0; k < ncx; ++k) {
+    const int xs = std::max(xa + k * C, box.x0);
+    const int xe = std::min(xa + (k + 1) * C, box.x1);
+    if (xs >= lo[1] && xe - 1 <= hi[1]) {
+      if (k0 < 0)
+        k0 = k;
+      k1 = k + 1;
+    }
+  }
+  if (c0 >= 0 && k0 >= 0) {
+    box.c0 = c0;
+    box.c1 = c1;
+    box.k0 = k0;
+    box.k1 = k1;
+  }
   return box;
 }
 
-// the stream blocks of a sweep, padded to a multiple of the XCD count so
-// the general blocks after them start on XCD 0
+// the stream blocks of a sweep, and of its core, each padded to a multiple
+// of the XCD count (the edge kernel's general blocks then start on XCD 0)
+inline int pad_xcds(int blocks) {
+  return (blocks + cw::kXcds - 1) / cw::kXcds * cw::kXcds;
+}
 inline int box_blocks(const cw::Box &b, int lanes) {
   if (b.y0 >= b.y1)
     return 0;
   const int cols = cw::kThreads / lanes;
   const int xa = b.x0 / cols * cols;
   const int nchunk = (b.y1 - b.y0 + b.chunk - 1) / b.chunk;
-  const int blocks = nchunk * ((b.x1 - xa + cols - 1) / cols);
-  return (blocks + cw::kXcds - 1) / cw::kXcds * cw::kXcds;
+  return pad_xcds(nchunk * ((b.x1 - xa + cols - 1) / cols));
+}
+inline int core_blocks(const cw::Box &b) {
+  return pad_xcds((b.c1 - b.c0) * (b.k1 - b.k0));
 }
 
 // where the general kernels run: the band around the box, or every node

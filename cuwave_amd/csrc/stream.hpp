@@ -5,23 +5,21 @@
 //
 // A thread is one shot lane of one column and marches a chunk of rows down
 // the slow axis, keeping the 2R + 1 values of its column that the slow-axis
-// taps read in a register queue: each row costs one new load per queued
-// field. The fast-axis taps are the same lane of the neighbouring columns,
-// each a contiguous run of the wave (common.hpp), served by L1 and L2. A
-// block is 256 / W neighbouring columns (W the shot lanes of a node, a
-// compile-time constant here), and blocks are dealt to the XCDs in
-// contiguous runs, rows of blocks across the box, so the columns whose taps
-// overlap share one L2.
+// taps read in a register queue. A block is C = 256 / W neighbouring columns
+// (W the shot lanes of a node, a compile-time constant here). Each row of
+// a tapped field reaches the block once, as one contiguous run staged in
+// LDS by the DMA loads of gfx950 (global_load_lds, 16 bytes a lane, no
+// registers on the way): C + 2R columns with the halo, in a ring of R + 2
+// rows. A step reads its fast-axis taps from LDS row y and its queue feed
+// from LDS row y + R while the DMA of row y + R + 1 and the thread's own
+// values of the next row (the centre of the overwritten slots, the
+// accumulators) are in flight; one barrier a row hands the staged row to
+// every wave. The per-node materials of a row are staged the same way.
+// That keeps the vector memory instructions a row few and wide, which is
+// what the texture path of a CU can sustain.
 //
-// Every load of a row is issued a row ahead into one of two row buffers
-// that alternate (the loop is unrolled by two, so no buffer is copied while
-// its loads are in flight), and all of them are issued on every row, the
-// last row's repeated at the end of a chunk: the compiler then waits for
-// exactly the older buffer and each wave keeps a whole row of loads in
-// flight while it computes. Fields and materials go through buffer
-// resources: one vector offset per thread for the fields and one for the
-// nodes, advanced a row at a time; a fast-axis tap is a constant in the
-// instruction, a row or a time slot one scalar offset.
+// Blocks are dealt to the XCDs in contiguous runs, rows of blocks across
+// the box, so vertically neighbouring chunks share one L2.
 //
 // A row of the slow axis is either in that axis' slab or not, so its layer
 // terms are uniform over the block; the fast axis' slab columns are fixed
@@ -35,43 +33,71 @@
 
 namespace cw {
 
-// the box of a sweep: rows [y0, y1), columns [x0, x1)
+// The box of a sweep, rows [y0, y1) and columns [x0, x1), cut into blocks
+// of `chunk` rows by C columns (the first block column starting at the
+// multiple of C below x0). Blocks [c0, c1) x [k0, k1) are the core: no
+// node of theirs has a layer term or reaches one, so the core kernel runs
+// them without any of that code; the rest, and core blocks holding points,
+// take the edge kernel.
 struct Box {
   int y0, y1, x0, x1;
-  int chunk; // rows a thread marches
+  int chunk;
+  int c0, c1, k0, k1;
 };
 
-// stream block `bid` of `nb` (a multiple of the XCD count, at least the
-// box's blocks): its rows [ya, yb) and first column xb, or false for a
-// padding block. Blocks start at multiples of their width, so no wave
-// straddles a point tile.
-__device__ __forceinline__ bool box_item(const Box &box, int cols, int bid,
-                                         int nb, int &ya, int &yb, int &xb) {
-  const int xa = box.x0 / cols * cols;
-  const int ncx = (box.x1 - xa + cols - 1) / cols;
-  const int nchunk = (box.y1 - box.y0 + box.chunk - 1) / box.chunk;
-  const int w = xcd_tile(bid, nb);
-  if (w >= nchunk * ncx)
-    return false;
-  ya = box.y0 + (w / ncx) * box.chunk;
-  yb = min(ya + box.chunk, box.y1);
-  xb = xa + (w % ncx) * cols;
-  return true;
-}
-
 // whether a point list holds a point in the tiles of rows [ya, yb) at tile
-// column tx: looked up once a chunk, so rows without points (nearly all)
-// skip the lists without waiting on a load
+// columns [tx0, tx1]: looked up once a block, so rows without points
+// (nearly all) skip the lists without waiting on a load
 template <typename T>
 __device__ __forceinline__ bool chunk_has(const Points<T> &pts,
                                           const Geom<T> &g, int ya, int yb,
-                                          int tx) {
+                                          int tx0, int tx1) {
   if (!pts.starts)
     return false;
   int any = 0;
   for (int ty = ya / TileShape<2>::TY; ty <= (yb - 1) / TileShape<2>::TY; ++ty)
-    any |= pts.starts[ty * g.tf + tx + 1] - pts.starts[ty * g.tf + tx];
+    any |= pts.starts[ty * g.tf + tx1 + 1] - pts.starts[ty * g.tf + tx0];
   return __builtin_amdgcn_readfirstlane(any) != 0;
+}
+
+// stream block `bid` of `nb` (a multiple of the XCD count) of the core or
+// of the edge kernel: its rows [ya, yb) and first column xb, or false for a
+// block the other kernel runs (or a padding block). A block starts at a
+// multiple of its width, so no wave straddles a point tile.
+template <bool EDGE, typename T>
+__device__ __forceinline__ bool box_item(const Box &box, const Geom<T> &g,
+                                         const Points<T> &src, bool use_src,
+                                         const Points<T> &rec, int cols,
+                                         int bid, int nb, int &ya, int &yb,
+                                         int &xb) {
+  const int xa = box.x0 / cols * cols;
+  const int ncx = (box.x1 - xa + cols - 1) / cols;
+  const int nchunk = (box.y1 - box.y0 + box.chunk - 1) / box.chunk;
+  const int w = xcd_tile(bid, nb);
+  int c, k;
+  if (EDGE) {
+    if (w >= nchunk * ncx)
+      return false;
+    c = w / ncx;
+    k = w % ncx;
+  } else {
+    const int kw = box.k1 - box.k0;
+    if (w >= (box.c1 - box.c0) * kw)
+      return false;
+    c = box.c0 + w / kw;
+    k = box.k0 + w % kw;
+  }
+  ya = box.y0 + c * box.chunk;
+  yb = min(ya + box.chunk, box.y1);
+  xb = xa + k * cols;
+  const bool core = c >= box.c0 && c < box.c1 && k >= box.k0 && k < box.k1;
+  if (EDGE && !core)
+    return true;
+  const int tx0 = xb / TileShape<2>::TX;
+  const int tx1 = min(xb + cols - 1, g.np[1] - 1) / TileShape<2>::TX;
+  const bool points = (use_src && chunk_has(src, g, ya, yb, tx0, tx1)) ||
+                      chunk_has(rec, g, ya, yb, tx0, tx1);
+  return EDGE == points;
 }
 
 // the profile of axis d at node a and at the half point below it
@@ -98,133 +124,163 @@ __device__ __forceinline__ void row_line(T *t, const T *x, T c) {
 
 constexpr int log2_of(int w) { return w <= 1 ? 0 : 1 + log2_of(w / 2); }
 
-// The offsets of a thread's march over its W-lane columns. `f` is the byte
-// offset of the field element (y, x - R) of its lane, `n` that of the node
-// (y - 1, x - R) and `m` the same node's in an int32 map, for the block's
-// first row; each row adds frow, nrow, mrow. Fast-axis taps are the
-// constants fx(k), nx(k), mx(k).
-template <typename T, int R, int W> struct March {
-  static constexpr uint32_t Z = sizeof(T);
+// Copy `bytes` contiguous bytes from global `src` to LDS `dst` by DMA, the
+// block's four waves taking 64-lane chunks in turn from wave `first`; lanes
+// past the end stay idle. SIZE is the bytes a lane moves (4 or 16).
+template <int SIZE>
+__device__ __forceinline__ void dma_block(const char *src, char *dst,
+                                          int bytes, int first) {
+  const int wave = ((int)(threadIdx.x >> 6) - first) & 3;
+  const int lane = threadIdx.x & 63;
+  constexpr int CH = 64 * SIZE;
+  for (int c = wave * CH; c < bytes; c += 4 * CH)
+    if (c + lane * SIZE < bytes)
+      lds_dma<SIZE>(src + c + lane * SIZE, dst + c);
+}
+
+// The staging geometry of a block of a W-lane field: C columns, each row of
+// the field staged with an R-column halo on both sides (SEGC columns, SEGB
+// bytes) in a ring of R + 2 rows: while a step reads rows y .. y + R (the
+// fast-axis taps of row y and the queue feed of row y + R), row y + R + 1
+// arrives. DMA moves 16 bytes a lane where a node's lanes allow.
+template <typename T, int R, int W> struct Stage {
   static constexpr int SH = log2_of(W);
-  uint32_t f, n, m;
-  uint32_t frow, nrow, mrow;
-  static constexpr uint32_t fx(int k) { return (uint32_t)((k + R) * W) * Z; }
-  static constexpr uint32_t nx(int k) { return (uint32_t)(k + R) * Z; }
-  static constexpr uint32_t mx(int k) { return (uint32_t)(k + R) * 4u; }
+  static constexpr int C = kThreads / W;
+  static constexpr int SEGC = C + 2 * R;
+  static constexpr int SEGB = SEGC * W * (int)sizeof(T);
+  static constexpr int RING = R + 2;
+  static constexpr int DMA = (W * sizeof(T)) % 16 == 0 ? 16 : 4;
 };
 
+// stage row y of a W-lane field level `f` (np1 the padded row length):
+// nodes xb - R .. xb + C + R - 1 of the row, clipped at the row's end
 template <typename T, int R, int W>
-__device__ __forceinline__ March<T, R, W> march_at(const Geom<T> &g, int ya,
-                                                   int xl, int b) {
-  using M = March<T, R, W>;
-  M m;
-  const uint32_t s0 = (uint32_t)g.s[0];
-  const uint32_t node = (uint32_t)ya * s0 + (uint32_t)(xl - R);
-  m.f = ((node << M::SH) + (uint32_t)b) * M::Z;
-  m.n = (node - s0) * M::Z;
-  m.m = (node - s0) * 4u;
-  m.frow = (s0 << M::SH) * M::Z;
-  m.nrow = s0 * M::Z;
-  m.mrow = s0 * 4u;
-  return m;
+__device__ __forceinline__ void stage_row(const T *f, int y, int xb, int s0,
+                                          int np1, char *ring, int first) {
+  using S = Stage<T, R, W>;
+  const int end = min(xb + S::C + R, np1);
+  dma_block<S::DMA>((const char *)(f + ((int64_t)y * s0 + xb - R) * W),
+                    ring + (y % S::RING) * S::SEGB,
+                    (end - (xb - R)) * W * (int)sizeof(T), first);
+}
+
+// stage `count` node values from node `node` of a per-node array (clipped
+// at the row end `lim`, exclusive, counted from `node`)
+template <typename T>
+__device__ __forceinline__ void stage_nodes(const T *a, int64_t node, int count,
+                                            int lim, T *dst, int first) {
+  dma_block<4>((const char *)(a + node), (char *)dst,
+               min(count, lim) * (int)sizeof(T), first);
 }
 
 // --------------------------------------------------------------- forward
-template <typename T, int R> struct FwdRow {
-  T feed;       // u^(t-1) at (y + R, x): the queue's new entry
-  T x[2 * R];   // u^(t-1) at (y, x -+ k)
-  T u0, mi, gp1, gm1, gp0;
-  int col;      // the node's strip column (or -1)
-};
-
-template <typename T, int R, int W>
+template <typename T, int R, int W, bool EDGE>
 __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
                                                const Box &box, const int t,
                                                int bid, int nb) {
-  using Mch = March<T, R, W>;
-  constexpr int SH = Mch::SH;
+  using S = Stage<T, R, W>;
+  constexpr int SH = S::SH, C = S::C;
   const Geom<T> &g = p.g;
   int ya, yb, xb;
-  if (!box_item(box, kThreads / W, bid, nb, ya, yb, xb))
+  if (!box_item<EDGE>(box, g, p.src, true, p.rec, C, bid, nb, ya, yb, xb))
     return;
+  // the staged rows of u^(t-1), and per material row slot (two alternating)
+  // the slow-axis cells [C], the fast-axis cells of x - 1 .. [C + 1], 1 / m
+  __shared__ __align__(16) char ring[S::RING * S::SEGB];
+  __shared__ __align__(16) T lmat[2][3][C + 1];
   const int b = threadIdx.x & (W - 1);
-  const int x = xb + (int)(threadIdx.x >> SH);
+  const int xc = (int)(threadIdx.x >> SH);
+  const int x = xb + xc;
   const bool active = x >= box.x0 && x < box.x1;
-  const int xl = min(max(x, box.x0), box.x1 - 1); // every thread loads
+  const int xl = min(max(x, box.x0), box.x1 - 1);
   const int tx = __builtin_amdgcn_readfirstlane(xl) / TileShape<2>::TX;
+  const int s0 = g.s[0], np1 = g.np[1];
   const int N0 = g.n[0], N1 = g.n[1];
   const int L0 = p.pml.lo[0], H0 = p.pml.hi[0];
   const int L1 = p.pml.lo[1], H1 = p.pml.hi[1];
   const int64_t size0 = ((int64_t)(L0 + H0) * g.np[1]) << SH;
   const int64_t size1 = ((int64_t)(L1 + H1) * g.np[0]) << SH;
-  const int prow1 = L1 + H1 ? slab_row(xl, N1, L1, H1) : -1;
+  const int prow1 = EDGE && L1 + H1 ? slab_row(xl, N1, L1, H1) : -1;
   Prof<T> pf1{};
   if (prow1 >= 0)
     pf1 = profile_at(p.pml.profile[1], N1, xl);
+  const T *u1 = p.u.at(t - 1);
   const uint32_t LB = (uint32_t)((g.nodes << SH) * sizeof(T));
-  const uint32_t NB = (uint32_t)(g.nodes * sizeof(T));
-  const Buf U1 = make_buf(p.u.at(t - 1), LB);
   const Buf U0 = make_buf(p.u.at(t - 2), LB);
   const Buf U2 = make_buf(p.u.at(t), LB);
-  // the prepared materials: cell means of the slow axis, of the fast axis,
-  // the inverse inertia
-  const Buf MAT = make_buf(p.cell, 3 * NB);
-  const Buf MAP =
-      make_buf(p.strip.map, p.strip.map ? (uint32_t)(g.nodes * 4) : 0u);
-  const Mch M = march_at<T, R, W>(g, ya, xl, b);
-  const uint32_t feed = R * M.frow;     // the field row y + R
-  const uint32_t cy = M.nrow;           // node row y, slow-axis cells
-  const uint32_t cx = NB + M.nrow;      // fast-axis cells
-  const uint32_t inv = 2 * NB + M.nrow; // inverse inertia
-  const bool has_src = chunk_has(p.src, g, ya, yb, tx);
-  const bool has_rec = chunk_has(p.rec, g, ya, yb, tx);
+  const Buf MAP = make_buf(p.strip.map, EDGE && p.strip.map
+                                            ? (uint32_t)(g.nodes * 4)
+                                            : 0u);
+  const T *cy = p.cell, *cx = p.cell + g.nodes, *inv = p.cell + 2 * g.nodes;
+  // this thread's element and node at row ya, and their row strides
+  const uint32_t frow = ((uint32_t)s0 << SH) * sizeof(T);
+  const uint32_t e0 = ((((uint32_t)ya * s0 + xl) << SH) + b) * sizeof(T);
+  const uint32_t m0 = ((uint32_t)ya * s0 + xl) * 4u, mrow = (uint32_t)s0 * 4u;
+  const bool has_src = EDGE && chunk_has(p.src, g, ya, yb, tx, tx);
+  const bool has_rec = EDGE && chunk_has(p.rec, g, ya, yb, tx, tx);
+  const int lane0 = ((xc + R) * W + b); // this thread in a staged row
 
-  auto load = [&](FwdRow<T, R> &r, int y) {
-    const uint32_t f = M.f + (uint32_t)(y - ya) * M.frow;
-    const uint32_t n = M.n + (uint32_t)(y - ya) * M.nrow;
-    r.feed = bload<T>(U1, f + Mch::fx(0), feed);
-#pragma unroll
-    for (int k = 1; k <= R; ++k) {
-      r.x[R - k] = bload<T>(U1, f + Mch::fx(-k), 0);
-      r.x[R - 1 + k] = bload<T>(U1, f + Mch::fx(k), 0);
-    }
-    r.u0 = bload<T>(U0, f + Mch::fx(0), 0);
-    r.mi = bload<T>(MAT, n + Mch::nx(0), inv);
-    r.gp1 = bload<T>(MAT, n + Mch::nx(0), cx);
-    r.gm1 = bload<T>(MAT, n + Mch::nx(-1), cx);
-    r.gp0 = bload<T>(MAT, n + Mch::nx(0), cy);
-    r.col = bload_int(MAP, M.m + (uint32_t)(y - ya) * M.mrow + Mch::mx(0),
-                      M.mrow);
+  auto stage_mat = [&](int y) {
+    const int64_t r = (int64_t)y * s0;
+    stage_nodes(cy, r + xb, C, np1 - xb, lmat[y & 1][0], 1);
+    stage_nodes(cx, r + xb - 1, C + 1, np1 - xb + 1, lmat[y & 1][1], 2);
+    stage_nodes(inv, r + xb, C, np1 - xb, lmat[y & 1][2], 3);
   };
 
-  // the queue's first rows ya - R .. ya + R - 1 (every offset non-negative:
-  // the box starts R + 2 rows down)
+  // the queue's first rows ya - R .. ya + R - 1, straight from memory
   T q[2 * R + 1];
+  {
+    const Buf U1 = make_buf(u1, LB);
 #pragma unroll
-  for (int j = 0; j < 2 * R; ++j)
-    q[j] = bload<T>(U1, M.f + Mch::fx(0) + (uint32_t)j * M.frow - R * M.frow,
-                    0);
-  T gm0 = bload<T>(MAT, M.n + Mch::nx(0), 0);
+    for (int j = 0; j < 2 * R; ++j)
+      q[j] = bload<T>(U1, e0 + (uint32_t)j * frow - R * frow, 0);
+  }
+  T gm0 = cy[(int64_t)(ya - 1) * s0 + xl];
+  for (int y = ya; y <= ya + R; ++y)
+    stage_row<T, R, W>(u1, y, xb, s0, np1, ring, 0);
+  stage_mat(ya);
+  T u0n = bload<T>(U0, e0, 0);
+  int coln = EDGE ? bload_int(MAP, m0, 0) : -1;
+  wait_vmcnt<0>();
+  __syncthreads();
 
-  auto step = [&](const FwdRow<T, R> &r, int y) {
-    q[2 * R] = r.feed;
+  for (int y = ya; y < yb; ++y) {
+    // the next row's loads, in this order (the wait at the end counts on
+    // it): its staged rows, then this thread's own values
+    stage_row<T, R, W>(u1, y + R + 1, xb, s0, np1, ring, 0);
+    stage_mat(y + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    const T u0c = u0n;
+    const int col = coln;
+    u0n = bload<T>(U0, e0 + (uint32_t)(y + 1 - ya) * frow, 0);
+    if (EDGE)
+      coln = bload_int(MAP, m0 + (uint32_t)(y + 1 - ya) * mrow, 0);
+    __builtin_amdgcn_sched_barrier(0);
+
+    const T *row = (const T *)(ring + (y % S::RING) * S::SEGB) + lane0;
+    q[2 * R] = ((const T *)(ring + ((y + R) % S::RING) * S::SEGB))[lane0];
+    const T gp0 = lmat[y & 1][0][xc];
+    T v = 0;
     if (active) {
       T tf[2 * R + 1];
-      row_line<T, R>(tf, r.x, q[R]);
-      const T mi = r.mi;
+#pragma unroll
+      for (int k = -R; k <= R; ++k)
+        tf[k + R] = k ? row[k * W] : q[R];
+      const T mi = lmat[y & 1][2][xc];
+      const T gp1 = lmat[y & 1][1][xc + 1], gm1 = lmat[y & 1][1][xc];
       T div[2];
-      div[0] = flux<T, R>(q, r.gp0, gm0, g.op);
-      div[1] = flux<T, R>(tf, r.gp1, r.gm1, g.op);
+      div[0] = flux<T, R>(q, gp0, gm0, g.op);
+      div[1] = flux<T, R>(tf, gp1, gm1, g.op);
       const T lap = combine_axes<T, 2, R>(div, g.f);
-      T v = -r.u0 + (T)2 * q[R] + mi * lap;
-      const int prow0 = L0 + H0 ? slab_row(y, N0, L0, H0) : -1;
+      v = -u0c + (T)2 * q[R] + mi * lap;
+      const int prow0 = EDGE && L0 + H0 ? slab_row(y, N0, L0, H0) : -1;
       if (prow0 >= 0) {
         const Prof<T> pf = profile_at(p.pml.profile[0], N0, y);
         T Dp, Dm;
         cell_diffs<T, R>(q, g.op, Dp, Dm);
         v += pml_forward(p.pml.aux[0] +
                              (((int64_t)prow0 * g.np[1] + x) << SH) + b,
-                         size0, Dp, Dm, r.gp0, gm0, g.f[0], mi, pf.an, pf.bn,
+                         size0, Dp, Dm, gp0, gm0, g.f[0], mi, pf.an, pf.bn,
                          pf.ah, pf.bh, pf.ah1, pf.bh1);
       }
       if (prow1 >= 0) {
@@ -232,200 +288,218 @@ __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
         cell_diffs<T, R>(tf, g.op, Dp, Dm);
         v += pml_forward(p.pml.aux[1] +
                              (((int64_t)y * (L1 + H1) + prow1) << SH) + b,
-                         size1, Dp, Dm, r.gp1, r.gm1, g.f[1], mi, pf1.an,
-                         pf1.bn, pf1.ah, pf1.bh, pf1.ah1, pf1.bh1);
+                         size1, Dp, Dm, gp1, gm1, g.f[1], mi, pf1.an, pf1.bn,
+                         pf1.ah, pf1.bh, pf1.ah1, pf1.bh1);
       }
-      const int nc = y * g.s[0] + x;
+      const int nc = y * s0 + x;
       const int tile = (y / TileShape<2>::TY) * g.tf + tx;
       if (has_src)
         for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
           if (p.src.takes(k, nc, b))
             v += p.src.weight[k] * p.signal[p.src.at(t, k, b)];
-      bstore(U2, v, M.f + (uint32_t)(y - ya) * M.frow + Mch::fx(0), 0);
       if (has_rec)
         for (int k = p.rec.starts[tile]; k < p.rec.starts[tile + 1]; ++k)
           if (p.rec.takes(k, nc, b))
             p.record[p.rec.at(t, k, b)] = v;
-      if ((prow0 >= 0 || prow1 >= 0) && p.strip.map) {
-        const int col = r.col;
-        if (col >= 0) {
-          const int64_t at =
-              (((int64_t)(t - p.strip.first) * p.strip.count + col) << SH) + b;
-          if (p.strip.bf16)
-            ((uint16_t *)p.strip.record)[at] = bf16_bits((float)v);
-          else
-            ((T *)p.strip.record)[at] = v;
-        }
+      if ((prow0 >= 0 || prow1 >= 0) && p.strip.map && col >= 0) {
+        const int64_t at =
+            (((int64_t)(t - p.strip.first) * p.strip.count + col) << SH) + b;
+        if (p.strip.bf16)
+          ((uint16_t *)p.strip.record)[at] = bf16_bits((float)v);
+        else
+          ((T *)p.strip.record)[at] = v;
       }
     }
-    gm0 = r.gp0;
+    // every thread stores (off the box, out of range: dropped), so the
+    // count of operations after the staging copies is fixed
+    bstore(U2, v, active ? e0 + (uint32_t)(y - ya) * frow : kDrop, 0);
+    gm0 = gp0;
 #pragma unroll
     for (int j = 0; j < 2 * R; ++j)
       q[j] = q[j + 1];
-  };
-
-  FwdRow<T, R> A, B;
-  load(A, ya);
-  for (int y = ya; y < yb; y += 2) {
-    load(B, min(y + 1, yb - 1));
-    step(A, y);
-    load(A, min(y + 2, yb - 1));
-    if (y + 1 < yb)
-      step(B, y + 1);
+    // the next row staged: this wave's copies (its own loads and the store
+    // may still run), then every wave's
+    wait_vmcnt<(EDGE ? 2 : 1) + 1>();
+    __syncthreads();
   }
 }
 
 // --------------------------------------------------------------- adjoint
-template <typename T, int R> struct BwdRow {
-  T lfeed, ufeed;   // lambda^(n+1), u^n at (y + R, x)
-  T lx[2 * R], ux[2 * R]; // their fast-axis taps at row y
-  T cx[2 * R];      // fast-axis cell means at x - R .. x + R - 1
-  T cfeed;          // slow-axis cell mean of row y + R - 1
-  T lo, uo, mi;     // lambda^(n+2), u^(n+1), 1 / m at (y, x)
-  T accm, accs;     // the gradient accumulators at the node
-  T sk[5];          // stiffness at the node and its four neighbours
-  int col;          // the node's strip column (or -1)
-};
-
-template <typename T, int R, int W, bool STIFF>
+template <typename T, int R, int W, bool STIFF, bool EDGE>
 __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
                                                 const Box &box, const int n,
                                                 int bid, int nb) {
-  using Mch = March<T, R, W>;
-  constexpr int SH = Mch::SH;
+  using S = Stage<T, R, W>;
+  constexpr int SH = S::SH, C = S::C;
   const Geom<T> &g = p.g;
+  const bool reverse = p.reverse && n <= p.steps - 2;
   int ya, yb, xb;
-  if (!box_item(box, kThreads / W, bid, nb, ya, yb, xb))
+  if (!box_item<EDGE>(box, g, p.src, reverse, p.rec, C, bid, nb, ya, yb, xb))
     return;
+  // the staged rows of lambda^(n+1) and u^n; per material row slot the
+  // fast-axis cells of x - R .. [C + 2R], 1 / m [C], the slow-axis cell of
+  // row y + R - 1 [C], the accumulators [C] each
+  __shared__ __align__(16) char lring[S::RING * S::SEGB];
+  __shared__ __align__(16) char uring[S::RING * S::SEGB];
+  __shared__ __align__(16) T lmat[2][5][C + 2 * R];
   const int b = threadIdx.x & (W - 1);
-  const int x = xb + (int)(threadIdx.x >> SH);
+  const int xc = (int)(threadIdx.x >> SH);
+  const int x = xb + xc;
   const bool active = x >= box.x0 && x < box.x1;
   const int xl = min(max(x, box.x0), box.x1 - 1);
   const int tx = __builtin_amdgcn_readfirstlane(xl) / TileShape<2>::TX;
+  const int s0 = g.s[0], np1 = g.np[1];
   const int N0 = g.n[0], N1 = g.n[1];
   const int L0 = p.pml.lo[0], H0 = p.pml.hi[0];
   const int L1 = p.pml.lo[1], H1 = p.pml.hi[1];
   const int64_t size0 = ((int64_t)(L0 + H0) * g.np[1]) << SH;
   const int64_t size1 = ((int64_t)(L1 + H1) * g.np[0]) << SH;
   const int m = p.steps - 1 - n;
-  const bool reverse = p.reverse && n <= p.steps - 2;
   const bool chain = W == 1 && !p.combine;
   // the column: its slab row along the fast axis, whether the transposed
   // taps of that axis reach the slab, its profile
-  const int prow1 = L1 + H1 ? slab_row(xl, N1, L1, H1) : -1;
-  const bool ext1 = (L1 && xl - 1 < L1 + R) || (H1 && N1 - 2 - xl < H1 + R);
+  const int prow1 = EDGE && L1 + H1 ? slab_row(xl, N1, L1, H1) : -1;
+  const bool ext1 =
+      EDGE && ((L1 && xl - 1 < L1 + R) || (H1 && N1 - 2 - xl < H1 + R));
   Prof<T> pf1{};
   if (prow1 >= 0)
     pf1 = profile_at(p.pml.profile[1], N1, xl);
+  const T *l1 = p.lam.at(n + 1), *u1 = p.u.at(n);
   const uint32_t LB = (uint32_t)((g.nodes << SH) * sizeof(T));
   const uint32_t NB = (uint32_t)(g.nodes * sizeof(T));
-  // lambda: both slots under one resource, each level a slot offset; u the
-  // same when the reverse march writes its second slot, else one level
-  const uint32_t l1s = p.lam.slot_of(n + 1) * LB, l0s = p.lam.slot_of(n) * LB;
-  const Buf LAM = make_buf(p.lam.base, 2 * LB);
-  const Buf U = reverse ? make_buf(p.u.base, 2 * LB) : make_buf(p.u.at(n), LB);
-  const uint32_t u1s = reverse ? p.u.slot_of(n) * LB : 0u;
-  const uint32_t uos = reverse ? p.u.slot_of(n + 1) * LB : 0u;
-  const Buf MAT = make_buf(p.cell, 3 * NB);
+  const Buf L0b = make_buf(p.lam.at(n), LB);
+  // without the reverse march u^(n + 1) is neither read nor written: an
+  // empty resource turns its loads into zeros that touch no memory
+  const Buf UOb = make_buf(p.u.at(n + 1), reverse ? LB : 0u);
   const Buf GM = make_buf(p.gmass, NB);
   const Buf GS = make_buf(p.gstiff, p.combine ? 0u : NB);
   const Buf SK = make_buf(p.stiff, STIFF ? NB : 0u);
-  const Buf MAP = make_buf(p.strip.map, reverse && p.strip.map
+  const Buf MAP = make_buf(p.strip.map, EDGE && reverse && p.strip.map
                                             ? (uint32_t)(g.nodes * 4)
                                             : 0u);
-  const Mch M = march_at<T, R, W>(g, ya, xl, b);
-  const uint32_t feed = R * M.frow;
-  const uint32_t cy = M.nrow, cx = NB + M.nrow, inv = 2 * NB + M.nrow;
-  const uint32_t cfeed = R * M.nrow; // slow-axis cell of row y + R - 1
-  const bool has_rec = chunk_has(p.rec, g, ya, yb, tx);
-  const bool has_src = reverse && chunk_has(p.src, g, ya, yb, tx);
+  const T *cy = p.cell, *cx = p.cell + g.nodes, *inv = p.cell + 2 * g.nodes;
+  const uint32_t frow = ((uint32_t)s0 << SH) * sizeof(T);
+  const uint32_t e0 = ((((uint32_t)ya * s0 + xl) << SH) + b) * sizeof(T);
+  const uint32_t n0 = ((uint32_t)ya * s0 + xl) * sizeof(T);
+  const uint32_t nrow = (uint32_t)s0 * sizeof(T);
+  const uint32_t m0 = ((uint32_t)ya * s0 + xl) * 4u, mrow = (uint32_t)s0 * 4u;
+  const bool has_rec = EDGE && chunk_has(p.rec, g, ya, yb, tx, tx);
+  const bool has_src = EDGE && reverse && chunk_has(p.src, g, ya, yb, tx, tx);
+  const int lane0 = ((xc + R) * W + b);
 
-  auto load = [&](BwdRow<T, R> &r, int y) {
-    const uint32_t f = M.f + (uint32_t)(y - ya) * M.frow;
-    const uint32_t nn = M.n + (uint32_t)(y - ya) * M.nrow;
-    r.lfeed = bload<T>(LAM, f + Mch::fx(0), l1s + feed);
-    r.ufeed = bload<T>(U, f + Mch::fx(0), u1s + feed);
-#pragma unroll
-    for (int k = 1; k <= R; ++k) {
-      r.lx[R - k] = bload<T>(LAM, f + Mch::fx(-k), l1s);
-      r.lx[R - 1 + k] = bload<T>(LAM, f + Mch::fx(k), l1s);
-      r.ux[R - k] = bload<T>(U, f + Mch::fx(-k), u1s);
-      r.ux[R - 1 + k] = bload<T>(U, f + Mch::fx(k), u1s);
-    }
-#pragma unroll
-    for (int k = 0; k < 2 * R; ++k)
-      r.cx[k] = bload<T>(MAT, nn + Mch::nx(k - R), cx);
-    r.cfeed = bload<T>(MAT, nn + Mch::nx(0), cfeed);
-    r.lo = bload<T>(LAM, f + Mch::fx(0), l0s);
-    r.uo = bload<T>(U, f + Mch::fx(0), uos);
-    r.mi = bload<T>(MAT, nn + Mch::nx(0), inv);
-    r.accm = bload<T>(GM, nn + Mch::nx(0), M.nrow);
-    r.accs = bload<T>(GS, nn + Mch::nx(0), M.nrow);
+  auto stage_mat = [&](int y) {
+    const int64_t r = (int64_t)y * s0;
+    T(*s)[C + 2 * R] = lmat[y & 1];
+    stage_nodes(cx, r + xb - R, C + 2 * R, np1 - xb + R, s[0], 2);
+    stage_nodes(inv, r + xb, C, np1 - xb, s[1], 3);
+    stage_nodes(cy, r + (int64_t)(R - 1) * s0 + xb, C, np1 - xb, s[2], 0);
+  };
+  // the thread's own values of a row, one row ahead
+  struct Own {
+    T lo, uo, accm, accs, sk[5];
+    int col;
+  };
+  auto own = [&](Own &o, int y) {
+    const uint32_t e = e0 + (uint32_t)(y - ya) * frow;
+    const uint32_t nn = n0 + (uint32_t)(y - ya) * nrow;
+    o.lo = bload<T>(L0b, e, 0);
+    o.uo = bload<T>(UOb, e, 0);
+    o.accm = bload<T>(GM, nn, 0);
+    o.accs = bload<T>(GS, nn, 0);
     if (STIFF) {
-      r.sk[0] = bload<T>(SK, nn + Mch::nx(0), M.nrow);
-      r.sk[1] = bload<T>(SK, nn + Mch::nx(0), 2 * M.nrow);
-      r.sk[2] = bload<T>(SK, nn + Mch::nx(0), 0);
-      r.sk[3] = bload<T>(SK, nn + Mch::nx(1), M.nrow);
-      r.sk[4] = bload<T>(SK, nn + Mch::nx(-1), M.nrow);
+      o.sk[0] = bload<T>(SK, nn, 0);
+      o.sk[1] = bload<T>(SK, nn + nrow, 0);
+      o.sk[2] = bload<T>(SK, nn - nrow, 0);
+      o.sk[3] = bload<T>(SK, nn + (uint32_t)sizeof(T), 0);
+      o.sk[4] = bload<T>(SK, nn - (uint32_t)sizeof(T), 0);
     }
-    r.col = bload_int(MAP, M.m + (uint32_t)(y - ya) * M.mrow + Mch::mx(0),
-                      M.mrow);
+    o.col = EDGE ? bload_int(MAP, m0 + (uint32_t)(y - ya) * mrow, 0) : -1;
   };
 
   // queues along the slow axis: lambda^(n + 1) and u^n rows y - R .. y + R,
-  // the cell means of rows y - R .. y + R - 1
+  // the cell means of rows y - R .. y + R - 1; the first rows from memory
   T ql[2 * R + 1], qu[2 * R + 1], qc[2 * R];
+  {
+    const Buf L1b = make_buf(l1, LB), U1b = make_buf(u1, LB);
 #pragma unroll
-  for (int j = 0; j < 2 * R; ++j) {
-    // rows ya - R + j (the box starts R + 2 rows down: offsets stay positive)
-    const uint32_t f = M.f + Mch::fx(0) + (uint32_t)j * M.frow - R * M.frow;
-    ql[j] = bload<T>(LAM, f, l1s);
-    qu[j] = bload<T>(U, f, u1s);
-    if (j < 2 * R - 1)
-      qc[j] = bload<T>(MAT, M.n + Mch::nx(0) + (uint32_t)(j + 1) * M.nrow -
-                                R * M.nrow,
-                       0);
+    for (int j = 0; j < 2 * R; ++j) {
+      ql[j] = bload<T>(L1b, e0 + (uint32_t)j * frow - R * frow, 0);
+      qu[j] = bload<T>(U1b, e0 + (uint32_t)j * frow - R * frow, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 2 * R - 1; ++j)
+      qc[j] = cy[(int64_t)(ya - R + j) * s0 + xl];
   }
+  for (int y = ya; y <= ya + R; ++y) {
+    stage_row<T, R, W>(l1, y, xb, s0, np1, lring, 0);
+    stage_row<T, R, W>(u1, y, xb, s0, np1, uring, 2);
+  }
+  stage_mat(ya);
+  Own on;
+  own(on, ya);
+  wait_vmcnt<0>();
+  __syncthreads();
 
-  auto step = [&](const BwdRow<T, R> &r, int y) {
-    ql[2 * R] = r.lfeed;
-    qu[2 * R] = r.ufeed;
-    qc[2 * R - 1] = r.cfeed;
+  for (int y = ya; y < yb; ++y) {
+    // the next row's loads, in this order (the wait at the end counts on
+    // it): its staged rows, then this thread's own values
+    stage_row<T, R, W>(l1, y + R + 1, xb, s0, np1, lring, 0);
+    stage_row<T, R, W>(u1, y + R + 1, xb, s0, np1, uring, 2);
+    stage_mat(y + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    const Own o = on;
+    own(on, y + 1);
+    __builtin_amdgcn_sched_barrier(0);
+
+    const T(*mt)[C + 2 * R] = lmat[y & 1];
+    const int ls = (y % S::RING) * S::SEGB, fs = ((y + R) % S::RING) * S::SEGB;
+    ql[2 * R] = ((const T *)(lring + fs))[lane0];
+    qu[2 * R] = ((const T *)(uring + fs))[lane0];
+    qc[2 * R - 1] = mt[2][xc];
+    const uint32_t e = e0 + (uint32_t)(y - ya) * frow;
+    const uint32_t nn = n0 + (uint32_t)(y - ya) * nrow;
+    T ln = 0, v = 0, gm_new = 0, gs_new = 0;
+    bool gstore = false;
     if (active) {
-      const uint32_t f = M.f + (uint32_t)(y - ya) * M.frow + Mch::fx(0);
-      const int nc = y * g.s[0] + x;
+      const int nc = y * s0 + x;
+      const T *lrow = (const T *)(lring + ls) + lane0;
+      const T *urow = (const T *)(uring + ls) + lane0;
       T lt1[2 * R + 1], ut1[2 * R + 1], chi1[R], clo1[R], chi0[R], clo0[R];
-      row_line<T, R>(lt1, r.lx, ql[R]);
-      row_line<T, R>(ut1, r.ux, qu[R]);
+#pragma unroll
+      for (int k = -R; k <= R; ++k) {
+        lt1[k + R] = k ? lrow[k * W] : ql[R];
+        ut1[k + R] = k ? urow[k * W] : qu[R];
+      }
 #pragma unroll
       for (int j = 1; j <= R; ++j) {
-        chi1[j - 1] = r.cx[R + j - 1];
-        clo1[j - 1] = r.cx[R - j];
+        chi1[j - 1] = mt[0][xc + R + j - 1];
+        clo1[j - 1] = mt[0][xc + R - j];
         chi0[j - 1] = qc[R + j - 1];
         clo0[j - 1] = qc[R - j];
       }
-      const T mi = r.mi, lo_ = r.lo;
+      const T mi = mt[1][xc], lo_ = o.lo;
       // this row along the slow axis: slab row, reach of the transposed taps
-      const int prow0 = L0 + H0 ? slab_row(y, N0, L0, H0) : -1;
-      const bool ext0 = (L0 && y - 1 < L0 + R) || (H0 && N0 - 2 - y < H0 + R);
+      const int prow0 = EDGE && L0 + H0 ? slab_row(y, N0, L0, H0) : -1;
+      const bool ext0 =
+          EDGE && ((L0 && y - 1 < L0 + R) || (H0 && N0 - 2 - y < H0 + R));
       const bool layer = prow0 >= 0 || prow1 >= 0;
       const bool grad = !(p.skip_layer && layer);
-      const T accm = r.accm, accs = r.accs;
+      const T accm = o.accm, accs = o.accs;
       const T lc = ql[R], uc = qu[R];
       T div_l = 0;
       div_l += g.f[0] * transposed<T, R>(ql, chi0, clo0, g.op);
       div_l += g.f[1] * transposed<T, R>(lt1, chi1, clo1, g.op);
-      T ln = -lo_ + (T)2 * lc + mi * div_l;
+      ln = -lo_ + (T)2 * lc + mi * div_l;
       // the inertia density by parts in time (scalar_sensitivity.cu:512-514)
       T cm = chain ? accm : (T)0;
       cm = cm - p.mf * uc * (ln - (T)2 * lc + lo_);
       T cs = chain ? accs : (T)0;
       if (STIFF && grad) {
         T gsum = 0;
-        gsum += stiffness_density<T, R>(qu, ql, lc, r.sk[0], r.sk[1], r.sk[2],
+        gsum += stiffness_density<T, R>(qu, ql, lc, o.sk[0], o.sk[1], o.sk[2],
                                         g.f[0], g.op);
-        gsum += stiffness_density<T, R>(ut1, lt1, lc, r.sk[0], r.sk[3],
-                                        r.sk[4], g.f[1], g.op);
+        gsum += stiffness_density<T, R>(ut1, lt1, lc, o.sk[0], o.sk[3],
+                                        o.sk[4], g.f[1], g.op);
         cs = cs - p.mf * gsum;
       }
       // the transposed layer: the slab nodes' tap fields within reach
@@ -459,7 +533,6 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
             ln += load;
             cm += -p.mf * uc * load;
           }
-      bstore(LAM, ln, f, l0s);
       // the memory the next level gathers, from this level's final lambda
       if (prow0 >= 0) {
         const Prof<T> pf0 = profile_at(p.pml.profile[0], N0, y);
@@ -476,11 +549,23 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
                            clo1[0], g.f[1], pf1.an, pf1.bn, pf1.ah, pf1.bh,
                            pf1.ah1, pf1.bh1);
       }
-      if (grad)
-        add_gradient(p, nc, b, accm, accs, cm, cs);
+      if (grad) {
+        // the gradient as add_gradient forms it, stored below
+        if (p.combine) {
+          gm_new = accm + shot_sum(p.jm * cm + p.js * cs, W);
+          gstore = b == 0;
+        } else if (W == 1) {
+          gm_new = cm;
+          gs_new = cs;
+          gstore = true;
+        } else {
+          gm_new = accm + shot_sum(cm, W);
+          gs_new = accs + shot_sum(cs, W);
+          gstore = b == 0;
+        }
+      }
       if (reverse) {
-        const int strip_col = layer && p.strip.map ? r.col : -1;
-        T v;
+        const int strip_col = layer && p.strip.map ? o.col : -1;
         if (strip_col >= 0) {
           // a layer node is replayed from the forward sweep's record
           const int64_t at =
@@ -495,15 +580,21 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
           div[0] = flux<T, R>(qu, chi0[0], clo0[0], g.op);
           div[1] = flux<T, R>(ut1, chi1[0], clo1[0], g.op);
           const T lap = combine_axes<T, 2, R>(div, g.f);
-          v = -r.uo + (T)2 * uc + mi * lap;
+          v = -o.uo + (T)2 * uc + mi * lap;
           if (has_src)
             for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
               if (p.src.takes(k, nc, b))
                 v += p.src.weight[k] * p.signal[p.src.at(n + 1, k, b)];
         }
-        bstore(U, v, f, uos);
       }
     }
+    // four stores every thread issues (off the box or without a value, out
+    // of range: dropped; an empty resource drops them all), so the count of
+    // operations after the staging copies is fixed
+    bstore(L0b, ln, active ? e : kDrop, 0);
+    bstore(GM, gm_new, gstore ? nn : kDrop, 0);
+    bstore(GS, gs_new, gstore ? nn : kDrop, 0);
+    bstore(UOb, v, active ? e : kDrop, 0);
 #pragma unroll
     for (int j = 0; j < 2 * R; ++j) {
       ql[j] = ql[j + 1];
@@ -512,16 +603,10 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
 #pragma unroll
     for (int j = 0; j < 2 * R - 1; ++j)
       qc[j] = qc[j + 1];
-  };
-
-  BwdRow<T, R> A, B;
-  load(A, ya);
-  for (int y = ya; y < yb; y += 2) {
-    load(B, min(y + 1, yb - 1));
-    step(A, y);
-    load(A, min(y + 2, yb - 1));
-    if (y + 1 < yb)
-      step(B, y + 1);
+    // the next row staged: this wave's copies (its own loads and the four
+    // stores may still run), then every wave's
+    wait_vmcnt<4 + (STIFF ? 5 : 0) + (EDGE ? 1 : 0) + 4>();
+    __syncthreads();
   }
 }
 
@@ -537,7 +622,7 @@ __global__ __launch_bounds__(kThreads) void forward_step(const FwdParams<T> p,
                                                          const int streams) {
   if constexpr (NDIM == 2 && W > 0) {
     if ((int)blockIdx.x < streams) {
-      forward_stream<T, R, W>(p, box, t, blockIdx.x, streams);
+      forward_stream<T, R, W, true>(p, box, t, blockIdx.x, streams);
       return;
     }
   }
@@ -552,12 +637,27 @@ __global__ __launch_bounds__(kThreads) void backward_step(const BwdParams<T> p,
                                                           const int streams) {
   if constexpr (NDIM == 2 && W > 0) {
     if ((int)blockIdx.x < streams) {
-      backward_stream<T, R, W, STIFF>(p, box, n, blockIdx.x, streams);
+      backward_stream<T, R, W, STIFF, true>(p, box, n, blockIdx.x, streams);
       return;
     }
   }
   backward_general<T, NDIM, R>(p, n, blockIdx.x - streams,
                                gridDim.x - streams);
+}
+
+// the core blocks of a 2D step: no layer, no wall, no point, no strip
+template <typename T, int R, int W>
+__global__ __launch_bounds__(kThreads) void forward_core(const FwdParams<T> p,
+                                                         const Box box,
+                                                         const int t) {
+  forward_stream<T, R, W, false>(p, box, t, blockIdx.x, gridDim.x);
+}
+
+template <typename T, int R, int W, bool STIFF>
+__global__ __launch_bounds__(kThreads) void backward_core(const BwdParams<T> p,
+                                                          const Box box,
+                                                          const int n) {
+  backward_stream<T, R, W, STIFF, false>(p, box, n, blockIdx.x, gridDim.x);
 }
 
 } // namespace cw

@@ -1,0 +1,117 @@
+"""Grids large enough that a 2D sweep runs its core kernels, against the oracle.
+
+A 2D sweep launches two kernels a step: the core kernel streams the blocks
+of the stream box that hold no layer node, no node whose transposed taps
+reach a layer, and no source or sensor; the edge kernel takes every other
+node. The golden grids are too small to hold a core block, so these cases
+build larger ones (sizes chosen so both kernels run, with points inside and
+outside the core rectangle) and compare with the oracle on the same inputs:
+
+- five shots (eight lanes: blocks 32 columns wide) on a 134 x 200 grid;
+- one shot (one lane: blocks 256 columns wide) on a 100 x 560 grid.
+"""
+
+from __future__ import annotations
+
+import importlib
+
+import numpy as np
+import pytest
+
+from helpers import golden, host, oracle_sim, product_sim, rel_max
+
+pytestmark = pytest.mark.gpu
+
+S = importlib.import_module("cuwave_amd.sensitivity")
+from cuwave_amd import utils, wave  # noqa: E402
+
+F32_GRAD, F64 = 1e-4, 1e-10
+
+
+def _case(Nx, precision, N, seed):
+    """An eighth-order acoustic CPML case (the physics of a golden case) on a larger grid."""
+    base = golden("adj_pml_acoustic_o8_f32")
+    z = {k: base[k] for k in ("kind", "dx", "dt", "space_order", "stencil", "boundary",
+                              "materials", "pml", "pml_faces")}
+    z.update(Nx=np.array(Nx), N=np.array(N), precision=np.array(precision))
+    sim = product_sim(z)
+    rng = np.random.default_rng(seed)
+    shape = tuple(sim.Nx_padded)
+    smooth = np.cumsum(np.cumsum(rng.standard_normal(shape), 0), 1)
+    smooth = (smooth - smooth.min()) / (smooth.max() - smooth.min())
+    z["indicator"] = (0.2 + 0.6 * smooth).astype(sim.np_dtype)
+    return z, sim, rng
+
+
+def _positions(rng, Nx, count, margin=3):
+    return np.stack([rng.integers(margin, n - margin, count) for n in Nx]).astype(np.int32)
+
+
+def test_a_batch_on_a_grid_with_core_blocks_steps_as_the_oracle():
+    from oracle import driver
+
+    z, sim, rng = _case((134, 200), "float32", 48, 11)
+    osim = oracle_sim(z)
+    print("case core forward f32 134x200 five shots", flush=True)
+    sig = rng.standard_normal((sim.N, 1)).astype(sim.np_dtype)
+    # shot 0 sits in a core block (rows 38..101, columns 32..159 of an eight-lane
+    # sweep), shot 1 in the layer, the rest anywhere
+    positions = [np.array([[60], [70]]), np.array([[5], [150]])]
+    positions += [_positions(rng, sim.Nx, 1) for _ in range(3)]
+    sensors = _positions(rng, sim.Nx, 12)
+    sources = [wave.Source(p, sig * (1 + k)) for k, p in enumerate(positions)]
+    fields, records = wave.simulate_shots(sim, sources, z["indicator"], sensors)
+    for k, s in enumerate(sources):
+        f, r = driver.simulate(osim, s.position, s.signal, z["indicator"], sensors)
+        np.testing.assert_array_equal(host(fields[k]), f, err_msg=f"shot {k}")
+        np.testing.assert_array_equal(host(records[k]), r, err_msg=f"shot {k}")
+
+
+def test_one_shot_on_a_wide_grid_matches_the_oracle_adjoints():
+    from oracle import driver
+
+    z, sim, rng = _case((100, 560), "float64", 64, 12)
+    osim = oracle_sim(z)
+    print("case core adjoint f64 100x560 one shot", flush=True)
+    sig = rng.standard_normal((sim.N, 1))
+    # the source and three sensors in a core block (rows 38..69, columns
+    # 256..511 of a one-lane sweep), close enough that the forward and adjoint
+    # fields overlap within the run (else the gradient is round-off alone)
+    src = wave.Source(np.array([[50], [300]]), sig)
+    near = np.array([[46, 55, 60], [306, 294, 303]], dtype=np.int32)
+    sensors = np.concatenate([near, _positions(rng, sim.Nx, 3)], axis=1)
+    observed = 0.5 * rng.standard_normal((sim.N, sensors.shape[1]))
+    ind = z["indicator"]
+    cost, g, traces, _ = S.sensitivity(sim, src, ind, sensors, S.l2_misfit(observed))
+    ocost, og, otraces, _ = driver.sensitivity(osim, src.position, src.signal, ind, sensors,
+                                               driver.l2_misfit(observed))
+    np.testing.assert_array_equal(host(traces), otraces)
+    assert abs(cost - ocost) <= F64 * abs(ocost)
+    assert rel_max(g["mass"], og["mass"]) < F64
+    assert rel_max(g["stiff"], og["stiff"]) < F64
+    _, g, _, _ = S.reconstruction_sensitivity(sim, src, ind, sensors, S.l2_misfit(observed))
+    _, og, _, _ = driver.reconstruction_sensitivity(osim, src.position, src.signal, ind, sensors,
+                                                    driver.l2_misfit(observed))
+    assert rel_max(g["mass"], og["mass"]) < F64
+    assert rel_max(g["stiff"], og["stiff"]) < F64
+
+
+def test_a_batched_misfit_gradient_with_core_blocks_matches_the_oracle():
+    from oracle import driver
+
+    z, sim, rng = _case((134, 200), "float32", 48, 13)
+    osim = oracle_sim(z)
+    print("case core misfit f32 134x200 five shots", flush=True)
+    sig = rng.standard_normal((sim.N, 1)).astype(sim.np_dtype)
+    positions = [np.array([[60], [70]])] + [_positions(rng, sim.Nx, 1) for _ in range(4)]
+    coords = np.stack([rng.uniform(3, n - 4, 9) * d for n, d in zip(sim.Nx, sim.dx)], axis=1)
+    observed = [0.1 * rng.standard_normal((sim.N, len(coords))).astype(sim.np_dtype)
+                for _ in positions]
+    sources = [wave.Source(p, sig * (1 + k)) for k, p in enumerate(positions)]
+    cost, gradient = utils.misfit_gradient(sim, sources, z["indicator"],
+                                           utils.Sensors(sim, coords), observed)
+    ocost, ogradient = driver.misfit_gradient(
+        osim, [(s.position, s.signal) for s in sources], z["indicator"],
+        driver.Sensors(osim, coords), observed)
+    assert abs(cost - ocost) <= 1e-5 * abs(ocost)
+    assert rel_max(gradient, ogradient) < F32_GRAD
