@@ -120,61 +120,42 @@ template <typename T> cw::Strip<T> make_strip(const cw_strip &s) {
 // ---------------------------------------------------------- stream box
 constexpr int kChunk = 32; // rows a streaming thread marches
 
-// the stream box of a 2D grid: every node order / 2 + 2 off the walls, out
-// of reach of the wall mirror, the refill, the held nodes and the adjoint's
-// reflected taps; empty on other grids, on grids too small for one and
-// where two levels pass the 4 GiB a buffer resource addresses. Its core is
-// the rectangle of blocks whose nodes neither lie in a layer slab nor reach
-// one with their transposed taps (order / 2 past the slab).
+// the stream box of a 2D grid: the nodes order / 2 + 2 off the walls (out of
+// reach of the wall mirror, the refill, the held nodes and the adjoint's
+// reflected taps) that neither lie in a layer slab nor reach one with their
+// transposed taps (order / 2 past the slab); empty on other grids, on grids
+// too small for one and where two levels pass the 4 GiB a buffer resource
+// addresses. The general kernels take every other node: the layers are
+// thin, and there one independent thread a node hides the latency of the
+// layer memory that a marching thread would wait on row after row.
 inline cw::Box stream_box(const cw_grid &g, const cw_pml &pml) {
   cw::Box box{};
   const int R = g.order / 2, margin = R + 2;
   box.chunk = kChunk;
-  if (g.ndim != 2 || g.n[0] - margin <= margin || g.n[1] - margin <= margin)
+  if (g.ndim != 2)
     return box;
   const int64_t level = (int64_t)g.np[0] * g.np[1] * g.lanes * (g.dtype ? 8 : 4);
   if (2 * level >= ((int64_t)1 << 32))
     return box;
-  box.y0 = margin;
-  box.y1 = g.n[0] - margin;
-  box.x0 = margin;
-  box.x1 = g.n[1] - margin;
-  // the rows and columns no layer term reaches: [lo, hi] per axis
   int lo[2], hi[2];
   for (int d = 0; d < 2; ++d) {
-    lo[d] = pml.lo[d] ? pml.lo[d] + R + 1 : 0;
-    hi[d] = pml.hi[d] ? g.n[d] - 3 - pml.hi[d] - R : g.n[d];
+    lo[d] = std::max(margin, pml.lo[d] ? pml.lo[d] + R + 1 : 0);
+    hi[d] = std::min(g.n[d] - margin, pml.hi[d] ? g.n[d] - 2 - pml.hi[d] - R : g.n[d]);
   }
+  if (hi[0] <= lo[0] || hi[1] <= lo[1])
+    return box;
+  box.y0 = lo[0];
+  box.y1 = hi[0];
+  box.x0 = lo[1];
+  box.x1 = hi[1];
+  // every block of the box is a core block: the edge kernel's stream blocks
+  // are those holding sources or sensors
   const int C = cw::kThreads / g.lanes;
   const int xa = box.x0 / C * C;
-  const int nchunk = (box.y1 - box.y0 + box.chunk - 1) / box.chunk;
-  const int ncx = (box.x1 - xa + C - 1) / C;
-  box.c0 = box.c1 = box.k0 = box.k1 = 0;
-  int c0 = -1, c1 = -1, k0 = -1, k1 = -1;
-  for (int c = 0; c < nchunk; ++c) {
-    const int ya = box.y0 + c * box.chunk;
-    const int yb = std::min(ya + box.chunk, box.y1);
-    if (ya >= lo[0] && yb - 1 <= hi[0]) {
-      if (c0 < 0)
-        c0 = c;
-      c1 = c + 1;
-    }
-  }
-  for (int k = 0; k < ncx; ++k) {
-    const int xs = std::max(xa + k * C, box.x0);
-    const int xe = std::min(xa + (k + 1) * C, box.x1);
-    if (xs >= lo[1] && xe - 1 <= hi[1]) {
-      if (k0 < 0)
-        k0 = k;
-      k1 = k + 1;
-    }
-  }
-  if (c0 >= 0 && k0 >= 0) {
-    box.c0 = c0;
-    box.c1 = c1;
-    box.k0 = k0;
-    box.k1 = k1;
-  }
+  box.c0 = 0;
+  box.c1 = (box.y1 - box.y0 + box.chunk - 1) / box.chunk;
+  box.k0 = 0;
+  box.k1 = (box.x1 - xa + C - 1) / C;
   return box;
 }
 
