@@ -69,6 +69,124 @@ template <> __device__ __forceinline__ double fmat(double a, double b, double c)
   return __builtin_fma(a, b, c);
 }
 
+// ------------------------------------------------------------ lane groups
+// V neighbouring shot lanes of one node that one thread of a general kernel
+// holds: 16 bytes (a single wide load or store) where the node has that
+// many lanes. Every operation acts lane by lane in the order the scalar
+// expression states, so a lane's bits are those of the one-lane arithmetic.
+template <typename T, int V> struct alignas(sizeof(T) * V) Lanes {
+  T v[V];
+  Lanes() = default;
+  __device__ __forceinline__ Lanes(T x) {
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+      v[i] = x;
+  }
+  __device__ __forceinline__ T &operator[](int i) { return v[i]; }
+  __device__ __forceinline__ const T &operator[](int i) const { return v[i]; }
+};
+
+// the lanes a thread of a general kernel holds at `lanes` shot lanes
+template <typename T>
+__host__ __device__ constexpr int lanes_per_thread(int lanes) {
+  return lanes < (int)(16 / sizeof(T)) ? lanes : (int)(16 / sizeof(T));
+}
+
+// the value type of V lanes: T itself for one
+template <typename T, int V> struct LaneType { using type = Lanes<T, V>; };
+template <typename T> struct LaneType<T, 1> { using type = T; };
+template <typename T, int V> using LanesOf = typename LaneType<T, V>::type;
+
+#define CW_LANES_OP(OP)                                                        \
+  template <typename T, int V>                                                 \
+  __device__ __forceinline__ Lanes<T, V> operator OP(const Lanes<T, V> &a,     \
+                                                     const Lanes<T, V> &b) {   \
+    Lanes<T, V> r;                                                             \
+    _Pragma("unroll") for (int i = 0; i < V; ++i) r.v[i] = a.v[i] OP b.v[i];   \
+    return r;                                                                  \
+  }                                                                            \
+  template <typename T, int V>                                                 \
+  __device__ __forceinline__ Lanes<T, V> operator OP(T a,                      \
+                                                     const Lanes<T, V> &b) {   \
+    Lanes<T, V> r;                                                             \
+    _Pragma("unroll") for (int i = 0; i < V; ++i) r.v[i] = a OP b.v[i];        \
+    return r;                                                                  \
+  }                                                                            \
+  template <typename T, int V>                                                 \
+  __device__ __forceinline__ Lanes<T, V> operator OP(const Lanes<T, V> &a,     \
+                                                     T b) {                    \
+    Lanes<T, V> r;                                                             \
+    _Pragma("unroll") for (int i = 0; i < V; ++i) r.v[i] = a.v[i] OP b;        \
+    return r;                                                                  \
+  }                                                                            \
+  template <typename T, int V>                                                 \
+  __device__ __forceinline__ Lanes<T, V> &operator OP##=(Lanes<T, V> &a,       \
+                                                         const Lanes<T, V> &b) \
+  {                                                                            \
+    a = a OP b;                                                                \
+    return a;                                                                  \
+  }
+CW_LANES_OP(+)
+CW_LANES_OP(-)
+CW_LANES_OP(*)
+CW_LANES_OP(/)
+#undef CW_LANES_OP
+
+template <typename T, int V>
+__device__ __forceinline__ Lanes<T, V> operator-(const Lanes<T, V> &a) {
+  Lanes<T, V> r;
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+    r.v[i] = -a.v[i];
+  return r;
+}
+
+template <typename T, int V>
+__device__ __forceinline__ Lanes<T, V> fmat(const Lanes<T, V> &a,
+                                           const Lanes<T, V> &b,
+                                           const Lanes<T, V> &c) {
+  Lanes<T, V> r;
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+    r.v[i] = fmat(a.v[i], b.v[i], c.v[i]);
+  return r;
+}
+template <typename T, int V>
+__device__ __forceinline__ Lanes<T, V> fmat(T a, const Lanes<T, V> &b,
+                                           const Lanes<T, V> &c) {
+  return fmat(Lanes<T, V>(a), b, c);
+}
+template <typename T, int V>
+__device__ __forceinline__ Lanes<T, V> fmat(const Lanes<T, V> &a, T b,
+                                           const Lanes<T, V> &c) {
+  return fmat(a, Lanes<T, V>(b), c);
+}
+
+// lane i of a value (a one-lane value is its own lane 0)
+template <typename T>
+__device__ __forceinline__ T &lane(T &x, int) { return x; }
+template <typename T>
+__device__ __forceinline__ const T &lane(const T &x, int) { return x; }
+template <typename T, int V>
+__device__ __forceinline__ T &lane(Lanes<T, V> &x, int i) { return x.v[i]; }
+template <typename T, int V>
+__device__ __forceinline__ const T &lane(const Lanes<T, V> &x, int i) {
+  return x.v[i];
+}
+
+constexpr int log2_of(int v) { return v <= 1 ? 0 : 1 + log2_of(v >> 1); }
+
+// the V lanes at element e of a per-shot array (e a multiple of V)
+template <typename Q, typename T>
+__device__ __forceinline__ Q lanes_at(const T *__restrict__ p, int64_t e) {
+  return *reinterpret_cast<const Q *>(p + e);
+}
+template <typename Q, typename T>
+__device__ __forceinline__ void lanes_put(T *__restrict__ p, int64_t e,
+                                          const Q &v) {
+  *reinterpret_cast<Q *>(p + e) = v;
+}
+
 // whether node a is the wall node of a face of `odd` (bit 0 low, bit 1 high)
 __device__ __forceinline__ bool held_wall(int a, int N, int odd) {
   return ((odd & 1) && a == 1) || ((odd & 2) && a == N - 2);
@@ -77,28 +195,28 @@ __device__ __forceinline__ bool held_wall(int a, int N, int odd) {
 // tap j of the line through element e at node a (s the element stride of
 // the axis), past the stored ghost the value of its mirror about the wall
 // node, negated past a face of `odd`
-template <typename T>
-__device__ __forceinline__ T mirrored(const T *__restrict__ u, int64_t e,
+template <typename Q, typename T>
+__device__ __forceinline__ Q mirrored(const T *__restrict__ u, int64_t e,
                                       int64_t s, int a, int N, int odd, int j) {
   int b = a + j;
   const bool neg = (b < 0 && (odd & 1)) || (b > N - 1 && (odd & 2));
   b = b < 0 ? 2 - b : b > N - 1 ? 2 * (N - 2) - b : b;
-  const T v = u[e + (int64_t)(b - a) * s];
+  const Q v = lanes_at<Q>(u, e + (int64_t)(b - a) * s);
   return neg ? -v : v;
 }
 
 // tap j through the image about the wall node, for a field no refill keeps
 // past it, zero on a held wall node
-template <typename T>
-__device__ __forceinline__ T reflected(const T *__restrict__ u, int64_t e,
+template <typename Q, typename T>
+__device__ __forceinline__ Q reflected(const T *__restrict__ u, int64_t e,
                                        int64_t s, int a, int N, int odd,
                                        int j) {
   int b = a + j;
   const bool neg = (b < 1 && (odd & 1)) || (b > N - 2 && (odd & 2));
   b = b < 1 ? 2 - b : b > N - 2 ? 2 * (N - 2) - b : b;
   if (held_wall(b, N, odd))
-    return (T)0;
-  const T v = u[e + (int64_t)(b - a) * s];
+    return Q((T)0);
+  const Q v = lanes_at<Q>(u, e + (int64_t)(b - a) * s);
   return neg ? -v : v;
 }
 
@@ -212,6 +330,19 @@ __device__ __forceinline__ T shot_sum(T v, int lanes) {
   for (int m = 1; m < lanes; m <<= 1)
     v += __shfl_xor(v, m, kWave);
   return v;
+}
+
+// the same sum over `lanes` lanes held V to a thread: the pairs, quads, ...
+// inside the thread, then the butterfly over the node's threads, so the sum
+// has the bits of the butterfly over one lane a thread
+template <typename T, int V>
+__device__ __forceinline__ T shot_sum(Lanes<T, V> x, int lanes) {
+#pragma unroll
+  for (int m = 1; m < V; m <<= 1)
+#pragma unroll
+    for (int i = 0; i < V; i += 2 * m)
+      x.v[i] += x.v[i + m];
+  return shot_sum(x.v[0], lanes / V);
 }
 
 // round-to-nearest-even bfloat16 bits of a float, as a narrowed record keeps

@@ -192,8 +192,9 @@ inline cw::Region band_of(const cw_grid &g, const cw::Box &b) {
   return r;
 }
 
-inline int general_blocks(const cw_grid &gr, const cw::Region &r) {
-  const int per = cw::kThreads / gr.lanes; // nodes a block takes
+// the general blocks of a sweep at V shot lanes a thread
+inline int general_blocks(const cw_grid &gr, const cw::Region &r, int V) {
+  const int per = cw::kThreads / (gr.lanes / V); // nodes a block takes
   int64_t nodes = r.band ? r.count : 1;
   if (!r.band)
     for (int d = 0; d < gr.ndim; ++d)

@@ -146,7 +146,7 @@ template <typename T, int NDIM, int R> struct Backward {
 template <typename T, int NDIM, int R> struct Superposed {
   static int run(const cw_superposed_args &a, hipStream_t stream) {
     const cw::Geom<T> g = make_geom<T>(a.grid);
-    const int blocks = cwh::general_blocks(a.grid, cw::Region{});
+    const int blocks = cwh::general_blocks(a.grid, cw::Region{}, 1);
     const T *cell = (const T *)a.mat.prep;
     hipLaunchKernelGGL((cw::superposed_kernel<T, NDIM, R>), dim3(blocks),
                        dim3(cw::kThreads), 0, stream, g, cell + NDIM * g.nodes,

@@ -109,10 +109,10 @@ template <typename T> struct BwdParams {
 // ------------------------------------------------------------------ helpers
 // the cell flux divergence of one axis over taps t[0 .. 2R] centred on R,
 // rounded as cuwave/kernels/scalar.cu:14-33 pins it
-template <typename T, int R>
-__device__ __forceinline__ T flux(const T *t, T gp, T gm, const T *op) {
-  const T uc = t[R];
-  T Dp = t[R + 1] - uc, Dm = uc - t[R - 1];
+template <typename T, int R, typename Q = T>
+__device__ __forceinline__ Q flux(const Q *t, T gp, T gm, const T *op) {
+  const Q uc = t[R];
+  Q Dp = t[R + 1] - uc, Dm = uc - t[R - 1];
   if (R > 1) {
     Dp = op[0] * Dp;
     Dm = op[0] * Dm;
@@ -122,33 +122,33 @@ __device__ __forceinline__ T flux(const T *t, T gp, T gm, const T *op) {
     Dp = fmat(op[k - 1], t[R + k] - t[R + 1 - k], Dp);
     Dm = fmat(op[k - 1], t[R + k - 1] - t[R - k], Dm);
   }
-  const T mm = Dm * gm;
+  const Q mm = Dm * gm;
   return fmat(Dp, gp, -mm);
 }
 
 // the axes combined as cuwave/kernels/scalar.cu:416-436 rounds them
-template <typename T, int NDIM, int R>
-__device__ __forceinline__ T combine_axes(const T *d, const T *f) {
+template <typename T, int NDIM, int R, typename Q = T>
+__device__ __forceinline__ Q combine_axes(const Q *d, const T *f) {
   if (NDIM == 1)
     return f[0] * d[0];
   if (NDIM == 2) {
     if (R == 1) {
-      const T p = f[1] * d[1];
+      const Q p = f[1] * d[1];
       return fmat(f[0], d[0], p);
     }
-    const T p = f[0] * d[0];
+    const Q p = f[0] * d[0];
     return fmat(f[1], d[1], p);
   }
-  const T p = f[1] * d[1];
+  const Q p = f[1] * d[1];
   return fmat(f[2], d[2], fmat(f[0], d[0], p));
 }
 
 // the two cell differences with plain sums (cuwave/kernels/scalar.cu:68-82,
 // scalar_sensitivity.cu:116-135), over taps centred on R
-template <typename T, int R>
-__device__ __forceinline__ void cell_diffs(const T *t, const T *op, T &Dp,
-                                           T &Dm) {
-  const T uc = t[R];
+template <typename T, int R, typename Q = T>
+__device__ __forceinline__ void cell_diffs(const Q *t, const T *op, Q &Dp,
+                                           Q &Dm) {
+  const Q uc = t[R];
   Dp = op[0] * (t[R + 1] - uc);
   Dm = op[0] * (uc - t[R - 1]);
 #pragma unroll
@@ -166,37 +166,38 @@ template <typename T> __device__ __forceinline__ T harmonic(T a, T b) {
 // 499-532): the three memories, `size` elements apart, updated in place,
 // their share of the field returned. `an .. bh1` are the profile at the node
 // and at the half point below it.
-template <typename T>
-__device__ __forceinline__ T pml_forward(T *aux, int64_t size, T Dp, T Dm,
+template <typename T, typename Q = T>
+__device__ __forceinline__ Q pml_forward(T *aux, int64_t size, Q Dp, Q Dm,
                                          T gp, T gm, T f, T mi, T an, T bn,
                                          T ah, T bh, T ah1, T bh1) {
-  const T pp = bh * aux[0] + ah * Dp;
-  const T pm = bh1 * aux[size] + ah1 * Dm;
-  const T z = bn * aux[2 * size] + an * f * (gp * (Dp + pp) - gm * (Dm + pm));
-  aux[0] = pp;
-  aux[size] = pm;
-  aux[2 * size] = z;
+  const Q pp = bh * lanes_at<Q>(aux, 0) + ah * Dp;
+  const Q pm = bh1 * lanes_at<Q>(aux, size) + ah1 * Dm;
+  const Q z = bn * lanes_at<Q>(aux, 2 * size) +
+              an * f * (gp * (Dp + pp) - gm * (Dm + pm));
+  lanes_put(aux, 0, pp);
+  lanes_put(aux, size, pm);
+  lanes_put(aux, 2 * size, z);
   return mi * (f * (gp * pp - gm * pm) + z);
 }
 
 // the adjoint memory of a slab node stepped by lambda (scalar_sensitivity.cu:
 // 517-542 with q = W lambda), then the two tap fields of the new memory
 // (:232-251): cur and nxt address the node's kEta rows, `size` apart
-template <typename T>
+template <typename T, typename Q = T>
 __device__ __forceinline__ void pml_adjoint_update(const T *cur, T *nxt,
-                                                   int64_t size, T q, T gp,
+                                                   int64_t size, Q q, T gp,
                                                    T gm, T f, T an, T bn,
                                                    T ah, T bh, T ah1, T bh1) {
-  const T ez = cur[2 * size];
-  const T ep = bh * (cur[0] + an * f * gp * ez) + f * gp * q;
-  const T em = bh1 * (cur[size] - an * f * gm * ez) - f * gm * q;
-  const T zz = bn * ez + q;
-  nxt[0] = ep;
-  nxt[size] = em;
-  nxt[2 * size] = zz;
-  const T e2 = an * f * zz;
-  nxt[3 * size] = ah * ep + gp * (1 + ah) * e2;
-  nxt[4 * size] = ah1 * em - gm * (1 + ah1) * e2;
+  const Q ez = lanes_at<Q>(cur, 2 * size);
+  const Q ep = bh * (lanes_at<Q>(cur, 0) + an * f * gp * ez) + f * gp * q;
+  const Q em = bh1 * (lanes_at<Q>(cur, size) - an * f * gm * ez) - f * gm * q;
+  const Q zz = bn * ez + q;
+  lanes_put(nxt, 0, ep);
+  lanes_put(nxt, size, em);
+  lanes_put(nxt, 2 * size, zz);
+  const Q e2 = an * f * zz;
+  lanes_put(nxt, 3 * size, ah * ep + gp * (1 + ah) * e2);
+  lanes_put(nxt, 4 * size, ah1 * em - gm * (1 + ah1) * e2);
 }
 
 // d(Dp)/d(u at row q) and d(Dm)/d(u at row q) (scalar_sensitivity.cu:219-230)
@@ -216,16 +217,16 @@ __device__ __forceinline__ T minus_tap(int q, const T *op) {
 // the transposed layer of one axis at a node: the tap fields of the slab
 // nodes within R along the axis, `at(k)` the element of the node k away or
 // -1 off the slab, `size` the elements between memory rows
-template <typename T, int R, typename At>
-__device__ __forceinline__ T gather(const T *__restrict__ eta, int64_t size,
+template <typename T, int R, typename Q = T, typename At>
+__device__ __forceinline__ Q gather(const T *__restrict__ eta, int64_t size,
                                     const T *op, At at) {
-  T sum = 0;
+  Q sum = (T)0;
 #pragma unroll
   for (int k = -R; k <= R; ++k) {
     const int64_t i = at(k);
     if (i >= 0)
-      sum += eta[3 * size + i] * plus_tap<T, R>(-k, op) +
-             eta[4 * size + i] * minus_tap<T, R>(-k, op);
+      sum += lanes_at<Q>(eta, 3 * size + i) * plus_tap<T, R>(-k, op) +
+             lanes_at<Q>(eta, 4 * size + i) * minus_tap<T, R>(-k, op);
   }
   return sum;
 }
@@ -233,13 +234,13 @@ __device__ __forceinline__ T gather(const T *__restrict__ eta, int64_t size,
 // scalar_sensitivity.cu:137-154 transposed_divergence_taps over taps
 // l[0 .. 2R] and cell means: hi[j - 1] the cell (a + j - 1, a + j),
 // lo[j - 1] the cell (a - j, a - j + 1)
-template <typename T, int R>
-__device__ __forceinline__ T transposed(const T *l, const T *chi,
+template <typename T, int R, typename Q = T>
+__device__ __forceinline__ Q transposed(const Q *l, const T *chi,
                                         const T *clo, const T *op) {
-  T div = 0, lh = l[R], ll = l[R];
+  Q div = (T)0, lh = l[R], ll = l[R];
 #pragma unroll
   for (int j = 1; j <= R; ++j) {
-    const T lh1 = l[R + j], ll1 = l[R - j];
+    const Q lh1 = l[R + j], ll1 = l[R - j];
     div += op[j - 1] * (chi[j - 1] * (lh1 - lh) - clo[j - 1] * (ll - ll1));
     lh = lh1;
     ll = ll1;
@@ -248,13 +249,13 @@ __device__ __forceinline__ T transposed(const T *l, const T *chi,
 }
 
 // scalar_sensitivity.cu:116-135 stiffness_gradient_taps
-template <typename T, int R>
-__device__ __forceinline__ T stiffness_density(const T *u, const T *l, T lc,
+template <typename T, int R, typename Q = T>
+__device__ __forceinline__ Q stiffness_density(const Q *u, const Q *l, Q lc,
                                                T sc, T sp, T sm, T factor,
                                                const T *op) {
   const T dgp = sp * sp / ((sc + sp) * (sc + sp));
   const T dgm = sm * sm / ((sc + sm) * (sc + sm));
-  T Dp, Dm;
+  Q Dp, Dm;
   cell_diffs<T, R>(u, op, Dp, Dm);
   return factor * (dgp * Dp * (l[R + 1] - lc) + dgm * Dm * (lc - l[R - 1]));
 }
@@ -262,22 +263,23 @@ __device__ __forceinline__ T stiffness_density(const T *u, const T *l, T lc,
 // the gradient of one level added into the per-node accumulators: with one
 // lane the reference's chain (the densities subtracted from the accumulator
 // itself, which `cm` and `cs` then hold), with several the shots' densities
-// summed across the lanes first; every lane of the node takes part
-template <typename T>
+// summed across the lanes first; every thread of the node takes part, the
+// one holding lane 0 (`first`) stores
+template <typename T, typename Q>
 __device__ __forceinline__ void add_gradient(const BwdParams<T> &p,
-                                             int64_t idx, int b, T accm,
-                                             T accs, T cm, T cs) {
+                                             int64_t idx, bool first, T accm,
+                                             T accs, Q cm, Q cs) {
   const int lanes = p.g.lanes;
   if (p.combine) {
     const T x = shot_sum(p.jm * cm + p.js * cs, lanes);
-    if (b == 0)
+    if (first)
       p.gmass[idx] = accm + x;
   } else if (lanes == 1) {
-    p.gmass[idx] = cm;
-    p.gstiff[idx] = cs;
+    p.gmass[idx] = lane(cm, 0);
+    p.gstiff[idx] = lane(cs, 0);
   } else {
     const T xm = shot_sum(cm, lanes), xs = shot_sum(cs, lanes);
-    if (b == 0) {
+    if (first) {
       p.gmass[idx] = accm + xm;
       p.gstiff[idx] = accs + xs;
     }
@@ -313,13 +315,13 @@ __device__ __forceinline__ bool band_node(const Region &r, const Geom<T> &g,
   return true;
 }
 
-// this thread's node, every lane of a node the same: the band's, or the
-// interior's in row-major order; block `bid` of `nb` general blocks
+// this thread's node, every thread of a node (2^ts of them) the same: the
+// band's, or the interior's in row-major order; block `bid` of `nb`
 template <int NDIM, typename T>
 __device__ __forceinline__ bool general_node(const Region &r, const Geom<T> &g,
-                                             int *a, int bid, int nb) {
-  const int per = kThreads >> g.lshift;
-  const int t = xcd_tile(bid, nb) * per + (int)(threadIdx.x >> g.lshift);
+                                             int *a, int bid, int nb, int ts) {
+  const int per = kThreads >> ts;
+  const int t = xcd_tile(bid, nb) * per + (int)(threadIdx.x >> ts);
   if (NDIM == 2 && r.band)
     return t < r.count && band_node(r, g, t, a);
   if (t >= g.interior)
@@ -365,16 +367,18 @@ __global__ __launch_bounds__(kThreads) void prepare_kernel(
 }
 
 // ------------------------------------------------------------ forward step
-// step t at the nodes of general block `bid` of `nb`
-template <typename T, int NDIM, int R>
+// step t at the nodes of general block `bid` of `nb`, V shot lanes a thread
+template <typename T, int NDIM, int R, int V>
 __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
                                                 const int t, int bid, int nb) {
+  using Q = LanesOf<T, V>;
   const Geom<T> &g = p.g;
+  const int ts = g.lshift - log2_of(V);
   int a[3];
-  if (!general_node<NDIM>(p.region, g, a, bid, nb))
+  if (!general_node<NDIM>(p.region, g, a, bid, nb, ts))
     return;
   const int sh = g.lshift;
-  const int b = threadIdx.x & (g.lanes - 1);
+  const int b = (int)(threadIdx.x & ((1u << ts) - 1)) * V; // first lane
   int64_t idx = 0;
   bool held = false, deep = true;
 #pragma unroll
@@ -402,25 +406,25 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
   }
   const int tile = point_tile<NDIM>(g, a);
   const T *__restrict__ u1 = p.u.at(t - 1);
-  T taps[NDIM][2 * R + 1];
+  Q taps[NDIM][2 * R + 1];
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
     const int64_t se = (int64_t)g.s[d] << sh;
 #pragma unroll
     for (int j = -R; j <= R; ++j)
-      taps[d][j + R] = deep ? u1[e + j * se]
-                            : mirrored(u1, e, se, a[d], g.n[d],
-                                       g.dirichlet >> (2 * d), j);
+      taps[d][j + R] = deep ? lanes_at<Q>(u1, e + j * se)
+                            : mirrored<Q>(u1, e, se, a[d], g.n[d],
+                                          g.dirichlet >> (2 * d), j);
   }
-  T div[NDIM];
+  Q div[NDIM];
 #pragma unroll
   for (int d = 0; d < NDIM; ++d)
     div[d] = flux<T, R>(taps[d], gp[d], gm[d], g.op);
-  const T lap = combine_axes<T, NDIM, R>(div, g.f);
-  const T uc = taps[0][R];
-  T v = -p.u.at(t - 2)[e] + (T)2 * uc + mi * lap;
+  const Q lap = combine_axes<T, NDIM, R>(div, g.f);
+  const Q uc = taps[0][R];
+  Q v = -lanes_at<Q>(p.u.at(t - 2), e) + (T)2 * uc + mi * lap;
   if (held)
-    v = 0;
+    v = (T)0;
   if (layer) {
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) {
@@ -432,9 +436,9 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
       const int64_t size = slab_size<NDIM>(g, d, L, H) << sh;
       T *aux = p.pml.aux[d] +
                (slab_index<NDIM>(g, a, d, prow[d], L, H) << sh) + b;
-      T Dp, Dm;
+      Q Dp, Dm;
       cell_diffs<T, R>(taps[d], g.op, Dp, Dm);
-      const T share =
+      const Q share =
           pml_forward(aux, size, Dp, Dm, gp[d], gm[d], g.f[d], mi, an[ad],
                       bn[ad], ah[ad], bh[ad], ah[ad - 1], bh[ad - 1]);
       if (!held)
@@ -443,10 +447,12 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
   }
   if (p.src.starts)
     for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
-      if (p.src.takes(k, idx, b))
-        v += p.src.weight[k] * p.signal[p.src.at(t, k, b)];
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+        if (p.src.takes(k, idx, b + i))
+          lane(v, i) += p.src.weight[k] * p.signal[p.src.at(t, k, b + i)];
   T *__restrict__ u2 = p.u.at(t);
-  u2[e] = v;
+  lanes_put(u2, e, v);
   // the ghost refill folded in: a mirror node writes its face ghost
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
@@ -454,23 +460,28 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
     const int odd = g.dirichlet >> (2 * d);
     const int64_t se = (int64_t)g.s[d] << sh;
     if ((faces & 1) && a[d] == 2)
-      u2[e - 2 * se] = (odd & 1) ? -v : v;
+      lanes_put(u2, e - 2 * se, (odd & 1) ? -v : v);
     if ((faces & 2) && a[d] == g.n[d] - 3)
-      u2[e + 2 * se] = (odd & 2) ? -v : v;
+      lanes_put(u2, e + 2 * se, (odd & 2) ? -v : v);
   }
   if (p.rec.starts)
     for (int k = p.rec.starts[tile]; k < p.rec.starts[tile + 1]; ++k)
-      if (p.rec.takes(k, idx, b))
-        p.record[p.rec.at(t, k, b)] = v;
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+        if (p.rec.takes(k, idx, b + i))
+          p.record[p.rec.at(t, k, b + i)] = lane(v, i);
   if (layer && p.strip.map) {
     const int col = p.strip.map[idx];
     if (col >= 0) {
       const int64_t at =
           (((int64_t)(t - p.strip.first) * p.strip.count + col) << sh) + b;
-      if (p.strip.bf16)
-        ((uint16_t *)p.strip.record)[at] = bf16_bits((float)v);
-      else
-        ((T *)p.strip.record)[at] = v;
+      if (p.strip.bf16) {
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+          ((uint16_t *)p.strip.record)[at + i] = bf16_bits((float)lane(v, i));
+      } else {
+        lanes_put((T *)p.strip.record, at, v);
+      }
     }
   }
 }
@@ -480,17 +491,19 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
 // block `bid` of `nb`: lambda^n over the slot of lambda^(n + 2), its
 // gradients against u^n, the transposed layer, and the reverse march
 // u^(n - 1) over the slot of u^(n + 1)
-template <typename T, int NDIM, int R>
+template <typename T, int NDIM, int R, int V>
 __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
                                                  const int n, int bid,
                                                  int nb) {
+  using Q = LanesOf<T, V>;
   constexpr int REACH = R + 2; // scalar_sensitivity.cu:44 WALL_REACH
   const Geom<T> &g = p.g;
+  const int ts = g.lshift - log2_of(V);
   int a[3];
-  if (!general_node<NDIM>(p.region, g, a, bid, nb))
+  if (!general_node<NDIM>(p.region, g, a, bid, nb, ts))
     return;
   const int sh = g.lshift;
-  const int b = threadIdx.x & (g.lanes - 1);
+  const int b = (int)(threadIdx.x & ((1u << ts) - 1)) * V; // first lane
   int64_t idx = 0;
   bool held = false;
   T w = 1; // the cell weight W of the node
@@ -511,10 +524,11 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
   if (p.stiff_grad) {
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) {
-      sp[d] = near[d] ? reflected(p.stiff, idx, g.s[d], a[d], g.n[d], 0, 1)
+      sp[d] = near[d] ? reflected<T>(p.stiff, idx, g.s[d], a[d], g.n[d], 0, 1)
                       : p.stiff[idx + g.s[d]];
-      sm[d] = near[d] ? reflected(p.stiff, idx, g.s[d], a[d], g.n[d], 0, -1)
-                      : p.stiff[idx - g.s[d]];
+      sm[d] = near[d]
+                  ? reflected<T>(p.stiff, idx, g.s[d], a[d], g.n[d], 0, -1)
+                  : p.stiff[idx - g.s[d]];
     }
   }
   // the layer: slab rows (the memory this node owns) and the reach of the
@@ -548,24 +562,25 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
       accs = p.gstiff[idx];
   }
 
-  const T lc = l1[e], lo_ = l0[e], uc = u1[e];
-  const T lw = held ? (T)0 : lc;
-  T div_l = 0, gsum = 0;
+  const Q lc = lanes_at<Q>(l1, e), lo_ = lanes_at<Q>(l0, e);
+  const Q uc = lanes_at<Q>(u1, e);
+  const Q lw = held ? Q((T)0) : lc;
+  Q div_l = (T)0, gsum = (T)0;
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
     const int odd = g.dirichlet >> (2 * d);
     const int64_t se = (int64_t)g.s[d] << sh;
-    T lt[2 * R + 1];
+    Q lt[2 * R + 1];
     T ch[R], cl[R];
     if (near[d]) {
 #pragma unroll
       for (int j = -R; j <= R; ++j)
-        lt[j + R] = reflected(l1, e, se, a[d], g.n[d], odd, j);
+        lt[j + R] = reflected<Q>(l1, e, se, a[d], g.n[d], odd, j);
       lt[R] = lc;
       T ks[2 * R + 1];
 #pragma unroll
       for (int j = -R; j <= R; ++j)
-        ks[j + R] = reflected(p.stiff, idx, g.s[d], a[d], g.n[d], 0, j);
+        ks[j + R] = reflected<T>(p.stiff, idx, g.s[d], a[d], g.n[d], 0, j);
       ks[R] = sc;
 #pragma unroll
       for (int j = 1; j <= R; ++j) {
@@ -575,7 +590,7 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
     } else {
 #pragma unroll
       for (int j = -R; j <= R; ++j)
-        lt[j + R] = l1[e + j * se];
+        lt[j + R] = lanes_at<Q>(l1, e + j * se);
 #pragma unroll
       for (int j = 1; j <= R; ++j) {
         ch[j - 1] = p.cell[d * g.nodes + idx + (int64_t)(j - 1) * g.s[d]];
@@ -584,11 +599,11 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
     }
     div_l += g.f[d] * transposed<T, R>(lt, ch, cl, g.op);
     if (p.stiff_grad && grad) {
-      T ut[2 * R + 1];
+      Q ut[2 * R + 1];
 #pragma unroll
       for (int j = -R; j <= R; ++j)
-        ut[j + R] = near[d] ? reflected(u1, e, se, a[d], g.n[d], odd, j)
-                            : u1[e + j * se];
+        ut[j + R] = near[d] ? reflected<Q>(u1, e, se, a[d], g.n[d], odd, j)
+                            : lanes_at<Q>(u1, e + j * se);
       ut[R] = uc;
       if (near[d])
         lt[R] = lw;
@@ -596,13 +611,13 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
                                       sm[d], g.f[d], g.op);
     }
   }
-  T ln = -lo_ + (T)2 * lc + mi * div_l;
+  Q ln = -lo_ + (T)2 * lc + mi * div_l;
   if (held)
-    ln = 0;
+    ln = (T)0;
   // the inertia density by parts in time (scalar_sensitivity.cu:512-514)
-  T cm = chain ? accm : (T)0;
+  Q cm = chain ? accm : (T)0;
   cm = cm - p.mf * uc * (ln - (T)2 * lc + lo_);
-  T cs = chain ? accs : (T)0;
+  Q cs = chain ? accs : (T)0;
   cs = cs - p.mf * gsum;
   if (reach) {
 #pragma unroll
@@ -616,7 +631,7 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
 #pragma unroll
       for (int k = 0; k < NDIM; ++k)
         at[k] = a[k];
-      T sum = gather<T, R>(eta, size, g.op, [&](int k) {
+      Q sum = gather<T, R, Q>(eta, size, g.op, [&](int k) {
         const int row = slab_row(a[d] + k, N, L, H);
         if (row < 0)
           return (int64_t)-1;
@@ -629,7 +644,7 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
         const int ghost = high ? 2 * (N - 2) - a[d] : 2 - a[d];
         if (high ? ghost < N - 1 : ghost > 0)
           continue;
-        T fold = 0;
+        Q fold = (T)0;
 #pragma unroll
         for (int k = 1; k <= R; ++k) {
           const int ai = high ? N - 1 - k : k;
@@ -637,25 +652,29 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
           if (row >= 0 && ghost - ai >= -R && ghost - ai <= R) {
             at[d] = ai;
             const int64_t i = slab_index<NDIM>(g, at, d, row, L, H) << sh;
-            fold += eta[3 * size + i] * plus_tap<T, R>(ghost - ai, g.op) +
-                    eta[4 * size + i] * minus_tap<T, R>(ghost - ai, g.op);
+            fold += lanes_at<Q>(eta, 3 * size + i) *
+                        plus_tap<T, R>(ghost - ai, g.op) +
+                    lanes_at<Q>(eta, 4 * size + i) *
+                        minus_tap<T, R>(ghost - ai, g.op);
           }
         }
         sum += ((g.dirichlet >> (2 * d + high)) & 1) ? -fold : fold;
       }
-      const T delta = mi * sum / w;
+      const Q delta = mi * sum / w;
       ln += delta;
       cm -= p.mf * uc * delta;
     }
   }
   if (p.rec.starts)
     for (int k = p.rec.starts[tile]; k < p.rec.starts[tile + 1]; ++k)
-      if (p.rec.takes(k, idx, b)) {
-        const T load = p.rec.weight[k] * p.adjoint[p.rec.at(m, k, b)];
-        ln += load;
-        cm += -p.mf * uc * load;
-      }
-  l0[e] = ln;
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+        if (p.rec.takes(k, idx, b + i)) {
+          const T load = p.rec.weight[k] * p.adjoint[p.rec.at(m, k, b + i)];
+          lane(ln, i) += load;
+          lane(cm, i) += -p.mf * lane(uc, i) * load;
+        }
+  lanes_put(l0, e, ln);
   // the memory the next level gathers, from this level's final lambda
   if (layer) {
 #pragma unroll
@@ -668,64 +687,73 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
       const T *an = p.pml.profile[d], *bn = an + N, *ah = an + 2 * N,
               *bh = an + 3 * N;
       pml_adjoint_update(p.pml.eta[d][n & 1] + i, p.pml.eta[d][(n + 1) & 1] + i,
-                         size, held ? (T)0 : w * ln,
+                         size, held ? Q((T)0) : w * ln,
                          p.cell[d * g.nodes + idx],
                          p.cell[d * g.nodes + idx - g.s[d]], g.f[d], an[ad],
                          bn[ad], ah[ad], bh[ad], ah[ad - 1], bh[ad - 1]);
     }
   }
   if (grad)
-    add_gradient(p, idx, b, accm, accs, cm, cs);
+    add_gradient(p, idx, b == 0, accm, accs, cm, cs);
   if (reverse) {
     // the lossless step run backwards: u^(n - 1) from u^n and u^(n + 1)
-    T v;
+    Q v;
     if (strip_col >= 0) {
       // a layer node is replayed from the forward sweep's record
       const int64_t at =
           (((int64_t)(n - 1 - p.strip.first) * p.strip.count + strip_col)
            << sh) +
           b;
-      v = p.strip.bf16 ? (T)bf16_value(((const uint16_t *)p.strip.record)[at])
-                       : ((const T *)p.strip.record)[at];
+      if (p.strip.bf16) {
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+          lane(v, i) =
+              (T)bf16_value(((const uint16_t *)p.strip.record)[at + i]);
+      } else {
+        v = lanes_at<Q>((const T *)p.strip.record, at);
+      }
     } else {
       bool deep = true;
 #pragma unroll
       for (int d = 0; d < NDIM; ++d)
         deep &= a[d] >= R && a[d] <= g.n[d] - 1 - R;
-      T taps[NDIM][2 * R + 1];
+      Q taps[NDIM][2 * R + 1];
 #pragma unroll
       for (int d = 0; d < NDIM; ++d) {
         const int64_t se = (int64_t)g.s[d] << sh;
 #pragma unroll
         for (int j = -R; j <= R; ++j)
-          taps[d][j + R] = deep ? u1[e + j * se]
-                                : mirrored(u1, e, se, a[d], g.n[d],
-                                           g.dirichlet >> (2 * d), j);
+          taps[d][j + R] = deep ? lanes_at<Q>(u1, e + j * se)
+                                : mirrored<Q>(u1, e, se, a[d], g.n[d],
+                                              g.dirichlet >> (2 * d), j);
       }
-      T div[NDIM];
+      Q div[NDIM];
 #pragma unroll
       for (int d = 0; d < NDIM; ++d)
         div[d] = flux<T, R>(taps[d], p.cell[d * g.nodes + idx],
                             p.cell[d * g.nodes + idx - g.s[d]], g.op);
-      const T lap = combine_axes<T, NDIM, R>(div, g.f);
-      v = -uo[e] + (T)2 * uc + mi * lap;
+      const Q lap = combine_axes<T, NDIM, R>(div, g.f);
+      v = -lanes_at<Q>(uo, e) + (T)2 * uc + mi * lap;
       if (held)
-        v = 0;
+        v = (T)0;
       if (p.src.starts)
         for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
-          if (p.src.takes(k, idx, b))
-            v += p.src.weight[k] * p.signal[p.src.at(n + 1, k, b)];
+#pragma unroll
+          for (int i = 0; i < V; ++i)
+            if (p.src.takes(k, idx, b + i))
+              lane(v, i) +=
+                  p.src.weight[k] * p.signal[p.src.at(n + 1, k, b + i)];
     }
-    uo[e] = v;
+    lanes_put(uo, e, v);
 #pragma unroll
     for (int d = 0; d < NDIM; ++d) {
       const int faces = (g.neumann | g.dirichlet) >> (2 * d);
       const int odd = g.dirichlet >> (2 * d);
       const int64_t se = (int64_t)g.s[d] << sh;
       if ((faces & 1) && a[d] == 2)
-        uo[e - 2 * se] = (odd & 1) ? -v : v;
+        lanes_put(uo, e - 2 * se, (odd & 1) ? -v : v);
       if ((faces & 2) && a[d] == g.n[d] - 3)
-        uo[e + 2 * se] = (odd & 2) ? -v : v;
+        lanes_put(uo, e + 2 * se, (odd & 2) ? -v : v);
     }
   }
 }
@@ -740,7 +768,7 @@ __global__ __launch_bounds__(kThreads) void superposed_kernel(
     int step) {
   const Region full{};
   int a[3];
-  if (!general_node<NDIM>(full, g, a, blockIdx.x, gridDim.x))
+  if (!general_node<NDIM>(full, g, a, blockIdx.x, gridDim.x, g.lshift))
     return;
   int64_t idx = 0;
   bool held = false;
@@ -770,7 +798,7 @@ __global__ __launch_bounds__(kThreads) void superposed_kernel(
     T t[2 * R + 1];
 #pragma unroll
     for (int j = -R; j <= R; ++j)
-      t[j + R] = mirrored(u1, idx, g.s[d], a[d], g.n[d],
+      t[j + R] = mirrored<T>(u1, idx, g.s[d], a[d], g.n[d],
                           g.dirichlet >> (2 * d), j);
     div[d] = flux<T, R>(t, cell[d * g.nodes + idx],
                         cell[d * g.nodes + idx - g.s[d]], g.op);

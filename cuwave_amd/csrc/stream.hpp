@@ -122,8 +122,6 @@ __device__ __forceinline__ void row_line(T *t, const T *x, T c) {
   t[R] = c;
 }
 
-constexpr int log2_of(int w) { return w <= 1 ? 0 : 1 + log2_of(w / 2); }
-
 // Copy `bytes` contiguous bytes from global `src` to LDS `dst` by DMA, the
 // block's four waves taking 64-lane chunks in turn from wave `first`; lanes
 // past the end stay idle. SIZE is the bytes a lane moves (4 or 16).
@@ -611,11 +609,12 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
 }
 
 // ------------------------------------------------------------ the steps
-// One launch a step: the first `streams` blocks (a multiple of the XCD
-// count) march the stream box, the rest take the general nodes (the 2D wall
-// band, or every node in 1D and 3D). W is the shot lanes of a node for the
-// stream blocks (0 where there are none).
-template <typename T, int NDIM, int R, int W>
+// The edge launch of a step: the first `streams` blocks (a multiple of the
+// XCD count) march the stream box's blocks that hold sources or sensors,
+// the rest take the general nodes (the 2D frame around the box, or every
+// node in 1D and 3D), V shot lanes a thread. W is the shot lanes of a node
+// for the stream blocks (0 where there are none).
+template <typename T, int NDIM, int R, int W, int V>
 __global__ __launch_bounds__(kThreads) void forward_step(const FwdParams<T> p,
                                                          const Box box,
                                                          const int t,
@@ -626,11 +625,11 @@ __global__ __launch_bounds__(kThreads) void forward_step(const FwdParams<T> p,
       return;
     }
   }
-  forward_general<T, NDIM, R>(p, t, blockIdx.x - streams,
-                              gridDim.x - streams);
+  forward_general<T, NDIM, R, V>(p, t, blockIdx.x - streams,
+                                 gridDim.x - streams);
 }
 
-template <typename T, int NDIM, int R, int W, bool STIFF>
+template <typename T, int NDIM, int R, int W, int V, bool STIFF>
 __global__ __launch_bounds__(kThreads) void backward_step(const BwdParams<T> p,
                                                           const Box box,
                                                           const int n,
@@ -641,8 +640,8 @@ __global__ __launch_bounds__(kThreads) void backward_step(const BwdParams<T> p,
       return;
     }
   }
-  backward_general<T, NDIM, R>(p, n, blockIdx.x - streams,
-                               gridDim.x - streams);
+  backward_general<T, NDIM, R, V>(p, n, blockIdx.x - streams,
+                                  gridDim.x - streams);
 }
 
 // the core blocks of a 2D step: no layer, no wall, no point, no strip
