@@ -13,8 +13,11 @@ Work unit: one interior node (PML included) of one shot advanced one step by
 one sweep; a step is W = 2 B n_int N units (forward and adjoint sweeps).
 Gcell-updates/s = W / wall time of the step.
 
-The JSON line adds `roofline` for the dominant kernel (the fused adjoint step)
-and `cpu_baseline`, the C/OpenMP oracle on the host cores over a bounded sample.
+The JSON line adds `roofline` for the dominant work, the adjoint level (its two
+launches back to back: the core kernel over the layer-free rectangle and the
+general kernels over the frame around it, timed together with HIP events on the
+launch stream), and `cpu_baseline`, the C/OpenMP oracle on the host cores over
+a bounded sample.
 """
 
 from __future__ import annotations
@@ -122,7 +125,7 @@ def gpu_run(args):
     elapsed = time.perf_counter() - t0
     timings, Propagator.timings = Propagator.timings, None
     work = 2.0 * B * n_int * sim.N * args.steps
-    # the dominant kernel: the fused adjoint step, one launch a level for the batch
+    # the dominant work: an adjoint level (core + frame launches) for the batch
     per = {}
     for kind, launches, s, e in timings:
         ms = s.elapsed_time(e)
@@ -131,7 +134,7 @@ def gpu_run(args):
         tot[1] += launches
     bwd_ms = per["backward"][0] / per["backward"][1]
     fwd_ms = per["forward"][0] / per["forward"][1]
-    bytes_bwd = (24.0 * B + 16.0) * n_int  # b_alg of the adjoint sweep per launch
+    bytes_bwd = (24.0 * B + 16.0) * n_int  # algorithmic bytes of one adjoint level
     bytes_fwd = (12.0 * B + 8.0) * n_int
     achieved = bytes_bwd / (bwd_ms * 1e-3)
     return {
@@ -160,8 +163,8 @@ def gpu_run(args):
 
 
 def traffic_per_launch():
-    """HBM bytes per adjoint launch from the committed rocprofv3 PMC summary, if any."""
-    path = ROOT / "profiles" / "traffic_backward_scalar2d_b32.json"
+    """HBM bytes per adjoint level from the committed rocprofv3 PMC summary, if any."""
+    path = ROOT / "profiles" / "traffic_adjoint_level_scalar2d_b32.json"
     if not path.exists():
         return None
     return json.loads(path.read_text()).get("bytes_per_launch")
@@ -189,6 +192,20 @@ def cpu_baseline(sample_steps: int, shots: int = 1):
     return 2.0 * shots * n_int * sample_steps / elapsed / 1e9, elapsed
 
 
+def probe_copy(mib: int = 256, reps: int = 5) -> None:
+    """A plain device copy of `mib` MiB, `reps` times: the known-size kernel that
+    calibrates the FETCH_SIZE / WRITE_SIZE counters the traffic figures use."""
+    import torch
+
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "copy", "bytes_read": src.numel() * 4,
+                      "bytes_written": dst.numel() * 4, "reps": reps}))
+
+
 def main():
     p = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     p.add_argument("--gpus", type=int, default=1)
@@ -199,7 +216,12 @@ def main():
     p.add_argument("--roi", type=int, default=ROI)
     p.add_argument("--cpu-sample", type=int, default=1500, help="oracle steps for the CPU baseline")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--probe-copy", action="store_true",
+                   help="run only a 256 MiB device copy (counter calibration)")
     args = p.parse_args()
+    if args.probe_copy:
+        probe_copy()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the shot batch shards over "
                                    "GPUs by design only; run with --gpus 1"}))
