@@ -203,8 +203,9 @@ inline int general_blocks(const cw_grid &gr, const cw::Region &r, int V) {
 }
 
 // ------------------------------------------------------------ launchers
-// forward steps t0 .. t1 - 1 and adjoint levels n1 - 1 .. n0, one launch
-// each; defined in steps.inc, one translation unit per (T, R)
+// forward steps t0 .. t1 - 1 and adjoint levels n1 - 1 .. n0, each the
+// core launch (2D) and the frame launch; defined in steps.inc, one
+// translation unit per (T, R)
 template <typename T, int NDIM, int R>
 int forward_steps(const cw_forward_args &a, int t0, int t1,
                   hipStream_t stream);
