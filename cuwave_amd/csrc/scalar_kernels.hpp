@@ -546,10 +546,15 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
   const bool grad = !(p.skip_layer && layer);
   const int m = p.steps - 1 - n;
   const int tile = point_tile<NDIM>(g, a);
-  const bool reverse = p.reverse && n <= p.steps - 2;
   int strip_col = -1;
-  if (reverse && layer && p.strip.map)
+  if (p.reverse && layer && p.strip.map)
     strip_col = p.strip.map[idx];
+  // the reverse march rebuilds u^(n - 1) where something reads it: with the
+  // layer's gradient skipped, a layer node off the strip is out of reach of
+  // every reversible node and of every strip node (the strip holds all the
+  // layer nodes within reach of a reversible one), so its value is never read
+  const bool reverse = p.reverse && n <= p.steps - 2 &&
+                       !(p.skip_layer && layer && p.strip.map && strip_col < 0);
   T *__restrict__ l0 = p.lam.at(n);
   const T *__restrict__ l1 = p.lam.at(n + 1);
   const T *__restrict__ u1 = p.u.at(n);

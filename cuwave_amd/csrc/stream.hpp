@@ -243,11 +243,21 @@ __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
   __syncthreads();
 
   for (int y = ya; y < yb; ++y) {
-    // the next row's loads, in this order (the wait at the end counts on
-    // it): its staged rows, then this thread's own values
+    // this row's staged values into registers first: once the next row's
+    // copies are in flight, every read of LDS would wait for them
+    const T *row = (const T *)(ring + (y % S::RING) * S::SEGB) + lane0;
+    T tf[2 * R + 1];
+#pragma unroll
+    for (int k = -R; k <= R; ++k)
+      if (k)
+        tf[k + R] = row[k * W];
+    q[2 * R] = ((const T *)(ring + ((y + R) % S::RING) * S::SEGB))[lane0];
+    const T gp0 = lmat[y & 1][0][xc], mi = lmat[y & 1][2][xc];
+    const T gp1 = lmat[y & 1][1][xc + 1], gm1 = lmat[y & 1][1][xc];
+    __builtin_amdgcn_sched_barrier(0);
+    // the next row's loads: its staged rows, then this thread's own values
     stage_row<T, R, W>(u1, y + R + 1, xb, s0, np1, ring, 0);
     stage_mat(y + 1);
-    __builtin_amdgcn_sched_barrier(0);
     const T u0c = u0n;
     const int col = coln;
     u0n = bload<T>(U0, e0 + (uint32_t)(y + 1 - ya) * frow, 0);
@@ -255,17 +265,9 @@ __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
       coln = bload_int(MAP, m0 + (uint32_t)(y + 1 - ya) * mrow, 0);
     __builtin_amdgcn_sched_barrier(0);
 
-    const T *row = (const T *)(ring + (y % S::RING) * S::SEGB) + lane0;
-    q[2 * R] = ((const T *)(ring + ((y + R) % S::RING) * S::SEGB))[lane0];
-    const T gp0 = lmat[y & 1][0][xc];
+    tf[R] = q[R];
     T v = 0;
     if (active) {
-      T tf[2 * R + 1];
-#pragma unroll
-      for (int k = -R; k <= R; ++k)
-        tf[k + R] = k ? row[k * W] : q[R];
-      const T mi = lmat[y & 1][2][xc];
-      const T gp1 = lmat[y & 1][1][xc + 1], gm1 = lmat[y & 1][1][xc];
       T div[2];
       div[0] = flux<T, R>(q, gp0, gm0, g.op);
       div[1] = flux<T, R>(tf, gp1, gm1, g.op);
@@ -315,9 +317,9 @@ __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
 #pragma unroll
     for (int j = 0; j < 2 * R; ++j)
       q[j] = q[j + 1];
-    // the next row staged: this wave's copies (its own loads and the store
-    // may still run), then every wave's
-    wait_vmcnt<(EDGE ? 2 : 1) + 1>();
+    // the next row's loads done (the store may still run), then every
+    // wave's copies
+    wait_vmcnt<1>();
     __syncthreads();
   }
 }
@@ -439,43 +441,54 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
   __syncthreads();
 
   for (int y = ya; y < yb; ++y) {
-    // the next row's loads, in this order (the wait at the end counts on
-    // it): its staged rows, then this thread's own values
-    stage_row<T, R, W>(l1, y + R + 1, xb, s0, np1, lring, 0);
-    stage_row<T, R, W>(u1, y + R + 1, xb, s0, np1, uring, 2);
-    stage_mat(y + 1);
-    __builtin_amdgcn_sched_barrier(0);
-    const Own o = on;
-    own(on, y + 1);
-    __builtin_amdgcn_sched_barrier(0);
-
+    // this row's staged values into registers first: once the next row's
+    // copies are in flight, every read of LDS would wait for them
     const T(*mt)[C + 2 * R] = lmat[y & 1];
     const int ls = (y % S::RING) * S::SEGB, fs = ((y + R) % S::RING) * S::SEGB;
     ql[2 * R] = ((const T *)(lring + fs))[lane0];
     qu[2 * R] = ((const T *)(uring + fs))[lane0];
     qc[2 * R - 1] = mt[2][xc];
+    T lt1[2 * R + 1], ut1[2 * R + 1], chi1[R], clo1[R];
+    {
+      const T *lrow = (const T *)(lring + ls) + lane0;
+      const T *urow = (const T *)(uring + ls) + lane0;
+#pragma unroll
+      for (int k = -R; k <= R; ++k)
+        if (k) {
+          lt1[k + R] = lrow[k * W];
+          ut1[k + R] = urow[k * W];
+        }
+#pragma unroll
+      for (int j = 1; j <= R; ++j) {
+        chi1[j - 1] = mt[0][xc + R + j - 1];
+        clo1[j - 1] = mt[0][xc + R - j];
+      }
+    }
+    const T mi = mt[1][xc];
+    __builtin_amdgcn_sched_barrier(0);
+    // the next row's loads: its staged rows, then this thread's own values
+    stage_row<T, R, W>(l1, y + R + 1, xb, s0, np1, lring, 0);
+    stage_row<T, R, W>(u1, y + R + 1, xb, s0, np1, uring, 2);
+    stage_mat(y + 1);
+    const Own o = on;
+    own(on, y + 1);
+    __builtin_amdgcn_sched_barrier(0);
+
+    lt1[R] = ql[R];
+    ut1[R] = qu[R];
     const uint32_t e = e0 + (uint32_t)(y - ya) * frow;
     const uint32_t nn = n0 + (uint32_t)(y - ya) * nrow;
     T ln = 0, v = 0, gm_new = 0, gs_new = 0;
     bool gstore = false;
     if (active) {
       const int nc = y * s0 + x;
-      const T *lrow = (const T *)(lring + ls) + lane0;
-      const T *urow = (const T *)(uring + ls) + lane0;
-      T lt1[2 * R + 1], ut1[2 * R + 1], chi1[R], clo1[R], chi0[R], clo0[R];
-#pragma unroll
-      for (int k = -R; k <= R; ++k) {
-        lt1[k + R] = k ? lrow[k * W] : ql[R];
-        ut1[k + R] = k ? urow[k * W] : qu[R];
-      }
+      T chi0[R], clo0[R];
 #pragma unroll
       for (int j = 1; j <= R; ++j) {
-        chi1[j - 1] = mt[0][xc + R + j - 1];
-        clo1[j - 1] = mt[0][xc + R - j];
         chi0[j - 1] = qc[R + j - 1];
         clo0[j - 1] = qc[R - j];
       }
-      const T mi = mt[1][xc], lo_ = o.lo;
+      const T lo_ = o.lo;
       // this row along the slow axis: slab row, reach of the transposed taps
       const int prow0 = EDGE && L0 + H0 ? slab_row(y, N0, L0, H0) : -1;
       const bool ext0 =
@@ -601,9 +614,9 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
 #pragma unroll
     for (int j = 0; j < 2 * R - 1; ++j)
       qc[j] = qc[j + 1];
-    // the next row staged: this wave's copies (its own loads and the four
-    // stores may still run), then every wave's
-    wait_vmcnt<4 + (STIFF ? 5 : 0) + (EDGE ? 1 : 0) + 4>();
+    // the next row's loads done (the four stores may still run), then
+    // every wave's copies
+    wait_vmcnt<4>();
     __syncthreads();
   }
 }
