@@ -214,10 +214,9 @@ __device__ __forceinline__ Q reflected(const T *__restrict__ u, int64_t e,
   int b = a + j;
   const bool neg = (b < 1 && (odd & 1)) || (b > N - 2 && (odd & 2));
   b = b < 1 ? 2 - b : b > N - 2 ? 2 * (N - 2) - b : b;
-  if (held_wall(b, N, odd))
-    return Q((T)0);
+  // loaded either way (no branch splits a line's loads), zeroed if held
   const Q v = lanes_at<Q>(u, e + (int64_t)(b - a) * s);
-  return neg ? -v : v;
+  return held_wall(b, N, odd) ? Q((T)0) : neg ? -v : v;
 }
 
 // the row of a slab node of axis d (rows lo off the low wall, hi off the

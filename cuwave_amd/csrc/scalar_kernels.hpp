@@ -166,31 +166,41 @@ template <typename T> __device__ __forceinline__ T harmonic(T a, T b) {
 // 499-532): the three memories, `size` elements apart, updated in place,
 // their share of the field returned. `an .. bh1` are the profile at the node
 // and at the half point below it.
+// `m` holds the three memories as read (so a caller may load them early)
 template <typename T, typename Q = T>
-__device__ __forceinline__ Q pml_forward(T *aux, int64_t size, Q Dp, Q Dm,
-                                         T gp, T gm, T f, T mi, T an, T bn,
-                                         T ah, T bh, T ah1, T bh1) {
-  const Q pp = bh * lanes_at<Q>(aux, 0) + ah * Dp;
-  const Q pm = bh1 * lanes_at<Q>(aux, size) + ah1 * Dm;
-  const Q z = bn * lanes_at<Q>(aux, 2 * size) +
-              an * f * (gp * (Dp + pp) - gm * (Dm + pm));
+__device__ __forceinline__ Q pml_forward_from(const Q *m, T *aux,
+                                              int64_t size, Q Dp, Q Dm, T gp,
+                                              T gm, T f, T mi, T an, T bn,
+                                              T ah, T bh, T ah1, T bh1) {
+  const Q pp = bh * m[0] + ah * Dp;
+  const Q pm = bh1 * m[1] + ah1 * Dm;
+  const Q z = bn * m[2] + an * f * (gp * (Dp + pp) - gm * (Dm + pm));
   lanes_put(aux, 0, pp);
   lanes_put(aux, size, pm);
   lanes_put(aux, 2 * size, z);
   return mi * (f * (gp * pp - gm * pm) + z);
+}
+template <typename T, typename Q = T>
+__device__ __forceinline__ Q pml_forward(T *aux, int64_t size, Q Dp, Q Dm,
+                                         T gp, T gm, T f, T mi, T an, T bn,
+                                         T ah, T bh, T ah1, T bh1) {
+  const Q m[3] = {lanes_at<Q>(aux, 0), lanes_at<Q>(aux, size),
+                  lanes_at<Q>(aux, 2 * size)};
+  return pml_forward_from(m, aux, size, Dp, Dm, gp, gm, f, mi, an, bn, ah, bh,
+                          ah1, bh1);
 }
 
 // the adjoint memory of a slab node stepped by lambda (scalar_sensitivity.cu:
 // 517-542 with q = W lambda), then the two tap fields of the new memory
 // (:232-251): cur and nxt address the node's kEta rows, `size` apart
 template <typename T, typename Q = T>
-__device__ __forceinline__ void pml_adjoint_update(const T *cur, T *nxt,
-                                                   int64_t size, Q q, T gp,
-                                                   T gm, T f, T an, T bn,
-                                                   T ah, T bh, T ah1, T bh1) {
-  const Q ez = lanes_at<Q>(cur, 2 * size);
-  const Q ep = bh * (lanes_at<Q>(cur, 0) + an * f * gp * ez) + f * gp * q;
-  const Q em = bh1 * (lanes_at<Q>(cur, size) - an * f * gm * ez) - f * gm * q;
+__device__ __forceinline__ void pml_adjoint_from(const Q *c, T *nxt,
+                                                 int64_t size, Q q, T gp, T gm,
+                                                 T f, T an, T bn, T ah, T bh,
+                                                 T ah1, T bh1) {
+  const Q ez = c[2];
+  const Q ep = bh * (c[0] + an * f * gp * ez) + f * gp * q;
+  const Q em = bh1 * (c[1] - an * f * gm * ez) - f * gm * q;
   const Q zz = bn * ez + q;
   lanes_put(nxt, 0, ep);
   lanes_put(nxt, size, em);
@@ -198,6 +208,16 @@ __device__ __forceinline__ void pml_adjoint_update(const T *cur, T *nxt,
   const Q e2 = an * f * zz;
   lanes_put(nxt, 3 * size, ah * ep + gp * (1 + ah) * e2);
   lanes_put(nxt, 4 * size, ah1 * em - gm * (1 + ah1) * e2);
+}
+// the same, `cur` the memory to read (`size` apart)
+template <typename T, typename Q = T>
+__device__ __forceinline__ void pml_adjoint_update(const T *cur, T *nxt,
+                                                   int64_t size, Q q, T gp,
+                                                   T gm, T f, T an, T bn,
+                                                   T ah, T bh, T ah1, T bh1) {
+  const Q c[3] = {lanes_at<Q>(cur, 0), lanes_at<Q>(cur, size),
+                  lanes_at<Q>(cur, 2 * size)};
+  pml_adjoint_from(c, nxt, size, q, gp, gm, f, an, bn, ah, bh, ah1, bh1);
 }
 
 // d(Dp)/d(u at row q) and d(Dm)/d(u at row q) (scalar_sensitivity.cu:219-230)
@@ -217,6 +237,9 @@ __device__ __forceinline__ T minus_tap(int q, const T *op) {
 // the transposed layer of one axis at a node: the tap fields of the slab
 // nodes within R along the axis, `at(k)` the element of the node k away or
 // -1 off the slab, `size` the elements between memory rows
+// Every tap's two loads are issued whether or not its node is on the slab
+// (off it from element 0, always in range) and the term is then kept or
+// dropped: no branch splits the loads, so they are all in flight together.
 template <typename T, int R, typename Q = T, typename At>
 __device__ __forceinline__ Q gather(const T *__restrict__ eta, int64_t size,
                                     const T *op, At at) {
@@ -224,9 +247,11 @@ __device__ __forceinline__ Q gather(const T *__restrict__ eta, int64_t size,
 #pragma unroll
   for (int k = -R; k <= R; ++k) {
     const int64_t i = at(k);
-    if (i >= 0)
-      sum += lanes_at<Q>(eta, 3 * size + i) * plus_tap<T, R>(-k, op) +
-             lanes_at<Q>(eta, 4 * size + i) * minus_tap<T, R>(-k, op);
+    const bool on = i >= 0;
+    const int64_t j = on ? i : 0;
+    const Q term = lanes_at<Q>(eta, 3 * size + j) * plus_tap<T, R>(-k, op) +
+                   lanes_at<Q>(eta, 4 * size + j) * minus_tap<T, R>(-k, op);
+    sum = on ? sum + term : sum;
   }
   return sum;
 }
@@ -505,32 +530,18 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
   const int sh = g.lshift;
   const int b = (int)(threadIdx.x & ((1u << ts) - 1)) * V; // first lane
   int64_t idx = 0;
-  bool held = false;
+  bool held = false, any_near = false;
   T w = 1; // the cell weight W of the node
+  bool near[NDIM];
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
     idx += (int64_t)a[d] * g.s[d];
     held |= held_wall(a[d], g.n[d], g.dirichlet >> (2 * d));
     w *= (a[d] == 1 || a[d] == g.n[d] - 2) ? (T)0.5 : (T)1;
+    near[d] = a[d] < REACH || a[d] > g.n[d] - 1 - REACH;
+    any_near |= near[d];
   }
   const int64_t e = (idx << sh) + b;
-  bool near[NDIM];
-#pragma unroll
-  for (int d = 0; d < NDIM; ++d)
-    near[d] = a[d] < REACH || a[d] > g.n[d] - 1 - REACH;
-  const T mi = p.inv[idx];
-  const T sc = p.stiff[idx];
-  T sp[NDIM], sm[NDIM];
-  if (p.stiff_grad) {
-#pragma unroll
-    for (int d = 0; d < NDIM; ++d) {
-      sp[d] = near[d] ? reflected<T>(p.stiff, idx, g.s[d], a[d], g.n[d], 0, 1)
-                      : p.stiff[idx + g.s[d]];
-      sm[d] = near[d]
-                  ? reflected<T>(p.stiff, idx, g.s[d], a[d], g.n[d], 0, -1)
-                  : p.stiff[idx - g.s[d]];
-    }
-  }
   // the layer: slab rows (the memory this node owns) and the reach of the
   // transposed taps, R rows past the front
   int prow[NDIM];
@@ -546,74 +557,165 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
   const bool grad = !(p.skip_layer && layer);
   const int m = p.steps - 1 - n;
   const int tile = point_tile<NDIM>(g, a);
-  int strip_col = -1;
-  if (p.reverse && layer && p.strip.map)
-    strip_col = p.strip.map[idx];
-  // the reverse march rebuilds u^(n - 1) where something reads it: with the
-  // layer's gradient skipped, a layer node off the strip is out of reach of
-  // every reversible node and of every strip node (the strip holds all the
-  // layer nodes within reach of a reversible one), so its value is never read
-  const bool reverse = p.reverse && n <= p.steps - 2 &&
-                       !(p.skip_layer && layer && p.strip.map && strip_col < 0);
   T *__restrict__ l0 = p.lam.at(n);
   const T *__restrict__ l1 = p.lam.at(n + 1);
   const T *__restrict__ u1 = p.u.at(n);
   T *__restrict__ uo = p.u.at(n + 1);
   const bool chain = g.lanes == 1 && !p.combine;
+
+  // The loads that need only the node come first, so that they are all in
+  // flight together rather than one branch after another.
+  int strip_col = -1;
+  if (p.reverse && layer && p.strip.map)
+    strip_col = p.strip.map[idx];
+  int r0 = 0, r1 = 0, s0 = 0, s1 = 0;
+  if (p.rec.starts) {
+    r0 = p.rec.starts[tile];
+    r1 = p.rec.starts[tile + 1];
+  }
+  if (p.reverse && p.src.starts) {
+    s0 = p.src.starts[tile];
+    s1 = p.src.starts[tile + 1];
+  }
   T accm = 0, accs = 0;
   if (grad) {
     accm = p.gmass[idx];
     if (!p.combine)
       accs = p.gstiff[idx];
   }
+  const T mi = p.inv[idx];
+  const T sc = p.stiff[idx];
+  // the layer memory this node owns, as the previous level left it
+  Q cur[NDIM][3];
+  int64_t own[NDIM];
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    own[d] = 0;
+    if (prow[d] >= 0) {
+      const int L = p.pml.lo[d], H = p.pml.hi[d];
+      const int64_t size = slab_size<NDIM>(g, d, L, H) << sh;
+      own[d] = (slab_index<NDIM>(g, a, d, prow[d], L, H) << sh) + b;
+      const T *c = p.pml.eta[d][n & 1] + own[d];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        cur[d][k] = lanes_at<Q>(c, k * size);
+    }
+  }
+  // the reverse march rebuilds u^(n - 1) where something reads it: with the
+  // layer's gradient skipped, a layer node off the strip is out of reach of
+  // every reversible node and of every strip node (the strip holds all the
+  // layer nodes within reach of a reversible one), so its value is never read
+  const bool reverse = p.reverse && n <= p.steps - 2 &&
+                       !(p.skip_layer && layer && p.strip.map && strip_col < 0);
+  Q replay = (T)0;
+  if (reverse && strip_col >= 0) {
+    // a layer node is replayed from the forward sweep's record
+    const int64_t at =
+        (((int64_t)(n - 1 - p.strip.first) * p.strip.count + strip_col) << sh) +
+        b;
+    if (p.strip.bf16) {
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+        lane(replay, i) =
+            (T)bf16_value(((const uint16_t *)p.strip.record)[at + i]);
+    } else {
+      replay = lanes_at<Q>((const T *)p.strip.record, at);
+    }
+  }
+  T sp[NDIM], sm[NDIM];
+  if (p.stiff_grad) {
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+      sp[d] = near[d] ? reflected<T>(p.stiff, idx, g.s[d], a[d], g.n[d], 0, 1)
+                      : p.stiff[idx + g.s[d]];
+      sm[d] = near[d]
+                  ? reflected<T>(p.stiff, idx, g.s[d], a[d], g.n[d], 0, -1)
+                  : p.stiff[idx - g.s[d]];
+    }
+  }
 
   const Q lc = lanes_at<Q>(l1, e), lo_ = lanes_at<Q>(l0, e);
   const Q uc = lanes_at<Q>(u1, e);
   const Q lw = held ? Q((T)0) : lc;
   Q div_l = (T)0, gsum = (T)0;
+  if (!any_near) {
+    // off the walls: every axis' taps and cell means in one run of loads
+    Q lt[NDIM][2 * R + 1];
+    T ch[NDIM][R], cl[NDIM][R];
 #pragma unroll
-  for (int d = 0; d < NDIM; ++d) {
-    const int odd = g.dirichlet >> (2 * d);
-    const int64_t se = (int64_t)g.s[d] << sh;
-    Q lt[2 * R + 1];
-    T ch[R], cl[R];
-    if (near[d]) {
-#pragma unroll
-      for (int j = -R; j <= R; ++j)
-        lt[j + R] = reflected<Q>(l1, e, se, a[d], g.n[d], odd, j);
-      lt[R] = lc;
-      T ks[2 * R + 1];
+    for (int d = 0; d < NDIM; ++d) {
+      const int64_t se = (int64_t)g.s[d] << sh;
 #pragma unroll
       for (int j = -R; j <= R; ++j)
-        ks[j + R] = reflected<T>(p.stiff, idx, g.s[d], a[d], g.n[d], 0, j);
-      ks[R] = sc;
+        if (j)
+          lt[d][j + R] = lanes_at<Q>(l1, e + j * se);
+      lt[d][R] = lc;
 #pragma unroll
       for (int j = 1; j <= R; ++j) {
-        ch[j - 1] = harmonic(ks[R + j - 1], ks[R + j]);
-        cl[j - 1] = harmonic(ks[R - j + 1], ks[R - j]);
-      }
-    } else {
-#pragma unroll
-      for (int j = -R; j <= R; ++j)
-        lt[j + R] = lanes_at<Q>(l1, e + j * se);
-#pragma unroll
-      for (int j = 1; j <= R; ++j) {
-        ch[j - 1] = p.cell[d * g.nodes + idx + (int64_t)(j - 1) * g.s[d]];
-        cl[j - 1] = p.cell[d * g.nodes + idx - (int64_t)j * g.s[d]];
+        ch[d][j - 1] = p.cell[d * g.nodes + idx + (int64_t)(j - 1) * g.s[d]];
+        cl[d][j - 1] = p.cell[d * g.nodes + idx - (int64_t)j * g.s[d]];
       }
     }
-    div_l += g.f[d] * transposed<T, R>(lt, ch, cl, g.op);
-    if (p.stiff_grad && grad) {
-      Q ut[2 * R + 1];
 #pragma unroll
-      for (int j = -R; j <= R; ++j)
-        ut[j + R] = near[d] ? reflected<Q>(u1, e, se, a[d], g.n[d], odd, j)
-                            : lanes_at<Q>(u1, e + j * se);
-      ut[R] = uc;
-      if (near[d])
-        lt[R] = lw;
-      gsum += stiffness_density<T, R>(ut, lt, near[d] ? lw : lc, sc, sp[d],
-                                      sm[d], g.f[d], g.op);
+    for (int d = 0; d < NDIM; ++d)
+      div_l += g.f[d] * transposed<T, R>(lt[d], ch[d], cl[d], g.op);
+    if (p.stiff_grad && grad) {
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d) {
+        const int64_t se = (int64_t)g.s[d] << sh;
+        Q ut[2 * R + 1];
+#pragma unroll
+        for (int j = -R; j <= R; ++j)
+          ut[j + R] = j ? lanes_at<Q>(u1, e + j * se) : uc;
+        gsum += stiffness_density<T, R>(ut, lt[d], lc, sc, sp[d], sm[d],
+                                        g.f[d], g.op);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d) {
+      const int odd = g.dirichlet >> (2 * d);
+      const int64_t se = (int64_t)g.s[d] << sh;
+      Q lt[2 * R + 1];
+      T ch[R], cl[R];
+      if (near[d]) {
+#pragma unroll
+        for (int j = -R; j <= R; ++j)
+          lt[j + R] = reflected<Q>(l1, e, se, a[d], g.n[d], odd, j);
+        lt[R] = lc;
+        T ks[2 * R + 1];
+#pragma unroll
+        for (int j = -R; j <= R; ++j)
+          ks[j + R] = reflected<T>(p.stiff, idx, g.s[d], a[d], g.n[d], 0, j);
+        ks[R] = sc;
+#pragma unroll
+        for (int j = 1; j <= R; ++j) {
+          ch[j - 1] = harmonic(ks[R + j - 1], ks[R + j]);
+          cl[j - 1] = harmonic(ks[R - j + 1], ks[R - j]);
+        }
+      } else {
+#pragma unroll
+        for (int j = -R; j <= R; ++j)
+          lt[j + R] = lanes_at<Q>(l1, e + j * se);
+#pragma unroll
+        for (int j = 1; j <= R; ++j) {
+          ch[j - 1] = p.cell[d * g.nodes + idx + (int64_t)(j - 1) * g.s[d]];
+          cl[j - 1] = p.cell[d * g.nodes + idx - (int64_t)j * g.s[d]];
+        }
+      }
+      div_l += g.f[d] * transposed<T, R>(lt, ch, cl, g.op);
+      if (p.stiff_grad && grad) {
+        Q ut[2 * R + 1];
+#pragma unroll
+        for (int j = -R; j <= R; ++j)
+          ut[j + R] = near[d] ? reflected<Q>(u1, e, se, a[d], g.n[d], odd, j)
+                              : lanes_at<Q>(u1, e + j * se);
+        ut[R] = uc;
+        if (near[d])
+          lt[R] = lw;
+        gsum += stiffness_density<T, R>(ut, lt, near[d] ? lw : lc, sc, sp[d],
+                                        sm[d], g.f[d], g.op);
+      }
     }
   }
   Q ln = -lo_ + (T)2 * lc + mi * div_l;
@@ -654,14 +756,15 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
         for (int k = 1; k <= R; ++k) {
           const int ai = high ? N - 1 - k : k;
           const int row = slab_row(ai, N, L, H);
-          if (row >= 0 && ghost - ai >= -R && ghost - ai <= R) {
-            at[d] = ai;
-            const int64_t i = slab_index<NDIM>(g, at, d, row, L, H) << sh;
-            fold += lanes_at<Q>(eta, 3 * size + i) *
-                        plus_tap<T, R>(ghost - ai, g.op) +
-                    lanes_at<Q>(eta, 4 * size + i) *
-                        minus_tap<T, R>(ghost - ai, g.op);
-          }
+          const bool on = row >= 0 && ghost - ai >= -R && ghost - ai <= R;
+          at[d] = ai;
+          const int64_t i =
+              on ? slab_index<NDIM>(g, at, d, row, L, H) << sh : 0;
+          const Q term = lanes_at<Q>(eta, 3 * size + i) *
+                             plus_tap<T, R>(ghost - ai, g.op) +
+                         lanes_at<Q>(eta, 4 * size + i) *
+                             minus_tap<T, R>(ghost - ai, g.op);
+          fold = on ? fold + term : fold;
         }
         sum += ((g.dirichlet >> (2 * d + high)) & 1) ? -fold : fold;
       }
@@ -670,15 +773,14 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
       cm -= p.mf * uc * delta;
     }
   }
-  if (p.rec.starts)
-    for (int k = p.rec.starts[tile]; k < p.rec.starts[tile + 1]; ++k)
+  for (int k = r0; k < r1; ++k)
 #pragma unroll
-      for (int i = 0; i < V; ++i)
-        if (p.rec.takes(k, idx, b + i)) {
-          const T load = p.rec.weight[k] * p.adjoint[p.rec.at(m, k, b + i)];
-          lane(ln, i) += load;
-          lane(cm, i) += -p.mf * lane(uc, i) * load;
-        }
+    for (int i = 0; i < V; ++i)
+      if (p.rec.takes(k, idx, b + i)) {
+        const T load = p.rec.weight[k] * p.adjoint[p.rec.at(m, k, b + i)];
+        lane(ln, i) += load;
+        lane(cm, i) += -p.mf * lane(uc, i) * load;
+      }
   lanes_put(l0, e, ln);
   // the memory the next level gathers, from this level's final lambda
   if (layer) {
@@ -688,36 +790,20 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
         continue;
       const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d], ad = a[d];
       const int64_t size = slab_size<NDIM>(g, d, L, H) << sh;
-      const int64_t i = (slab_index<NDIM>(g, a, d, prow[d], L, H) << sh) + b;
       const T *an = p.pml.profile[d], *bn = an + N, *ah = an + 2 * N,
               *bh = an + 3 * N;
-      pml_adjoint_update(p.pml.eta[d][n & 1] + i, p.pml.eta[d][(n + 1) & 1] + i,
-                         size, held ? Q((T)0) : w * ln,
-                         p.cell[d * g.nodes + idx],
-                         p.cell[d * g.nodes + idx - g.s[d]], g.f[d], an[ad],
-                         bn[ad], ah[ad], bh[ad], ah[ad - 1], bh[ad - 1]);
+      pml_adjoint_from(cur[d], p.pml.eta[d][(n + 1) & 1] + own[d], size,
+                       held ? Q((T)0) : w * ln, p.cell[d * g.nodes + idx],
+                       p.cell[d * g.nodes + idx - g.s[d]], g.f[d], an[ad],
+                       bn[ad], ah[ad], bh[ad], ah[ad - 1], bh[ad - 1]);
     }
   }
   if (grad)
     add_gradient(p, idx, b == 0, accm, accs, cm, cs);
   if (reverse) {
     // the lossless step run backwards: u^(n - 1) from u^n and u^(n + 1)
-    Q v;
-    if (strip_col >= 0) {
-      // a layer node is replayed from the forward sweep's record
-      const int64_t at =
-          (((int64_t)(n - 1 - p.strip.first) * p.strip.count + strip_col)
-           << sh) +
-          b;
-      if (p.strip.bf16) {
-#pragma unroll
-        for (int i = 0; i < V; ++i)
-          lane(v, i) =
-              (T)bf16_value(((const uint16_t *)p.strip.record)[at + i]);
-      } else {
-        v = lanes_at<Q>((const T *)p.strip.record, at);
-      }
-    } else {
+    Q v = replay;
+    if (strip_col < 0) {
       bool deep = true;
 #pragma unroll
       for (int d = 0; d < NDIM; ++d)
@@ -741,13 +827,12 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
       v = -lanes_at<Q>(uo, e) + (T)2 * uc + mi * lap;
       if (held)
         v = (T)0;
-      if (p.src.starts)
-        for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
+      for (int k = s0; k < s1; ++k)
 #pragma unroll
-          for (int i = 0; i < V; ++i)
-            if (p.src.takes(k, idx, b + i))
-              lane(v, i) +=
-                  p.src.weight[k] * p.signal[p.src.at(n + 1, k, b + i)];
+        for (int i = 0; i < V; ++i)
+          if (p.src.takes(k, idx, b + i))
+            lane(v, i) +=
+                p.src.weight[k] * p.signal[p.src.at(n + 1, k, b + i)];
     }
     lanes_put(uo, e, v);
 #pragma unroll
