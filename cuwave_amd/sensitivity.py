@@ -265,19 +265,30 @@ class _Reconstruction:
             f.copy_(saved)
 
     def run(self, objectives, gradient_mode):
-        """Forward, objectives, adjoint; returns per-shot costs and traces.
+        """Forward, objectives, adjoint; returns per-shot costs.
 
         `objectives(record)` takes the (B, N, nrec) record and returns a list of
         costs and the (B, N, nrec) derivative; `gradient_mode` is a dict of the
         accumulators and how the kernel adds into them.
         """
-        sim = self.sim
+        sweep = self.sweep(gradient_mode)
+        costs, dphi = objectives(next(sweep))
+        try:
+            sweep.send(dphi)
+        except StopIteration:
+            pass
+        return costs
+
+    def sweep(self, gradient_mode):
+        """The whole schedule as a generator: it yields the (B, N, nrec) record
+        once, when the adjoint first needs its source, and takes back the
+        (B, N, nrec) derivative of the cost with respect to it (so the forward
+        half can run in one call and the adjoint half in a later one)."""
         self.lam = self.prop.levels_memory(2)
         self.eta = self.prop.eta_memory()
-        self.costs = None
+        self.adjoint = None
         self.seed = None
         self.gradient_mode = gradient_mode
-        self.objectives = objectives
         for action, *segs in checkpoint_schedule(len(self.starts) - 1, self.checkpoints):
             if action == "advance":
                 self.advance(self.starts[segs[0]], self.starts[segs[1]])
@@ -288,16 +299,15 @@ class _Reconstruction:
             elif action == "drop":
                 del self.snapshots[segs[0]]
             else:
-                self.leaf(segs[0])
-        return self.costs
+                yield from self.leaf(segs[0])
 
-    def leaf(self, k: int) -> None:
+    def leaf(self, k: int):
         sim, prop = self.sim, self.prop
         s, e = self.starts[k], self.starts[k + 1]
         strip = self.strip_window(s, e)
         self.advance(s, e, strip)
-        if self.costs is None:
-            self.costs, dphi = self.objectives(prop.shots(self.record))
+        if self.adjoint is None:
+            dphi = yield prop.shots(self.record)
             w = torch.as_tensor(sim.adjoint_weights(self.sensors), device=self.dev)
             dphi = torch.as_tensor(dphi, dtype=sim.dtype, device=self.dev)
             self.adjoint = prop.lane_table(torch.flip(dphi, (1,)) / w)
