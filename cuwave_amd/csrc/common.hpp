@@ -176,6 +176,20 @@ __device__ __forceinline__ const T &lane(const Lanes<T, V> &x, int i) {
 
 constexpr int log2_of(int v) { return v <= 1 ? 0 : 1 + log2_of(v >> 1); }
 
+// Pin a loaded value in place: an empty asm that reads and writes it, so the
+// compiler cannot sink the load into the arm of a select that uses it (it
+// turns such selects into branches, each waiting on its own load).
+#ifndef CW_KEEP
+#define CW_KEEP(x) asm volatile("" : "+v"(x))
+#endif
+template <typename T> __device__ __forceinline__ void keep(T &x) { CW_KEEP(x); }
+template <typename T, int V>
+__device__ __forceinline__ void keep(Lanes<T, V> &x) {
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+    CW_KEEP(x.v[i]);
+}
+
 // the V lanes at element e of a per-shot array (e a multiple of V)
 template <typename Q, typename T>
 __device__ __forceinline__ Q lanes_at(const T *__restrict__ p, int64_t e) {
@@ -215,8 +229,32 @@ __device__ __forceinline__ Q reflected(const T *__restrict__ u, int64_t e,
   const bool neg = (b < 1 && (odd & 1)) || (b > N - 2 && (odd & 2));
   b = b < 1 ? 2 - b : b > N - 2 ? 2 * (N - 2) - b : b;
   // loaded either way (no branch splits a line's loads), zeroed if held
-  const Q v = lanes_at<Q>(u, e + (int64_t)(b - a) * s);
+  Q v = lanes_at<Q>(u, e + (int64_t)(b - a) * s);
+  keep(v);
   return held_wall(b, N, odd) ? Q((T)0) : neg ? -v : v;
+}
+
+// taps -R .. R of a line as `reflected` gives them, t[R] included: every
+// tap's load issued first, then all of them kept, then the signs and zeros
+template <int R, typename Q, typename T>
+__device__ __forceinline__ void reflected_line(const T *__restrict__ u,
+                                               int64_t e, int64_t s, int a,
+                                               int N, int odd, Q *t) {
+  bool neg[2 * R + 1], zero[2 * R + 1];
+#pragma unroll
+  for (int j = -R; j <= R; ++j) {
+    int b = a + j;
+    neg[j + R] = (b < 1 && (odd & 1)) || (b > N - 2 && (odd & 2));
+    b = b < 1 ? 2 - b : b > N - 2 ? 2 * (N - 2) - b : b;
+    zero[j + R] = held_wall(b, N, odd);
+    t[j + R] = lanes_at<Q>(u, e + (int64_t)(b - a) * s);
+  }
+#pragma unroll
+  for (int j = 0; j <= 2 * R; ++j)
+    keep(t[j]);
+#pragma unroll
+  for (int j = 0; j <= 2 * R; ++j)
+    t[j] = zero[j] ? Q((T)0) : neg[j] ? -t[j] : t[j];
 }
 
 // the row of a slab node of axis d (rows lo off the low wall, hi off the

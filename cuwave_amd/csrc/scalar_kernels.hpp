@@ -243,15 +243,27 @@ __device__ __forceinline__ T minus_tap(int q, const T *op) {
 template <typename T, int R, typename Q = T, typename At>
 __device__ __forceinline__ Q gather(const T *__restrict__ eta, int64_t size,
                                     const T *op, At at) {
-  Q sum = (T)0;
+  bool on[2 * R + 1];
+  Q e3[2 * R + 1], e4[2 * R + 1];
 #pragma unroll
   for (int k = -R; k <= R; ++k) {
     const int64_t i = at(k);
-    const bool on = i >= 0;
-    const int64_t j = on ? i : 0;
-    const Q term = lanes_at<Q>(eta, 3 * size + j) * plus_tap<T, R>(-k, op) +
-                   lanes_at<Q>(eta, 4 * size + j) * minus_tap<T, R>(-k, op);
-    sum = on ? sum + term : sum;
+    on[k + R] = i >= 0;
+    const int64_t j = on[k + R] ? i : 0;
+    e3[k + R] = lanes_at<Q>(eta, 3 * size + j);
+    e4[k + R] = lanes_at<Q>(eta, 4 * size + j);
+  }
+#pragma unroll
+  for (int k = 0; k <= 2 * R; ++k) {
+    keep(e3[k]);
+    keep(e4[k]);
+  }
+  Q sum = (T)0;
+#pragma unroll
+  for (int k = -R; k <= R; ++k) {
+    const Q term = e3[k + R] * plus_tap<T, R>(-k, op) +
+                   e4[k + R] * minus_tap<T, R>(-k, op);
+    sum = on[k + R] ? sum + term : sum;
   }
   return sum;
 }
@@ -679,14 +691,10 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
       Q lt[2 * R + 1];
       T ch[R], cl[R];
       if (near[d]) {
-#pragma unroll
-        for (int j = -R; j <= R; ++j)
-          lt[j + R] = reflected<Q>(l1, e, se, a[d], g.n[d], odd, j);
+        reflected_line<R>(l1, e, se, a[d], g.n[d], odd, lt);
         lt[R] = lc;
         T ks[2 * R + 1];
-#pragma unroll
-        for (int j = -R; j <= R; ++j)
-          ks[j + R] = reflected<T>(p.stiff, idx, g.s[d], a[d], g.n[d], 0, j);
+        reflected_line<R>(p.stiff, idx, (int64_t)g.s[d], a[d], g.n[d], 0, ks);
         ks[R] = sc;
 #pragma unroll
         for (int j = 1; j <= R; ++j) {
@@ -706,10 +714,13 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
       div_l += g.f[d] * transposed<T, R>(lt, ch, cl, g.op);
       if (p.stiff_grad && grad) {
         Q ut[2 * R + 1];
+        if (near[d]) {
+          reflected_line<R>(u1, e, se, a[d], g.n[d], odd, ut);
+        } else {
 #pragma unroll
-        for (int j = -R; j <= R; ++j)
-          ut[j + R] = near[d] ? reflected<Q>(u1, e, se, a[d], g.n[d], odd, j)
-                              : lanes_at<Q>(u1, e + j * se);
+          for (int j = -R; j <= R; ++j)
+            ut[j + R] = lanes_at<Q>(u1, e + j * se);
+        }
         ut[R] = uc;
         if (near[d])
           lt[R] = lw;
@@ -751,20 +762,31 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
         const int ghost = high ? 2 * (N - 2) - a[d] : 2 - a[d];
         if (high ? ghost < N - 1 : ghost > 0)
           continue;
-        Q fold = (T)0;
+        bool on[R];
+        Q e3[R], e4[R];
 #pragma unroll
         for (int k = 1; k <= R; ++k) {
           const int ai = high ? N - 1 - k : k;
           const int row = slab_row(ai, N, L, H);
-          const bool on = row >= 0 && ghost - ai >= -R && ghost - ai <= R;
+          on[k - 1] = row >= 0 && ghost - ai >= -R && ghost - ai <= R;
           at[d] = ai;
           const int64_t i =
-              on ? slab_index<NDIM>(g, at, d, row, L, H) << sh : 0;
-          const Q term = lanes_at<Q>(eta, 3 * size + i) *
-                             plus_tap<T, R>(ghost - ai, g.op) +
-                         lanes_at<Q>(eta, 4 * size + i) *
-                             minus_tap<T, R>(ghost - ai, g.op);
-          fold = on ? fold + term : fold;
+              on[k - 1] ? slab_index<NDIM>(g, at, d, row, L, H) << sh : 0;
+          e3[k - 1] = lanes_at<Q>(eta, 3 * size + i);
+          e4[k - 1] = lanes_at<Q>(eta, 4 * size + i);
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          keep(e3[k]);
+          keep(e4[k]);
+        }
+        Q fold = (T)0;
+#pragma unroll
+        for (int k = 1; k <= R; ++k) {
+          const int ai = high ? N - 1 - k : k;
+          const Q term = e3[k - 1] * plus_tap<T, R>(ghost - ai, g.op) +
+                         e4[k - 1] * minus_tap<T, R>(ghost - ai, g.op);
+          fold = on[k - 1] ? fold + term : fold;
         }
         sum += ((g.dirichlet >> (2 * d + high)) & 1) ? -fold : fold;
       }
