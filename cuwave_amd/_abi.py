@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -99,6 +99,7 @@ class ForwardArgs(ctypes.Structure):
         ("record", VP),
         ("strip", Strip),
         ("steps", I32),
+        ("illum", VP),
     ]
 
 

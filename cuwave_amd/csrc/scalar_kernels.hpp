@@ -86,6 +86,7 @@ template <typename T> struct FwdParams {
   Points<T> rec;
   T *record;
   Strip<T> strip;
+  T *illum; // per node: sum over steps and lanes of u^2, or null
 };
 
 template <typename T> struct BwdParams {
@@ -490,6 +491,12 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
           lane(v, i) += p.src.weight[k] * p.signal[p.src.at(t, k, b + i)];
   T *__restrict__ u2 = p.u.at(t);
   lanes_put(u2, e, v);
+  if (p.illum) {
+    // every thread of the node takes part in the sum; one writer a node
+    const T s2 = shot_sum(v * v, g.lanes);
+    if (b == 0)
+      atomicAdd(p.illum + idx, s2);
+  }
   // the ghost refill folded in: a mirror node writes its face ghost
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {

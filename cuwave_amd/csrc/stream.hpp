@@ -172,7 +172,7 @@ __device__ __forceinline__ void stage_nodes(const T *a, int64_t node, int count,
 }
 
 // --------------------------------------------------------------- forward
-template <typename T, int R, int W, bool EDGE>
+template <typename T, int R, int W, bool EDGE, bool ILLUM>
 __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
                                                const Box &box, const int t,
                                                int bid, int nb) {
@@ -313,6 +313,12 @@ __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
     // every thread stores (off the box, out of range: dropped), so the
     // count of operations after the staging copies is fixed
     bstore(U2, v, active ? e0 + (uint32_t)(y - ya) * frow : kDrop, 0);
+    if (ILLUM && p.illum) {
+      // a column's lanes sum u^2 (off the box v is zero); one writer a node
+      const T s2 = shot_sum(v * v, W);
+      if (active && b == 0)
+        atomicAdd(p.illum + (int64_t)y * s0 + x, s2);
+    }
     gm0 = gp0;
 #pragma unroll
     for (int j = 0; j < 2 * R; ++j)
@@ -634,7 +640,7 @@ __global__ __launch_bounds__(kThreads) void forward_step(const FwdParams<T> p,
                                                          const int streams) {
   if constexpr (NDIM == 2 && W > 0) {
     if ((int)blockIdx.x < streams) {
-      forward_stream<T, R, W, true>(p, box, t, blockIdx.x, streams);
+      forward_stream<T, R, W, true, true>(p, box, t, blockIdx.x, streams);
       return;
     }
   }
@@ -657,12 +663,14 @@ __global__ __launch_bounds__(kThreads) void backward_step(const BwdParams<T> p,
                                   gridDim.x - streams);
 }
 
-// the core blocks of a 2D step: no layer, no wall, no point, no strip
-template <typename T, int R, int W>
+// the core blocks of a 2D step: no layer, no wall, no point, no strip;
+// ILLUM builds in the illumination sum (a separate build, so a step that
+// does not gather it compiles as if it did not exist)
+template <typename T, int R, int W, bool ILLUM>
 __global__ __launch_bounds__(kThreads) void forward_core(const FwdParams<T> p,
                                                          const Box box,
                                                          const int t) {
-  forward_stream<T, R, W, false>(p, box, t, blockIdx.x, gridDim.x);
+  forward_stream<T, R, W, false, ILLUM>(p, box, t, blockIdx.x, gridDim.x);
 }
 
 template <typename T, int R, int W, bool STIFF>

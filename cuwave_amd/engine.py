@@ -203,8 +203,9 @@ class Propagator:
     def forward(self, u: torch.Tensor, t0: int, t1: int, *, offset: int = 0, aux=None,
                 sources: PointList | None = None, signal: torch.Tensor | None = None,
                 sensors: PointList | None = None, record: torch.Tensor | None = None,
-                strip: dict | None = None) -> None:
-        """Steps t0 .. t1 - 1 into the levels of `u` (cw_scalar_forward)."""
+                strip: dict | None = None, illum: torch.Tensor | None = None) -> None:
+        """Steps t0 .. t1 - 1 into the levels of `u` (cw_scalar_forward); `illum`, a
+        per-node accumulator, gathers the illumination sum over steps and shots of u^2."""
         a = _abi.ForwardArgs()
         a.grid = self.grid
         a.mat = self.mat
@@ -220,6 +221,9 @@ class Propagator:
             a.record = _abi.ptr(record)
         if strip is not None:
             a.strip = self._strip(strip)
+        if illum is not None:
+            self._require_shape("illumination", illum, tuple(self.sim.Nx_padded))
+            a.illum = _abi.ptr(illum)
         a.steps = self.sim.N
         with self._timed("forward", t1 - t0):
             _abi.call("cw_scalar_forward", ctypes.byref(a), int(t0), int(t1), _abi.stream())

@@ -250,6 +250,38 @@ def simulate_shots(sim: Simulation, sources, indicator, sensors=None, record_eve
     return out if len(out) > 1 else out[0]
 
 
+def illumination(sim: Simulation, sources, indicator, batch: int | None = None) -> torch.Tensor:
+    """The illumination map: per node the sum over steps and shots of u^2.
+
+    Every shot of `sources` runs forward under `indicator`, `batch` at a time
+    (at most 64, all by default), and each step adds the squares of its
+    final field, summed over the shots, into one per-node map over the
+    padded grid. The forward kernels accumulate it as they step. Dividing a
+    gradient by it (plus a small floor) compensates for the geometric
+    spreading of the source wavefield.
+    """
+    from .engine import Propagator, source_points
+
+    sources = list(sources)
+    if not sources:
+        raise ValueError("no shots")
+    for s in sources:
+        sim.require_interior(s.position, "source")
+    step = min(batch or MAX_BATCH, MAX_BATCH)
+    if step <= 0:
+        raise ValueError(f"batch must be positive, got {batch}")
+    mat = sim.build_materials(indicator)
+    illum = None
+    for i in range(0, len(sources), step):
+        prop = Propagator(sim, mat, len(sources[i:i + step]))
+        if illum is None:
+            illum = prop.accumulator()
+        src, signal = source_points(prop, mat, sources[i:i + step])
+        U = prop.levels_memory(2)
+        prop.forward(U, 0, sim.N, aux=prop.aux_memory(), sources=src, signal=signal, illum=illum)
+    return illum
+
+
 def simulate(sim: Simulation, source: Source, indicator, sensors=None, record_every=None):
     """Run `sim` forward under `source` and material `indicator` (cuwave/wave.py:1071-1163).
 

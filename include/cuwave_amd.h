@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 3
+#define CW_ABI_VERSION 4
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -125,6 +125,10 @@ typedef struct cw_forward_args {
   void *record;
   cw_strip strip;        /* strip.map NULL for none */
   int32_t steps;         /* N, the rows of the signal and record tables */
+  /* the illumination map, NULL for none: per node, each step adds the sum
+   * over the shot lanes of u^2, u the step's final field (no reference
+   * counterpart; the closed form sum_t sum_s u^2) */
+  void *illum;
 } cw_forward_args;
 
 /* adjoint steps over levels n = n1 - 1 down to n0 (cuwave/sensitivity.py:

@@ -427,6 +427,29 @@ def simulate(sim: Sim, src_pos, src_sig, indicator, sensors=None):
     return (field_, um) if um is not None else field_
 
 
+def illumination(sim: Sim, shots, indicator):
+    """Per node the sum over the shots and steps of u^2, u each step's final
+    field as `simulate` marches it (no reference counterpart: the closed form
+    sum_s sum_t u_s(t)^2), accumulated in float64."""
+    mat = sim.build_materials(indicator)
+    st = Steps(sim, mat)
+    acc = np.zeros(sim.Nx_padded)
+    for src_pos, src_sig in shots:
+        st.memory = [np.zeros_like(m) for m in st.memory]
+        src_lin = sim.flatten(src_pos)
+        weight = sim.excitation_weights(mat, src_lin)
+        signal = np.ascontiguousarray(src_sig, dtype=sim.dtype)
+        U = np.zeros((2, *sim.Nx_padded), dtype=sim.dtype)
+        for t in range(sim.N):
+            u = st.fd(U[t % 2], U[1 - t % 2], U[t % 2])
+            st.excite(u, signal[t], src_lin, weight)
+            st.refill(u)
+            acc += u.astype(np.float64) ** 2
+    inner = np.zeros(sim.Nx_padded, dtype=bool)
+    inner[tuple(slice(1, n - 1) for n in sim.Nx)] = True
+    return np.where(inner, acc, 0.0)
+
+
 def l2_misfit(observed):
     def objective(traces):
         residual = traces - observed

@@ -115,3 +115,18 @@ def test_a_batched_misfit_gradient_with_core_blocks_matches_the_oracle():
         driver.Sensors(osim, coords), observed)
     assert abs(cost - ocost) <= 1e-5 * abs(ocost)
     assert rel_max(gradient, ogradient) < F32_GRAD
+
+
+def test_the_illumination_map_with_core_blocks_matches_the_oracle():
+    from oracle import driver
+
+    z, sim, rng = _case((134, 200), "float32", 48, 14)
+    osim = oracle_sim(z)
+    print("case core illumination f32 134x200 five shots", flush=True)
+    sig = rng.standard_normal((sim.N, 1)).astype(sim.np_dtype)
+    positions = [np.array([[60], [70]])] + [_positions(rng, sim.Nx, 1) for _ in range(4)]
+    sources = [wave.Source(p, sig * (1 + k)) for k, p in enumerate(positions)]
+    illum = wave.illumination(sim, sources, z["indicator"])
+    expect = driver.illumination(osim, [(s.position, s.signal) for s in sources],
+                                 z["indicator"])
+    assert rel_max(illum, expect) < 1e-5

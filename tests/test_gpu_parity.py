@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import golden, golden_names, host, product_sim, rel_l2, rel_max
+from helpers import golden, golden_names, host, oracle_sim, product_sim, rel_l2, rel_max
 
 pytestmark = pytest.mark.gpu
 
@@ -174,3 +174,21 @@ def test_the_gradient_matches_finite_differences_in_float64():
         indicator[node] = base
         fd = (plus - minus) / (2 * h)
         assert abs(gradient[node] - fd) <= 1e-6 * abs(fd), (node, gradient[node], fd)
+
+
+def test_the_illumination_map_matches_the_oracle():
+    # sum over shots and steps of u^2, three shots in one batch and in two
+    from oracle import driver
+
+    z = golden("fwd_pml_acoustic_o8_f32")
+    sim, osim = product_sim(z), oracle_sim(z)
+    print("case illumination fwd_pml_acoustic_o8_f32", flush=True)
+    rng = np.random.default_rng(7)
+    sources = []
+    for k in range(3):
+        pos = np.array([[rng.integers(3, sim.Nx[0] - 3)], [rng.integers(3, sim.Nx[1] - 3)]])
+        sources.append(wave.Source(pos, z["src_sig"][:, :1] * (1 + k)))
+    expect = driver.illumination(osim, [(s.position, s.signal) for s in sources], z["indicator"])
+    for batch in (None, 2):
+        illum = wave.illumination(sim, sources, z["indicator"], batch=batch)
+        assert rel_l2(illum, expect) < 1e-5, batch

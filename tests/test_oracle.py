@@ -230,3 +230,22 @@ def test_the_profiles_vanish_at_the_front_and_mirror_across_the_wall():
             assert a_node[wall] < 0.0 and 0.0 < b_node[wall] < 1.0
             assert a_node[front] == 0.0 and b_node[front] == 1.0
             assert a_node[ghost] == a_node[mirror] and b_node[ghost] == b_node[mirror]
+
+
+def test_the_illumination_after_one_step_is_the_source_injection_squared():
+    # from rest one step leaves only the injected points: u = w s(0) there
+    z = golden("fwd_pml_acoustic_o8_f32")
+    sim = oracle_sim(z)
+    sim.N = 1
+    illum = D.illumination(sim, [(z["src_pos"], z["src_sig"])], z["indicator"])
+    mat = sim.build_materials(z["indicator"])
+    lin = sim.flatten(z["src_pos"])
+    w = sim.excitation_weights(mat, lin)
+    expect = np.zeros(sim.Nx_padded)
+    for k, node in enumerate(lin):
+        v = sim.dtype(w[k]) * sim.dtype(z["src_sig"][0, k])
+        expect.flat[node] = float(v) ** 2
+    np.testing.assert_array_equal(illum, expect)
+    # two shots add
+    two = D.illumination(sim, [(z["src_pos"], z["src_sig"])] * 2, z["indicator"])
+    np.testing.assert_array_equal(two, 2 * expect)
