@@ -119,6 +119,7 @@ template <typename T> cw::Strip<T> make_strip(const cw_strip &s) {
 
 // ---------------------------------------------------------- stream box
 constexpr int kChunk = 32; // rows a streaming thread marches
+static_assert(kChunk <= 32, "a chunk's rows with points are one 32-bit mask");
 
 // the stream box of a 2D grid: the nodes order / 2 + 2 off the walls (out of
 // reach of the wall mirror, the refill, the held nodes and the adjoint's

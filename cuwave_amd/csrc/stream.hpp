@@ -60,6 +60,31 @@ __device__ __forceinline__ bool chunk_has(const Points<T> &pts,
   return __builtin_amdgcn_readfirstlane(any) != 0;
 }
 
+// the rows [ya, yb) (bit y - ya; a chunk is at most 32 rows) at which the
+// wave's point tile column `tx` holds a point of `pts`: the wave's lanes scan
+// the lists together, once a block, so that the rows without a point
+// (nearly all) skip the lists without a load
+template <typename T>
+__device__ __forceinline__ uint32_t point_rows(const Points<T> &pts,
+                                               const Geom<T> &g, int ya, int yb,
+                                               int tx) {
+  const int lane = threadIdx.x & (kWave - 1);
+  uint32_t mask = 0;
+  for (int ty = ya / TileShape<2>::TY; ty <= (yb - 1) / TileShape<2>::TY;
+       ++ty) {
+    const int tile = ty * g.tf + tx;
+    const int e = pts.starts[tile + 1];
+    for (int k = pts.starts[tile] + lane; k < e; k += kWave) {
+      const int row = pts.lin[k] / g.s[0];
+      if (row >= ya && row < yb)
+        mask |= 1u << (row - ya);
+    }
+  }
+  for (int m = 1; m < kWave; m <<= 1)
+    mask |= (uint32_t)__shfl_xor((int)mask, m, kWave);
+  return __builtin_amdgcn_readfirstlane(mask);
+}
+
 // stream block `bid` of `nb` (a multiple of the XCD count) of the core or
 // of the edge kernel: its rows [ya, yb) and first column xb, or false for a
 // block the other kernel runs (or a padding block). A block starts at a
@@ -216,6 +241,8 @@ __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
   const uint32_t m0 = ((uint32_t)ya * s0 + xl) * 4u, mrow = (uint32_t)s0 * 4u;
   const bool has_src = EDGE && chunk_has(p.src, g, ya, yb, tx, tx);
   const bool has_rec = EDGE && chunk_has(p.rec, g, ya, yb, tx, tx);
+  const uint32_t src_rows = has_src ? point_rows(p.src, g, ya, yb, tx) : 0u;
+  const uint32_t rec_rows = has_rec ? point_rows(p.rec, g, ya, yb, tx) : 0u;
   const int lane0 = ((xc + R) * W + b); // this thread in a staged row
 
   auto stage_mat = [&](int y) {
@@ -293,11 +320,11 @@ __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
       }
       const int nc = y * s0 + x;
       const int tile = (y / TileShape<2>::TY) * g.tf + tx;
-      if (has_src)
+      if ((src_rows >> (y - ya)) & 1u)
         for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
           if (p.src.takes(k, nc, b))
             v += p.src.weight[k] * p.signal[p.src.at(t, k, b)];
-      if (has_rec)
+      if ((rec_rows >> (y - ya)) & 1u)
         for (int k = p.rec.starts[tile]; k < p.rec.starts[tile + 1]; ++k)
           if (p.rec.takes(k, nc, b))
             p.record[p.rec.at(t, k, b)] = v;
@@ -391,6 +418,8 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
   const uint32_t m0 = ((uint32_t)ya * s0 + xl) * 4u, mrow = (uint32_t)s0 * 4u;
   const bool has_rec = EDGE && chunk_has(p.rec, g, ya, yb, tx, tx);
   const bool has_src = EDGE && reverse && chunk_has(p.src, g, ya, yb, tx, tx);
+  const uint32_t rec_rows = has_rec ? point_rows(p.rec, g, ya, yb, tx) : 0u;
+  const uint32_t src_rows = has_src ? point_rows(p.src, g, ya, yb, tx) : 0u;
   const int lane0 = ((xc + R) * W + b);
 
   auto stage_mat = [&](int y) {
@@ -543,7 +572,7 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
         cm -= p.mf * uc * delta;
       }
       const int tile = (y / TileShape<2>::TY) * g.tf + tx;
-      if (has_rec)
+      if ((rec_rows >> (y - ya)) & 1u)
         for (int k = p.rec.starts[tile]; k < p.rec.starts[tile + 1]; ++k)
           if (p.rec.takes(k, nc, b)) {
             const T load = p.rec.weight[k] * p.adjoint[p.rec.at(m, k, b)];
@@ -598,7 +627,7 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
           div[1] = flux<T, R>(ut1, chi1[0], clo1[0], g.op);
           const T lap = combine_axes<T, 2, R>(div, g.f);
           v = -o.uo + (T)2 * uc + mi * lap;
-          if (has_src)
+          if ((src_rows >> (y - ya)) & 1u)
             for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
               if (p.src.takes(k, nc, b))
                 v += p.src.weight[k] * p.signal[p.src.at(n + 1, k, b)];
