@@ -149,8 +149,8 @@ inline cw::Box stream_box(const cw_grid &g, const cw_pml &pml) {
   box.y1 = hi[0];
   box.x0 = lo[1];
   box.x1 = hi[1];
-  // every block of the box is a core block: the edge kernel's stream blocks
-  // are those holding sources or sensors
+  // every block of the box is a core block: the second launch's stream
+  // blocks are those holding sources or sensors
   const int C = cw::kThreads / g.lanes;
   const int xa = box.x0 / C * C;
   box.c0 = 0;
@@ -161,7 +161,7 @@ inline cw::Box stream_box(const cw_grid &g, const cw_pml &pml) {
 }
 
 // the stream blocks of a sweep, and of its core, each padded to a multiple
-// of the XCD count (the edge kernel's general blocks then start on XCD 0)
+// of the XCD count (the second launch's general blocks then start on XCD 0)
 inline int pad_xcds(int blocks) {
   return (blocks + cw::kXcds - 1) / cw::kXcds * cw::kXcds;
 }

@@ -1,7 +1,9 @@
-// Streaming kernels for the bulk of a 2D grid: every node at least
-// order / 2 + 2 nodes off the walls, the PML included. The general kernels
-// (scalar_kernels.hpp) take the thin band that is left, where the wall
-// mirror, the ghost refill and the held nodes live.
+// Streaming kernels for the bulk of a 2D grid: the rectangle of nodes at
+// least order / 2 + 2 off the walls that neither lie in a PML slab nor reach
+// one with their transposed taps (the stream box). The general kernels
+// (scalar_kernels.hpp) take the frame around it: the slabs, the ring their
+// transposed taps reach, and the wall band with the mirror, the ghost
+// refill and the held nodes.
 //
 // A thread is one shot lane of one column and marches a chunk of rows down
 // the slow axis, keeping the 2R + 1 values of its column that the slow-axis
@@ -10,20 +12,23 @@
 // a tapped field reaches the block once, as one contiguous run staged in
 // LDS by the DMA loads of gfx950 (global_load_lds, 16 bytes a lane, no
 // registers on the way): C + 2R columns with the halo, in a ring of R + 2
-// rows. A step reads its fast-axis taps from LDS row y and its queue feed
-// from LDS row y + R while the DMA of row y + R + 1 and the thread's own
-// values of the next row (the centre of the overwritten slots, the
-// accumulators) are in flight; one barrier a row hands the staged row to
-// every wave. The per-node materials of a row are staged the same way.
-// That keeps the vector memory instructions a row few and wide, which is
-// what the texture path of a CU can sustain.
+// rows. The per-node materials of a row are staged the same way.
+//
+// A row first reads its fast-axis taps (LDS row y), its queue feed (LDS row
+// y + R) and its materials into registers; then it issues the DMA of row
+// y + R + 1 and the thread's own values of the next row; then it computes
+// from registers while those loads fly. Reading LDS first matters: the
+// compiler cannot tell which LDS bytes an issued DMA writes, so any LDS
+// read after one waits for every load in flight. One wait and one barrier
+// end the row.
 //
 // Blocks are dealt to the XCDs in contiguous runs, rows of blocks across
 // the box, so vertically neighbouring chunks share one L2.
 //
-// A row of the slow axis is either in that axis' slab or not, so its layer
-// terms are uniform over the block; the fast axis' slab columns are fixed
-// per thread for the whole chunk.
+// The box holds no layer node, so the core kernel carries no layer, point
+// or strip code. A box block holding a source or sensor runs in the second
+// launch of the step instead, with the point code (the layer code there is
+// never reached).
 //
 // The arithmetic is the general kernels' term by term, in the same order,
 // so both paths give the same bits; the cell weight of every node here is 1.
@@ -35,10 +40,9 @@ namespace cw {
 
 // The box of a sweep, rows [y0, y1) and columns [x0, x1), cut into blocks
 // of `chunk` rows by C columns (the first block column starting at the
-// multiple of C below x0). Blocks [c0, c1) x [k0, k1) are the core: no
-// node of theirs has a layer term or reaches one, so the core kernel runs
-// them without any of that code; the rest, and core blocks holding points,
-// take the edge kernel.
+// multiple of C below x0). Blocks [c0, c1) x [k0, k1) are the core, which
+// the core kernel runs; with the box layer-free that is every block, and
+// the blocks holding points are handed to the second launch.
 struct Box {
   int y0, y1, x0, x1;
   int chunk;
@@ -86,7 +90,7 @@ __device__ __forceinline__ uint32_t point_rows(const Points<T> &pts,
 }
 
 // stream block `bid` of `nb` (a multiple of the XCD count) of the core or
-// of the edge kernel: its rows [ya, yb) and first column xb, or false for a
+// of the second launch: its rows [ya, yb) and first column xb, or false for a
 // block the other kernel runs (or a padding block). A block starts at a
 // multiple of its width, so no wave straddles a point tile.
 template <bool EDGE, typename T>
