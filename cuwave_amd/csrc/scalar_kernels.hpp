@@ -443,6 +443,18 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
     layer |= prow[d] >= 0;
   }
   const int tile = point_tile<NDIM>(g, a);
+  // the point-list bounds and the strip column first, in flight with the taps
+  int r0 = 0, r1 = 0, s0 = 0, s1 = 0, col = -1;
+  if (p.src.starts) {
+    s0 = p.src.starts[tile];
+    s1 = p.src.starts[tile + 1];
+  }
+  if (p.rec.starts) {
+    r0 = p.rec.starts[tile];
+    r1 = p.rec.starts[tile + 1];
+  }
+  if (layer && p.strip.map)
+    col = p.strip.map[idx];
   const T *__restrict__ u1 = p.u.at(t - 1);
   Q taps[NDIM][2 * R + 1];
 #pragma unroll
@@ -483,8 +495,7 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
         v += share;
     }
   }
-  if (p.src.starts)
-    for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
+  for (int k = s0; k < s1; ++k)
 #pragma unroll
       for (int i = 0; i < V; ++i)
         if (p.src.takes(k, idx, b + i))
@@ -508,14 +519,12 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
     if ((faces & 2) && a[d] == g.n[d] - 3)
       lanes_put(u2, e + 2 * se, (odd & 2) ? -v : v);
   }
-  if (p.rec.starts)
-    for (int k = p.rec.starts[tile]; k < p.rec.starts[tile + 1]; ++k)
+  for (int k = r0; k < r1; ++k)
 #pragma unroll
       for (int i = 0; i < V; ++i)
         if (p.rec.takes(k, idx, b + i))
           p.record[p.rec.at(t, k, b + i)] = lane(v, i);
-  if (layer && p.strip.map) {
-    const int col = p.strip.map[idx];
+  {
     if (col >= 0) {
       const int64_t at =
           (((int64_t)(t - p.strip.first) * p.strip.count + col) << sh) + b;
