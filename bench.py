@@ -206,6 +206,103 @@ def probe_copy(mib: int = 256, reps: int = 5) -> None:
                       "bytes_written": dst.numel() * 4, "reps": reps}))
 
 
+def workload_3d(nt: int, roi: int = 320, layer: int = 16):
+    """The 3D acoustic case: independent bulk modulus and density.
+
+    ROI roi^3 behind a `layer`-cell PML, dx = 1 mm, order 8, float32. Both
+    fields are smoothed noise (sigma 10), min-max scaled: rho in [900, 1300]
+    (seed 4), vp in [1400, 1800] (seed 5), kappa = rho vp^2. An air sphere of
+    radius roi/16 sits at the centre (rho 1.204, kappa 1.419e5). One Ricker
+    source at the centre of the top face; a 32 x 32 receiver grid on the top
+    plane. Returns the simulation, the (kappa, rho) indicator, the source and
+    the receiver nodes.
+    """
+    from scipy.ndimage import gaussian_filter
+
+    from cuwave_amd import boundary, scalar, wave
+    from cuwave_amd.signals import ricker
+
+    def field(seed, lo, hi):
+        f = gaussian_filter(np.random.default_rng(seed).standard_normal((roi,) * 3), 10)
+        return lo + (hi - lo) * (f - f.min()) / (f.max() - f.min())
+
+    rho, vp = field(4, 900.0, 1300.0), field(5, 1400.0, 1800.0)
+    kappa = rho * vp**2
+    c = (np.arange(roi) - roi / 2) ** 2
+    air = c[:, None, None] + c[None, :, None] + c[None, None, :] <= (roi / 16) ** 2
+    rho[air], kappa[air] = 1.204, 1.419e5
+    width = layer + 1
+    n = roi + 2 * width
+    dt = 0.4 * DX / 1800.0
+    sim = scalar.BulkDensityWave((n, n, n), (DX, DX, DX), nt, dt, precision="float32",
+                                 space_order=8, pml=boundary.Pml(width, 1800.0, 1e-6))
+    pad = [(width, n - width - roi), (width, n - width - roi),
+           (width, sim.Nx_padded[2] - width - roi)]
+    indicator = np.stack([np.pad(kappa, pad, mode="edge"), np.pad(rho, pad, mode="edge")])
+    wavelet = (ricker(np.arange(nt) * dt, 1.0, FREQ) / DX**3)[:, None]
+    source = wave.Source(np.array([[width], [width + roi // 2], [width + roi // 2]]), wavelet)
+    g = np.arange(32) * (roi // 32) + roi // 64 + width
+    sensors = np.stack([np.full(32 * 32, width), np.repeat(g, 32), np.tile(g, 32)])
+    return sim, indicator.astype(np.float32), source, sensors
+
+
+def gpu_run_3d(args):
+    """Forward + reconstruction adjoint of the 3D case: W = 2 n_int N units a step."""
+    import torch
+
+    from cuwave_amd import sensitivity, wave
+    from cuwave_amd.engine import Propagator
+
+    sim, indicator, source, sensors = workload_3d(args.nt)
+    n_int = int(np.prod([n - 2 for n in sim.Nx]))
+    observed = 0.9 * wave.simulate(sim, source, indicator, sensors)[1].cpu().numpy()
+    objective = sensitivity.l2_misfit(observed)
+
+    def step():
+        return sensitivity.reconstruction_sensitivity(sim, source, indicator, sensors,
+                                                      objective)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    Propagator.timings = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    timings, Propagator.timings = Propagator.timings, None
+    per = {}
+    for kind, launches, s, e in timings:
+        tot = per.setdefault(kind, [0.0, 0])
+        tot[0] += s.elapsed_time(e)
+        tot[1] += launches
+    bwd_ms = per["backward"][0] / per["backward"][1]
+    fwd_ms = per["forward"][0] / per["forward"][1]
+    # one shot: the reverse march 20 B, the adjoint 12 B and two accumulators
+    # read and written 16 B a node a level; the forward 20 B a node a step
+    bytes_bwd, bytes_fwd = 48.0 * n_int, 20.0 * n_int
+    achieved = bytes_bwd / (bwd_ms * 1e-3)
+    return {
+        "metric": "Gcell-updates/s (3D acoustic forward+adjoint)",
+        "value": 2.0 * n_int * sim.N * args.steps / elapsed / 1e9,
+        "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "3D acoustic gradient: forward + reconstruction adjoint, "
+                               "independent bulk modulus and density",
+                   "grid": f"{sim.Nx[0]}^3 (ROI 320^3 + 17-cell PML)", "space_order": 8,
+                   "shots": 1, "receivers": 1024, "time_steps": args.nt,
+                   "interior_nodes": n_int, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None},
+        "cpu_baseline": None,
+        "kernels": {"backward_ms": bwd_ms, "forward_ms": fwd_ms,
+                    "backward_GBps": achieved / 1e9,
+                    "forward_GBps": bytes_fwd / (fwd_ms * 1e-3) / 1e9},
+    }
+
+
 def main():
     p = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     p.add_argument("--gpus", type=int, default=1)
@@ -218,9 +315,17 @@ def main():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--probe-copy", action="store_true",
                    help="run only a 256 MiB device copy (counter calibration)")
+    p.add_argument("--workload", choices=("2d", "3d"), default="2d",
+                   help="3d: the 3D acoustic gradient on a 354^3 grid instead (its own line)")
     args = p.parse_args()
     if args.probe_copy:
         probe_copy()
+        return
+    if args.workload == "3d":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU"}))
+            sys.exit(2)
+        print(json.dumps(gpu_run_3d(args)))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the shot batch shards over "
