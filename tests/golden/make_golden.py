@@ -204,7 +204,7 @@ def forward_cases() -> None:
         save(name, **inputs(sim, source, indicator, sensors), field=field, traces=um)
 
 
-def config_one() -> None:
+def disk_forward() -> None:
     """The plain 2D forward: 256^2 region, 20-cell layer, order 8, 2000 steps."""
     roi, pml, dx, c1, c2 = 256, 21, 1e-3, 1500.0, 2000.0
     n = roi + 2 * (pml - 1) + 2
@@ -223,7 +223,7 @@ def config_one() -> None:
     cols = np.round(np.linspace(0, roi - 1, 16)).astype(np.int32) + pml
     sensors = np.stack([np.full(16, pml, dtype=np.int32), cols])
     field, um = simulate(sim, source, indicator, sensors)
-    save("config1_forward", **inputs(sim, source, indicator, sensors),
+    save("disk_298x298_o8_forward", **inputs(sim, source, indicator, sensors),
          field=field, traces=um)
 
 
@@ -321,14 +321,14 @@ def misfit_case() -> None:
 
 
 if __name__ == "__main__":
-    which = set(sys.argv[2:]) or {"stencils", "forward", "config1", "adjoint",
+    which = set(sys.argv[2:]) or {"stencils", "forward", "disk", "adjoint",
                                   "misfit"}
     if "stencils" in which:
         stencil_case()
     if "forward" in which:
         forward_cases()
-    if "config1" in which:
-        config_one()
+    if "disk" in which:
+        disk_forward()
     if "adjoint" in which:
         adjoint_cases()
     if "misfit" in which:

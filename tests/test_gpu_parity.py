@@ -31,7 +31,7 @@ def tol(sim, grad=False):
     return F32_GRAD if grad else F32_FIELD
 
 
-@pytest.mark.parametrize("name", golden_names("fwd_") + ["config1_forward"])
+@pytest.mark.parametrize("name", golden_names("fwd_") + ["disk_298x298_o8_forward"])
 def test_forward_matches_the_reference(name):
     z = golden(name)
     sim = product_sim(z)

@@ -46,7 +46,7 @@ def test_stencil_rows_are_consistent():
             assert abs(float(c @ (2 * np.arange(1, R + 1) - 1.0)) - 1.0) < 1e-12
 
 
-@pytest.mark.parametrize("name", golden_names("fwd_") + ["config1_forward"])
+@pytest.mark.parametrize("name", golden_names("fwd_") + ["disk_298x298_o8_forward"])
 def test_forward_matches_the_reference_bit_for_bit(name):
     z = golden(name)
     sim = oracle_sim(z)
