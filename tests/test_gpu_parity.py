@@ -192,3 +192,26 @@ def test_the_illumination_map_matches_the_oracle():
     for batch in (None, 2):
         illum = wave.illumination(sim, sources, z["indicator"], batch=batch)
         assert rel_l2(illum, expect) < 1e-5, batch
+
+
+def test_a_3d_order_8_float32_adjoint_matches_the_oracle():
+    # the 3D one-lane float adjoint at order 8 (its own register budget)
+    from oracle import driver
+
+    base = golden("adj_pml_3d_o4_f64")
+    z = {k: base[k] for k in base.files}
+    z.update(space_order=np.array(8), precision=np.array("float32"), N=np.array(40))
+    z["indicator"] = base["indicator"].astype(np.float32)
+    z["src_sig"] = base["src_sig"][:40].astype(np.float32)
+    sim, osim = product_sim(z), oracle_sim(z)
+    print("case 3d order 8 float32 adjoint", flush=True)
+    src = wave.Source(z["src_pos"], z["src_sig"])
+    observed = np.zeros((sim.N, z["sensors"].shape[1]), np.float32)
+    cost, g, traces, _ = S.sensitivity(sim, src, z["indicator"], z["sensors"],
+                                       S.l2_misfit(observed))
+    ocost, og, otraces, _ = driver.sensitivity(osim, z["src_pos"], z["src_sig"], z["indicator"],
+                                               z["sensors"], driver.l2_misfit(observed))
+    np.testing.assert_array_equal(host(traces), otraces)
+    assert abs(cost - ocost) <= F32_FIELD * abs(ocost)
+    assert rel_max(g["mass"], og["mass"]) < F32_GRAD
+    assert rel_max(g["stiff"], og["stiff"]) < F32_GRAD
