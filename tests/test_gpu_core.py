@@ -130,3 +130,23 @@ def test_the_illumination_map_with_core_blocks_matches_the_oracle():
     expect = driver.illumination(osim, [(s.position, s.signal) for s in sources],
                                  z["indicator"])
     assert rel_max(illum, expect) < 1e-5
+
+
+def test_at_full_size_a_batch_of_32_steps_each_shot_as_alone():
+    # the benchmark's grid (1090 x 1090, order 8, CPML), all 32 shots in one
+    # launch: three of them bitwise as when run alone, fields and traces
+    import sys
+
+    from helpers import ROOT
+
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    sim, true, _, sources, sensors = bench.workload(32, 120)
+    print("case full size 1090x1090 32 shots 120 steps", flush=True)
+    fields, records = wave.simulate_shots(sim, sources, true, sensors)
+    for k in (0, 17, 31):
+        f, r = wave.simulate(sim, sources[k], true, sensors)
+        assert host(f).any()
+        np.testing.assert_array_equal(host(fields[k]), host(f), err_msg=f"shot {k}")
+        np.testing.assert_array_equal(host(records[k]), host(r), err_msg=f"shot {k}")
