@@ -150,3 +150,35 @@ def test_at_full_size_a_batch_of_32_steps_each_shot_as_alone():
         assert host(f).any()
         np.testing.assert_array_equal(host(fields[k]), host(f), err_msg=f"shot {k}")
         np.testing.assert_array_equal(host(records[k]), host(r), err_msg=f"shot {k}")
+
+
+def test_at_full_size_the_batched_gradient_sums_the_shots_and_matches_the_oracle():
+    # the benchmark's grid, 120 steps: the gradient of 8 shots in one batch is
+    # the sum of the 8 run alone, and one shot's gradient is the oracle's
+    import sys
+
+    from oracle import driver
+
+    from helpers import ROOT
+
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    sim, true, start, sources, nodes = bench.workload(8, 120)
+    print("case full size gradient 1090x1090 8 shots 120 steps", flush=True)
+    coords = nodes.T * np.array(sim.dx)
+    sensors = utils.Sensors(sim, coords)
+    observed = utils.measure(sim, sources, true, sensors)
+    cost, g = utils.misfit_gradient(sim, sources, start, sensors, observed)
+    cost1, g1 = utils.misfit_gradient(sim, sources, start, sensors, observed, batch=1)
+    assert abs(cost - cost1) <= 1e-5 * abs(cost1)
+    assert rel_max(g, host(g1)) < F32_GRAD
+    osim = driver.Sim("AcousticWave", sim.Nx, sim.dx, sim.N, sim.dt, "float32", 8,
+                      materials=(1.0, 1.0, bench.C1**2, bench.C2**2),
+                      pml=driver.Pml(sim.pml.width, sim.pml.wavespeed, sim.pml.reflection))
+    s = sources[0]
+    c0, g0 = utils.misfit_gradient(sim, [s], start, sensors, observed[:1])
+    oc, og = driver.misfit_gradient(osim, [(s.position, s.signal)], start,
+                                    driver.Sensors(osim, coords), [host(observed[0])])
+    assert abs(c0 - oc) <= 1e-5 * abs(oc)
+    assert rel_max(g0, og) < F32_GRAD
