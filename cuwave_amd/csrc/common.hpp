@@ -320,10 +320,6 @@ __device__ __forceinline__ double bload<double>(const Buf &b, uint32_t vo,
   return __builtin_bit_cast(double,
                             __builtin_amdgcn_raw_buffer_load_b64(b.r, vo, so, 0));
 }
-__device__ __forceinline__ int bload_int(const Buf &b, uint32_t vo,
-                                         uint32_t so) {
-  return (int)__builtin_amdgcn_raw_buffer_load_b32(b.r, vo, so, 0);
-}
 __device__ __forceinline__ void bstore(const Buf &b, float v, uint32_t vo,
                                        uint32_t so) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), b.r,

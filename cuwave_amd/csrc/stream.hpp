@@ -129,28 +129,6 @@ __device__ __forceinline__ bool box_item(const Box &box, const Geom<T> &g,
   return EDGE == points;
 }
 
-// the profile of axis d at node a and at the half point below it
-template <typename T> struct Prof {
-  T an, bn, ah, bh, ah1, bh1;
-};
-template <typename T>
-__device__ __forceinline__ Prof<T> profile_at(const T *pr, int N, int a) {
-  return {pr[a], pr[N + a], pr[2 * N + a], pr[3 * N + a], pr[2 * N + a - 1],
-          pr[3 * N + a - 1]};
-}
-
-// taps[0 .. 2R] from the fast-axis taps x[0 .. 2R) (k = -R .. -1, 1 .. R)
-// and the centre
-template <typename T, int R>
-__device__ __forceinline__ void row_line(T *t, const T *x, T c) {
-#pragma unroll
-  for (int k = 0; k < R; ++k) {
-    t[k] = x[k];
-    t[R + 1 + k] = x[R + k];
-  }
-  t[R] = c;
-}
-
 // Copy `bytes` contiguous bytes from global `src` to LDS `dst` by DMA, the
 // block's four waves taking 64-lane chunks in turn from wave `first`; lanes
 // past the end stay idle. SIZE is the bytes a lane moves (4 or 16).
@@ -222,27 +200,14 @@ __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
   const int xl = min(max(x, box.x0), box.x1 - 1);
   const int tx = __builtin_amdgcn_readfirstlane(xl) / TileShape<2>::TX;
   const int s0 = g.s[0], np1 = g.np[1];
-  const int N0 = g.n[0], N1 = g.n[1];
-  const int L0 = p.pml.lo[0], H0 = p.pml.hi[0];
-  const int L1 = p.pml.lo[1], H1 = p.pml.hi[1];
-  const int64_t size0 = ((int64_t)(L0 + H0) * g.np[1]) << SH;
-  const int64_t size1 = ((int64_t)(L1 + H1) * g.np[0]) << SH;
-  const int prow1 = EDGE && L1 + H1 ? slab_row(xl, N1, L1, H1) : -1;
-  Prof<T> pf1{};
-  if (prow1 >= 0)
-    pf1 = profile_at(p.pml.profile[1], N1, xl);
   const T *u1 = p.u.at(t - 1);
   const uint32_t LB = (uint32_t)((g.nodes << SH) * sizeof(T));
   const Buf U0 = make_buf(p.u.at(t - 2), LB);
   const Buf U2 = make_buf(p.u.at(t), LB);
-  const Buf MAP = make_buf(p.strip.map, EDGE && p.strip.map
-                                            ? (uint32_t)(g.nodes * 4)
-                                            : 0u);
   const T *cy = p.cell, *cx = p.cell + g.nodes, *inv = p.cell + 2 * g.nodes;
-  // this thread's element and node at row ya, and their row strides
+  // this thread's element at row ya, and its row stride
   const uint32_t frow = ((uint32_t)s0 << SH) * sizeof(T);
   const uint32_t e0 = ((((uint32_t)ya * s0 + xl) << SH) + b) * sizeof(T);
-  const uint32_t m0 = ((uint32_t)ya * s0 + xl) * 4u, mrow = (uint32_t)s0 * 4u;
   const bool has_src = EDGE && chunk_has(p.src, g, ya, yb, tx, tx);
   const bool has_rec = EDGE && chunk_has(p.rec, g, ya, yb, tx, tx);
   const uint32_t src_rows = has_src ? point_rows(p.src, g, ya, yb, tx) : 0u;
@@ -269,7 +234,6 @@ __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
     stage_row<T, R, W>(u1, y, xb, s0, np1, ring, 0);
   stage_mat(ya);
   T u0n = bload<T>(U0, e0, 0);
-  int coln = EDGE ? bload_int(MAP, m0, 0) : -1;
   wait_vmcnt<0>();
   __syncthreads();
 
@@ -290,10 +254,7 @@ __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
     stage_row<T, R, W>(u1, y + R + 1, xb, s0, np1, ring, 0);
     stage_mat(y + 1);
     const T u0c = u0n;
-    const int col = coln;
     u0n = bload<T>(U0, e0 + (uint32_t)(y + 1 - ya) * frow, 0);
-    if (EDGE)
-      coln = bload_int(MAP, m0 + (uint32_t)(y + 1 - ya) * mrow, 0);
     __builtin_amdgcn_sched_barrier(0);
 
     tf[R] = q[R];
@@ -304,24 +265,6 @@ __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
       div[1] = flux<T, R>(tf, gp1, gm1, g.op);
       const T lap = combine_axes<T, 2, R>(div, g.f);
       v = -u0c + (T)2 * q[R] + mi * lap;
-      const int prow0 = EDGE && L0 + H0 ? slab_row(y, N0, L0, H0) : -1;
-      if (prow0 >= 0) {
-        const Prof<T> pf = profile_at(p.pml.profile[0], N0, y);
-        T Dp, Dm;
-        cell_diffs<T, R>(q, g.op, Dp, Dm);
-        v += pml_forward(p.pml.aux[0] +
-                             (((int64_t)prow0 * g.np[1] + x) << SH) + b,
-                         size0, Dp, Dm, gp0, gm0, g.f[0], mi, pf.an, pf.bn,
-                         pf.ah, pf.bh, pf.ah1, pf.bh1);
-      }
-      if (prow1 >= 0) {
-        T Dp, Dm;
-        cell_diffs<T, R>(tf, g.op, Dp, Dm);
-        v += pml_forward(p.pml.aux[1] +
-                             (((int64_t)y * (L1 + H1) + prow1) << SH) + b,
-                         size1, Dp, Dm, gp1, gm1, g.f[1], mi, pf1.an, pf1.bn,
-                         pf1.ah, pf1.bh, pf1.ah1, pf1.bh1);
-      }
       const int nc = y * s0 + x;
       const int tile = (y / TileShape<2>::TY) * g.tf + tx;
       if ((src_rows >> (y - ya)) & 1u)
@@ -332,14 +275,6 @@ __device__ __forceinline__ void forward_stream(const FwdParams<T> &p,
         for (int k = p.rec.starts[tile]; k < p.rec.starts[tile + 1]; ++k)
           if (p.rec.takes(k, nc, b))
             p.record[p.rec.at(t, k, b)] = v;
-      if ((prow0 >= 0 || prow1 >= 0) && p.strip.map && col >= 0) {
-        const int64_t at =
-            (((int64_t)(t - p.strip.first) * p.strip.count + col) << SH) + b;
-        if (p.strip.bf16)
-          ((uint16_t *)p.strip.record)[at] = bf16_bits((float)v);
-        else
-          ((T *)p.strip.record)[at] = v;
-      }
     }
     // every thread stores (off the box, out of range: dropped), so the
     // count of operations after the staging copies is fixed
@@ -374,11 +309,11 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
   if (!box_item<EDGE>(box, g, p.src, reverse, p.rec, C, bid, nb, ya, yb, xb))
     return;
   // the staged rows of lambda^(n+1) and u^n; per material row slot the
-  // fast-axis cells of x - R .. [C + 2R], 1 / m [C], the slow-axis cell of
-  // row y + R - 1 [C], the accumulators [C] each
+  // fast-axis cells of x - R .. [C + 2R], 1 / m [C] and the slow-axis cell
+  // of row y + R - 1 [C]
   __shared__ __align__(16) char lring[S::RING * S::SEGB];
   __shared__ __align__(16) char uring[S::RING * S::SEGB];
-  __shared__ __align__(16) T lmat[2][5][C + 2 * R];
+  __shared__ __align__(16) T lmat[2][3][C + 2 * R];
   const int b = threadIdx.x & (W - 1);
   const int xc = (int)(threadIdx.x >> SH);
   const int x = xb + xc;
@@ -386,21 +321,8 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
   const int xl = min(max(x, box.x0), box.x1 - 1);
   const int tx = __builtin_amdgcn_readfirstlane(xl) / TileShape<2>::TX;
   const int s0 = g.s[0], np1 = g.np[1];
-  const int N0 = g.n[0], N1 = g.n[1];
-  const int L0 = p.pml.lo[0], H0 = p.pml.hi[0];
-  const int L1 = p.pml.lo[1], H1 = p.pml.hi[1];
-  const int64_t size0 = ((int64_t)(L0 + H0) * g.np[1]) << SH;
-  const int64_t size1 = ((int64_t)(L1 + H1) * g.np[0]) << SH;
   const int m = p.steps - 1 - n;
   const bool chain = W == 1 && !p.combine;
-  // the column: its slab row along the fast axis, whether the transposed
-  // taps of that axis reach the slab, its profile
-  const int prow1 = EDGE && L1 + H1 ? slab_row(xl, N1, L1, H1) : -1;
-  const bool ext1 =
-      EDGE && ((L1 && xl - 1 < L1 + R) || (H1 && N1 - 2 - xl < H1 + R));
-  Prof<T> pf1{};
-  if (prow1 >= 0)
-    pf1 = profile_at(p.pml.profile[1], N1, xl);
   const T *l1 = p.lam.at(n + 1), *u1 = p.u.at(n);
   const uint32_t LB = (uint32_t)((g.nodes << SH) * sizeof(T));
   const uint32_t NB = (uint32_t)(g.nodes * sizeof(T));
@@ -411,15 +333,11 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
   const Buf GM = make_buf(p.gmass, NB);
   const Buf GS = make_buf(p.gstiff, p.combine ? 0u : NB);
   const Buf SK = make_buf(p.stiff, STIFF ? NB : 0u);
-  const Buf MAP = make_buf(p.strip.map, EDGE && reverse && p.strip.map
-                                            ? (uint32_t)(g.nodes * 4)
-                                            : 0u);
   const T *cy = p.cell, *cx = p.cell + g.nodes, *inv = p.cell + 2 * g.nodes;
   const uint32_t frow = ((uint32_t)s0 << SH) * sizeof(T);
   const uint32_t e0 = ((((uint32_t)ya * s0 + xl) << SH) + b) * sizeof(T);
   const uint32_t n0 = ((uint32_t)ya * s0 + xl) * sizeof(T);
   const uint32_t nrow = (uint32_t)s0 * sizeof(T);
-  const uint32_t m0 = ((uint32_t)ya * s0 + xl) * 4u, mrow = (uint32_t)s0 * 4u;
   const bool has_rec = EDGE && chunk_has(p.rec, g, ya, yb, tx, tx);
   const bool has_src = EDGE && reverse && chunk_has(p.src, g, ya, yb, tx, tx);
   const uint32_t rec_rows = has_rec ? point_rows(p.rec, g, ya, yb, tx) : 0u;
@@ -436,7 +354,6 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
   // the thread's own values of a row, one row ahead
   struct Own {
     T lo, uo, accm, accs, sk[5];
-    int col;
   };
   auto own = [&](Own &o, int y) {
     const uint32_t e = e0 + (uint32_t)(y - ya) * frow;
@@ -452,7 +369,6 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
       o.sk[3] = bload<T>(SK, nn + (uint32_t)sizeof(T), 0);
       o.sk[4] = bload<T>(SK, nn - (uint32_t)sizeof(T), 0);
     }
-    o.col = EDGE ? bload_int(MAP, m0 + (uint32_t)(y - ya) * mrow, 0) : -1;
   };
 
   // queues along the slow axis: lambda^(n + 1) and u^n rows y - R .. y + R,
@@ -528,12 +444,6 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
         clo0[j - 1] = qc[R - j];
       }
       const T lo_ = o.lo;
-      // this row along the slow axis: slab row, reach of the transposed taps
-      const int prow0 = EDGE && L0 + H0 ? slab_row(y, N0, L0, H0) : -1;
-      const bool ext0 =
-          EDGE && ((L0 && y - 1 < L0 + R) || (H0 && N0 - 2 - y < H0 + R));
-      const bool layer = prow0 >= 0 || prow1 >= 0;
-      const bool grad = !(p.skip_layer && layer);
       const T accm = o.accm, accs = o.accs;
       const T lc = ql[R], uc = qu[R];
       T div_l = 0;
@@ -544,36 +454,13 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
       T cm = chain ? accm : (T)0;
       cm = cm - p.mf * uc * (ln - (T)2 * lc + lo_);
       T cs = chain ? accs : (T)0;
-      if (STIFF && grad) {
+      if (STIFF) {
         T gsum = 0;
         gsum += stiffness_density<T, R>(qu, ql, lc, o.sk[0], o.sk[1], o.sk[2],
                                         g.f[0], g.op);
         gsum += stiffness_density<T, R>(ut1, lt1, lc, o.sk[0], o.sk[3],
                                         o.sk[4], g.f[1], g.op);
         cs = cs - p.mf * gsum;
-      }
-      // the transposed layer: the slab nodes' tap fields within reach
-      if (ext0) {
-        const T sum =
-            gather<T, R>(p.pml.eta[0][n & 1] + b, size0, g.op, [&](int k) {
-              const int row = slab_row(y + k, N0, L0, H0);
-              return row >= 0 ? ((int64_t)row * g.np[1] + x) << SH
-                              : (int64_t)-1;
-            });
-        const T delta = mi * sum;
-        ln += delta;
-        cm -= p.mf * uc * delta;
-      }
-      if (ext1) {
-        const T sum =
-            gather<T, R>(p.pml.eta[1][n & 1] + b, size1, g.op, [&](int k) {
-              const int row = slab_row(x + k, N1, L1, H1);
-              return row >= 0 ? ((int64_t)y * (L1 + H1) + row) << SH
-                              : (int64_t)-1;
-            });
-        const T delta = mi * sum;
-        ln += delta;
-        cm -= p.mf * uc * delta;
       }
       const int tile = (y / TileShape<2>::TY) * g.tf + tx;
       if ((rec_rows >> (y - ya)) & 1u)
@@ -583,59 +470,29 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
             ln += load;
             cm += -p.mf * uc * load;
           }
-      // the memory the next level gathers, from this level's final lambda
-      if (prow0 >= 0) {
-        const Prof<T> pf0 = profile_at(p.pml.profile[0], N0, y);
-        const int64_t i = (((int64_t)prow0 * g.np[1] + x) << SH) + b;
-        pml_adjoint_update(p.pml.eta[0][n & 1] + i,
-                           p.pml.eta[0][(n + 1) & 1] + i, size0, ln, chi0[0],
-                           clo0[0], g.f[0], pf0.an, pf0.bn, pf0.ah, pf0.bh,
-                           pf0.ah1, pf0.bh1);
-      }
-      if (prow1 >= 0) {
-        const int64_t i = (((int64_t)y * (L1 + H1) + prow1) << SH) + b;
-        pml_adjoint_update(p.pml.eta[1][n & 1] + i,
-                           p.pml.eta[1][(n + 1) & 1] + i, size1, ln, chi1[0],
-                           clo1[0], g.f[1], pf1.an, pf1.bn, pf1.ah, pf1.bh,
-                           pf1.ah1, pf1.bh1);
-      }
-      if (grad) {
-        // the gradient as add_gradient forms it, stored below
-        if (p.combine) {
-          gm_new = accm + shot_sum(p.jm * cm + p.js * cs, W);
-          gstore = b == 0;
-        } else if (W == 1) {
-          gm_new = cm;
-          gs_new = cs;
-          gstore = true;
-        } else {
-          gm_new = accm + shot_sum(cm, W);
-          gs_new = accs + shot_sum(cs, W);
-          gstore = b == 0;
-        }
+      // the gradient as add_gradient forms it, stored below
+      if (p.combine) {
+        gm_new = accm + shot_sum(p.jm * cm + p.js * cs, W);
+        gstore = b == 0;
+      } else if (W == 1) {
+        gm_new = cm;
+        gs_new = cs;
+        gstore = true;
+      } else {
+        gm_new = accm + shot_sum(cm, W);
+        gs_new = accs + shot_sum(cs, W);
+        gstore = b == 0;
       }
       if (reverse) {
-        const int strip_col = layer && p.strip.map ? o.col : -1;
-        if (strip_col >= 0) {
-          // a layer node is replayed from the forward sweep's record
-          const int64_t at =
-              (((int64_t)(n - 1 - p.strip.first) * p.strip.count + strip_col)
-               << SH) +
-              b;
-          v = p.strip.bf16
-                  ? (T)bf16_value(((const uint16_t *)p.strip.record)[at])
-                  : ((const T *)p.strip.record)[at];
-        } else {
-          T div[2];
-          div[0] = flux<T, R>(qu, chi0[0], clo0[0], g.op);
-          div[1] = flux<T, R>(ut1, chi1[0], clo1[0], g.op);
-          const T lap = combine_axes<T, 2, R>(div, g.f);
-          v = -o.uo + (T)2 * uc + mi * lap;
-          if ((src_rows >> (y - ya)) & 1u)
-            for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
-              if (p.src.takes(k, nc, b))
-                v += p.src.weight[k] * p.signal[p.src.at(n + 1, k, b)];
-        }
+        T div[2];
+        div[0] = flux<T, R>(qu, chi0[0], clo0[0], g.op);
+        div[1] = flux<T, R>(ut1, chi1[0], clo1[0], g.op);
+        const T lap = combine_axes<T, 2, R>(div, g.f);
+        v = -o.uo + (T)2 * uc + mi * lap;
+        if ((src_rows >> (y - ya)) & 1u)
+          for (int k = p.src.starts[tile]; k < p.src.starts[tile + 1]; ++k)
+            if (p.src.takes(k, nc, b))
+              v += p.src.weight[k] * p.signal[p.src.at(n + 1, k, b)];
       }
     }
     // four stores every thread issues (off the box or without a value, out
