@@ -28,6 +28,50 @@ inline int hip_status(const char *where) {
   return 0;
 }
 
+// The second stream a 2D sweep runs its edge launches on, beside the core
+// launches on the caller's stream, and the two events that order them. A
+// step's core and edge launches write disjoint nodes and read only levels
+// the step before finished, so each waits only for the other's previous
+// step. One set a device and host thread, kept for the process's life; the
+// stream is non-blocking, so the legacy null stream does not serialise it.
+struct Side {
+  hipStream_t stream = nullptr;
+  hipEvent_t core = nullptr, edge = nullptr;
+};
+
+inline Side *side_stream() {
+  constexpr int kDevices = 64;
+  thread_local Side sides[kDevices];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevices) {
+    fail("side stream: no current device");
+    return nullptr;
+  }
+  Side &s = sides[dev];
+  if (!s.stream) {
+    if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&s.core, hipEventDisableTiming) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&s.edge, hipEventDisableTiming) !=
+            hipSuccess) {
+      fail("side stream: creation failed");
+      s = Side{};
+      return nullptr;
+    }
+  }
+  return &s;
+}
+
+// the end of a step: each stream's next launch waits for the other's last;
+// after the final step the caller's stream holds every launch
+inline void meet(Side &s, hipStream_t main) {
+  hipEventRecord(s.core, main);
+  hipEventRecord(s.edge, s.stream);
+  hipStreamWaitEvent(main, s.edge, 0);
+  hipStreamWaitEvent(s.stream, s.core, 0);
+}
+
 inline int lanes_shift(int lanes) {
   int k = 0;
   while ((1 << k) < lanes)
