@@ -182,3 +182,43 @@ def test_at_full_size_the_batched_gradient_sums_the_shots_and_matches_the_oracle
                                     driver.Sensors(osim, coords), [host(observed[0])])
     assert abs(c0 - oc) <= 1e-5 * abs(oc)
     assert rel_max(g0, og) < F32_GRAD
+
+
+def test_on_a_caller_stream_the_frame_launches_join_it():
+    # a 2D sweep runs its frame kernels on a stream of the library's own: it
+    # must start after the caller stream's earlier work (the inputs it fills)
+    # and end before that stream's next work (here a copy of the outputs,
+    # enqueued with no synchronisation between)
+    import sys
+
+    import torch
+
+    from helpers import ROOT
+
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    sim, true, _, sources, sensors = bench.workload(32, 120)
+    print("case caller stream 1090x1090 32 shots 120 steps", flush=True)
+    fields, records = wave.simulate_shots(sim, sources, true, sensors)
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        f2, r2 = wave.simulate_shots(sim, sources, true, sensors)
+        f2c, r2c = f2.clone(), r2.clone()
+    side.synchronize()
+    for k in range(len(sources)):
+        np.testing.assert_array_equal(host(f2c[k]), host(fields[k]), err_msg=f"shot {k}")
+        np.testing.assert_array_equal(host(r2c[k]), host(records[k]), err_msg=f"shot {k}")
+    # the adjoint sweep too: the gradient is the same bit for bit
+    _, _, start, _, nodes = bench.workload(32, 120)
+    sensors = utils.Sensors(sim, nodes.T * np.array(sim.dx))
+    observed = utils.measure(sim, sources, true, sensors)
+    cost, g = utils.misfit_gradient(sim, sources, start, sensors, observed)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        cost2, g2 = utils.misfit_gradient(sim, sources, start, sensors, observed)
+        g2c = g2.clone()
+    side.synchronize()
+    assert cost2 == cost
+    np.testing.assert_array_equal(host(g2c), host(g))
