@@ -21,10 +21,11 @@
  *   write whole.
  * - `dtype` 0 is float32, 1 is float64; every field of a call has it.
  * - Launches are asynchronous on `stream` (a hipStream_t; 0 the null stream).
- *   A 2D sweep also launches its frame kernels on a non-blocking stream of
- *   the library's own (one a device and host thread, kept for the process's
- *   life), ordered after `stream`'s earlier work; `stream` waits for them
- *   before its next work, so to the caller the call stays one stream's.
+ *   A 2D adjoint sweep also launches its frame kernels on a non-blocking
+ *   stream of the library's own (one a device and host thread, kept for the
+ *   process's life), ordered after `stream`'s earlier work; `stream` waits
+ *   for them before its next work, so to the caller the call stays one
+ *   stream's.
  * - A return of 0 is success; anything else is an error whose text
  *   `cw_last_error()` returns (thread-local). Arguments are checked before
  *   any launch, so an error leaves the device untouched.
