@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 4
+ABI_VERSION = 5
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -146,6 +146,7 @@ class SuperposedArgs(ctypes.Structure):
 SIGNATURES = {
     "cw_version": ([], I32),
     "cw_last_error": ([], ctypes.c_char_p),
+    "cw_set_side_stream": ([I32], I32),
     "cw_tile_count": ([ctypes.POINTER(Grid), ctypes.POINTER(I32)], I32),
     "cw_point_lists": ([ctypes.POINTER(Grid), VP, I32, VP, VP], I32),
     "cw_scalar_prepare": ([ctypes.POINTER(Grid), ctypes.POINTER(Materials), VP], I32),

@@ -79,13 +79,21 @@ def require_off_held_walls(sim: Simulation, rows: np.ndarray) -> None:
 
 @dataclass(frozen=True)
 class Pml:
-    """A convolutional perfectly matched layer behind `faces`, for `Simulation.pml`."""
+    """A convolutional perfectly matched layer behind `faces`, for `Simulation.pml`.
 
-    width: int  # layer depth in nodes, the wall node included
-    wavespeed: float  # fastest wave speed in the layer, which sizes the damping
-    reflection: float = 1e-6  # target reflection at normal incidence
-    frequency: float = 0.0  # CFS shift pi * frequency through the layer, 0 for none
-    faces: tuple[int, ...] | None = None  # the `2 * axis + side` codes, None for all
+    `width` counts the layer's nodes from its front to the wall node, both
+    included. `wavespeed` is the fastest speed that reaches the layer and sets
+    the damping that gives a normal-incidence echo of `reflection`. A non-zero
+    `frequency` adds the complex-frequency shift alpha = pi * frequency
+    throughout the layer. `faces` names the lined faces by `2 * axis + side`
+    (None lines every face).
+    """
+
+    width: int
+    wavespeed: float
+    reflection: float = 1e-6
+    frequency: float = 0.0
+    faces: tuple[int, ...] | None = None
 
 
 def _canonical_faces(ndim: int, faces: Sequence[int] | None) -> tuple[int, ...]:
@@ -111,10 +119,10 @@ def validate_pml(sim: Simulation) -> None:
 
 
 def pml_widths(sim: Simulation) -> list[tuple[int, int]]:
-    """Per axis, the (low, high) layer depths in nodes, 0 for a face left unlined."""
-    faces = _canonical_faces(sim.ndim, sim.pml.faces)
-    return [tuple(sim.pml.width if 2 * d + side in faces else 0 for side in (0, 1))
-            for d in range(sim.ndim)]
+    """Layer depth behind the low and the high face of every axis (0 where unlined)."""
+    lined = set(_canonical_faces(sim.ndim, sim.pml.faces))
+    depth = sim.pml.width
+    return [(depth * (2 * d in lined), depth * (2 * d + 1 in lined)) for d in range(sim.ndim)]
 
 
 def pml_profiles(sim: Simulation) -> list[np.ndarray | None]:
@@ -153,15 +161,19 @@ def pml_profiles(sim: Simulation) -> list[np.ndarray | None]:
 
 
 def pml_nodes(sim: Simulation) -> np.ndarray:
-    """Interior nodes inside the layer, where the step carries memory terms."""
-    inside = np.zeros(sim.Nx_padded, dtype=bool)
+    """Mask of the interior nodes that carry layer memory (on some lined axis within the
+    layer depth of its wall node), over the padded grid."""
+    mask = np.zeros(sim.Nx_padded, dtype=bool)
     if sim.pml is None:
-        return inside
+        return mask
+    inner = np.ones(sim.Nx_padded, dtype=bool)
     for d, (lo, hi) in enumerate(pml_widths(sim)):
-        index = [slice(None)] * sim.ndim
-        for rows in (slice(1, 1 + lo), slice(sim.Nx[d] - 1 - hi, sim.Nx[d] - 1)):
-            index[d] = rows
-            inside[tuple(index)] = True
-    interior = np.zeros_like(inside)
-    interior[tuple(slice(1, n - 1) for n in sim.Nx)] = True
-    return inside & interior
+        n = sim.Nx[d]
+        a = np.arange(sim.Nx_padded[d])
+        shape = [1] * sim.ndim
+        shape[d] = -1
+        # wall node 1 and the lo - 1 rows past it; wall node n - 2 and hi - 1 before it
+        layer_rows = ((a >= 1) & (a <= lo)) | ((a >= n - 1 - hi) & (a <= n - 2))
+        mask |= layer_rows.reshape(shape)
+        inner &= ((a >= 1) & (a <= n - 2)).reshape(shape)
+    return mask & inner

@@ -32,36 +32,40 @@ inline int hip_status(const char *where) {
 // launches on the caller's stream, and the two events that order them. A
 // step's core and edge launches write disjoint nodes and read only levels
 // the step before finished, so each waits only for the other's previous
-// step. One set a device and host thread, kept for the process's life; the
-// stream is non-blocking, so the legacy null stream does not serialise it.
+// step. The stream is non-blocking, so the legacy null stream does not
+// serialise it.
+//
+// Sets live for the process in a pool per device (scalar.hip): a sweep
+// leases one for the time it takes to enqueue its launches and hands it
+// back, so a process holds as many sets as it ever had sweeps enqueueing at
+// once, whatever its host threads. Work a set's stream still holds from a
+// previous lease is ordered before the next lease's, which only delays it.
+// `cw_set_side_stream(0)` runs every launch on the caller's stream instead.
 struct Side {
   hipStream_t stream = nullptr;
   hipEvent_t core = nullptr, edge = nullptr;
+  int device = -1;
 };
 
-inline Side *side_stream() {
-  constexpr int kDevices = 64;
-  thread_local Side sides[kDevices];
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevices) {
-    fail("side stream: no current device");
-    return nullptr;
+// a set of the current device (nullptr on error, the text in cw_last_error),
+// or nullptr with *off set where side streams are switched off
+Side *lease_side(bool *off);
+void return_side(Side *s);
+
+struct SideLease {
+  Side *side = nullptr;
+  bool off = false;
+  explicit SideLease(bool want) {
+    if (want)
+      side = lease_side(&off);
   }
-  Side &s = sides[dev];
-  if (!s.stream) {
-    if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) !=
-            hipSuccess ||
-        hipEventCreateWithFlags(&s.core, hipEventDisableTiming) !=
-            hipSuccess ||
-        hipEventCreateWithFlags(&s.edge, hipEventDisableTiming) !=
-            hipSuccess) {
-      fail("side stream: creation failed");
-      s = Side{};
-      return nullptr;
-    }
+  ~SideLease() {
+    if (side)
+      return_side(side);
   }
-  return &s;
-}
+  SideLease(const SideLease &) = delete;
+  SideLease &operator=(const SideLease &) = delete;
+};
 
 // the end of a step: each stream's next launch waits for the other's last;
 // after the final step the caller's stream holds every launch

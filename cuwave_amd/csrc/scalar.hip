@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -21,6 +23,61 @@ int fail(const std::string &what) {
   return 1;
 }
 
+namespace {
+constexpr int kDevices = 64;
+std::mutex g_side_mutex;
+std::vector<Side *> g_side_free[kDevices];
+std::atomic<int> g_side_enabled{1};
+} // namespace
+
+Side *lease_side(bool *off) {
+  *off = g_side_enabled.load() == 0;
+  if (*off)
+    return nullptr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevices) {
+    fail("side stream: no current device");
+    return nullptr;
+  }
+  {
+    std::lock_guard<std::mutex> hold(g_side_mutex);
+    if (!g_side_free[dev].empty()) {
+      Side *s = g_side_free[dev].back();
+      g_side_free[dev].pop_back();
+      return s;
+    }
+  }
+  Side *s = new Side;
+  s->device = dev;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&s->core, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->edge, hipEventDisableTiming) != hipSuccess) {
+    fail("side stream: creation failed");
+    if (s->stream)
+      (void)hipStreamDestroy(s->stream);
+    if (s->core)
+      (void)hipEventDestroy(s->core);
+    if (s->edge)
+      (void)hipEventDestroy(s->edge);
+    delete s;
+    return nullptr;
+  }
+  return s;
+}
+
+void return_side(Side *s) {
+  std::lock_guard<std::mutex> hold(g_side_mutex);
+  g_side_free[s->device].push_back(s);
+}
+
+int side_enabled() { return g_side_enabled.load(); }
+void set_side_enabled(int on) { g_side_enabled.store(on ? 1 : 0); }
+
+} // namespace cwh
+
+namespace cwh {
+int side_enabled();
+void set_side_enabled(int on);
 } // namespace cwh
 
 namespace {
@@ -157,11 +214,19 @@ template <typename T, int NDIM, int R> struct Superposed {
   }
 };
 
+int check_forward(const cw_forward_args &a, int32_t t0, int32_t t1);
+
 } // namespace
 
 extern "C" {
 
 int cw_version(void) { return CW_ABI_VERSION; }
+
+int cw_set_side_stream(int32_t enabled) {
+  const int was = cwh::side_enabled();
+  cwh::set_side_enabled(enabled);
+  return was;
+}
 
 const char *cw_last_error(void) { return cwh::g_error.c_str(); }
 
@@ -228,6 +293,19 @@ int cw_scalar_forward(const cw_forward_args *args, int32_t t0, int32_t t1,
     return e;
   if (!args->mat.prep)
     return fail("forward needs the prepared materials");
+  if (int e = check_forward(*args, t0, t1))
+    return e;
+  if (t0 == t1)
+    return 0;
+  return dispatch<Forward>(args->grid, *args, t0, t1, (hipStream_t)stream);
+}
+
+} // extern "C"
+
+namespace {
+
+int check_forward(const cw_forward_args &a0, int32_t t0, int32_t t1) {
+  const cw_forward_args *args = &a0;
   if (int e = check_levels(args->u, "u"))
     return e;
   if (int e = check_layer(args->grid, args->pml, true))
@@ -249,10 +327,12 @@ int cw_scalar_forward(const cw_forward_args *args, int32_t t0, int32_t t1,
   if (args->strip.map && t0 < t1 &&
       (t0 < args->strip.first || t1 > args->strip.first + args->strip.rows))
     return fail("the strip record does not hold steps t0 .. t1");
-  if (t0 == t1)
-    return 0;
-  return dispatch<Forward>(args->grid, *args, t0, t1, (hipStream_t)stream);
+  return 0;
 }
+
+} // namespace
+
+extern "C" {
 
 int cw_scalar_backward(const cw_backward_args *args, int32_t n0, int32_t n1,
                        void *stream) {

@@ -17,6 +17,7 @@ as torch tensors on the GPU.
 
 from __future__ import annotations
 
+import itertools
 import math
 import warnings
 
@@ -24,7 +25,7 @@ import numpy as np
 import scipy.ndimage as ndi
 import torch
 
-from .boundary import pml_nodes, require_off_held_walls
+from .boundary import pml_nodes
 from .engine import Propagator, sensor_points, source_points
 from .wave import Simulation, Source, as_numpy
 
@@ -76,9 +77,10 @@ def adjoint_signal(sim: Simulation, dphi, sensors, scale: float = 1.0, delay: in
 
 # --------------------------------------------------------------------- helpers
 def _require(sim: Simulation, source: Source, sensors) -> None:
+    """The reference's checks before a sensitivity run: points off the ghosts
+    (cuwave/sensitivity.py:430-432, 546-548)."""
     sim.require_interior(sensors, "sensor")
     sim.require_interior(source.position, "source")
-    require_off_held_walls(sim, np.asarray(as_numpy(source.position)))
 
 
 def _finalize(sim: Simulation, grads: dict, device) -> dict:
@@ -98,59 +100,75 @@ def _zero_outside(grads: dict, mask: np.ndarray, device) -> dict:
 
 
 def reconstruction_nodes(sim: Simulation, indicator=None, span: int | None = None):
-    """Nodes a reverse march must replay, and where its gradient stays exact.
+    """The nodes a reverse march replays from a record, and those it rebuilds exactly.
 
-    A PML node is irreversible; the strip is every irreversible interior node
-    within the reach of a reversible one, diagonals included
-    (cuwave/sensitivity.py:160-195). Returns the (ndim, num) strip indices in row-major
-    order and the mask of reversible interior nodes.
+    Run backwards, the lossless step of a node is exact; a layer node's memory
+    terms decay forwards and cannot be run back. A reversible node's reverse step
+    reads every node within `sim.reach` of it along each axis and across the
+    diagonals (a box of side 2 reach + 1), so the record must hold each layer node
+    inside such a box around some reversible node (cuwave/sensitivity.py:160-195).
+
+    Returns the (ndim, num) record nodes in row-major order and the mask of the
+    reversible interior nodes, where the rebuilt field is exact.
     """
-    interior = np.zeros(sim.Nx_padded, dtype=bool)
-    interior[tuple(slice(1, n - 1) for n in sim.Nx)] = True
+    inside = np.zeros(sim.Nx_padded, dtype=bool)
+    inside[tuple(slice(1, n - 1) for n in sim.Nx)] = True
     if sim.pml is None:
-        return np.zeros((sim.ndim, 0), dtype=np.int32), interior
-    reversible = interior & ~pml_nodes(sim)
-    square = ndi.generate_binary_structure(sim.ndim, sim.ndim)
-    reach = ndi.binary_dilation(reversible, structure=square, iterations=sim.reach,
-                                brute_force=True)
-    strip = np.stack(np.nonzero(interior & ~reversible & reach)).astype(np.int32)
-    return strip, reversible
+        return np.zeros((sim.ndim, 0), dtype=np.int32), inside
+    exact = inside & ~pml_nodes(sim)
+    # a box maximum is the dilation by the full square, taken `reach` times over
+    seen = ndi.maximum_filter(exact.astype(np.uint8), size=2 * sim.reach + 1,
+                              mode="constant", cval=0).astype(bool)
+    record = np.argwhere(inside & ~exact & seen).T.astype(np.int32)
+    return record, exact
+
+
+def _reversible_segments(snapshots: int, replays: int) -> int:
+    """Segments a binomial treeverse reverses with `snapshots` spare snapshots when no
+    segment is marched more than `replays` extra times: C(snapshots + replays, snapshots)
+    (Griewank & Walther 2000, https://doi.org/10.1145/347837.347846)."""
+    return math.comb(snapshots + replays, snapshots)
 
 
 def segment_split(lo: int, hi: int, free: int) -> int:
-    """The binomial treeverse split of segments lo .. hi with `free` snapshot slots.
+    """The segment at which treeverse over segments lo .. hi - 1 places its next snapshot.
 
-    `free` slots and r replays reverse at most C(free + r, free) segments
-    (Griewank & Walther 2000, https://doi.org/10.1145/347837.347846).
+    The replay count is the smallest that `free` snapshots allow for the span; the
+    segments after the snapshot get exactly what one snapshot fewer reverses at that
+    count, and the ones before it the remainder (never fewer than one).
     """
-    r = 0
-    while math.comb(free + r, free) < hi - lo:
-        r += 1
-    return max(lo + 1, hi - math.comb(free - 1 + r, free - 1))
+    replays = next(r for r in itertools.count() if _reversible_segments(free, r) >= hi - lo)
+    return max(lo + 1, hi - _reversible_segments(free - 1, replays))
 
 
 def checkpoint_schedule(segments: int, slots: int) -> list[tuple]:
-    """Treeverse over `segments` with `slots` snapshots (cuwave/sensitivity.py:243-269).
+    """The reversal order of `segments` with `slots` snapshots besides the zero start.
 
-    Actions in order: ("restore", k), ("store", k), ("drop", k), ("advance", a, b)
-    and ("leaf", k), which replays segment k and reverses it.
+    Built with an explicit stack of pending ranges (lo, hi, free snapshots, the
+    snapshot to release when the range is done). Actions: ("restore", k) loads the
+    start state of segment k, ("store", k) snapshots it, ("drop", k) releases it,
+    ("advance", a, b) marches from segment a's start to segment b's, ("leaf", k)
+    marches segment k recording its strip and then reverses it. The schedule is the
+    reference's (cuwave/sensitivity.py:243-269): the `info` sweep count depends on it.
     """
-    actions = []
-
-    def reverse(lo, hi, free):
-        while hi - lo > 1:
-            if free == 0:
-                for k in range(hi - 1, lo, -1):
-                    actions.extend((("restore", lo), ("advance", lo, k), ("leaf", k)))
-                break
+    actions: list[tuple] = []
+    pending = [[0, segments, slots, None]]
+    while pending:
+        top = pending[-1]
+        lo, hi, free, _ = top
+        if hi - lo > 1 and free > 0:
             mid = segment_split(lo, hi, free)
-            actions.extend((("restore", lo), ("advance", lo, mid), ("store", mid)))
-            reverse(mid, hi, free - 1)
-            actions.append(("drop", mid))
-            hi = mid
-        actions.extend((("restore", lo), ("leaf", lo)))
-
-    reverse(0, segments, slots)
+            actions += [("restore", lo), ("advance", lo, mid), ("store", mid)]
+            top[1] = mid  # what is left of this range once the part past mid is reversed
+            pending.append([mid, hi, free - 1, mid])
+            continue
+        # no snapshot to spare: every segment past lo replays from lo, the last first
+        for k in range(hi - 1, lo, -1):
+            actions += [("restore", lo), ("advance", lo, k), ("leaf", k)]
+        actions += [("restore", lo), ("leaf", lo)]
+        pending.pop()
+        if top[3] is not None:
+            actions.append(("drop", top[3]))
     return actions
 
 

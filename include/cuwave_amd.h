@@ -22,10 +22,14 @@
  * - `dtype` 0 is float32, 1 is float64; every field of a call has it.
  * - Launches are asynchronous on `stream` (a hipStream_t; 0 the null stream).
  *   A 2D adjoint sweep also launches its frame kernels on a non-blocking
- *   stream of the library's own (one a device and host thread, kept for the
- *   process's life), ordered after `stream`'s earlier work; `stream` waits
- *   for them before its next work, so to the caller the call stays one
- *   stream's.
+ *   stream of the library's own, ordered after `stream`'s earlier work;
+ *   `stream` waits for them before its next work, so to the caller the call
+ *   stays one stream's. Those streams and their events come from a pool per
+ *   device that lives for the process: a call leases one set while it
+ *   enqueues and returns it, so the pool holds as many sets as calls ever
+ *   enqueued at once. A call made while `stream` is being captured into a
+ *   graph should find a set in the pool (make one uncaptured call first) or
+ *   switch side streams off with cw_set_side_stream(0).
  * - A return of 0 is success; anything else is an error whose text
  *   `cw_last_error()` returns (thread-local). Arguments are checked before
  *   any launch, so an error leaves the device untouched.
@@ -41,7 +45,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 4
+#define CW_ABI_VERSION 5
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -182,6 +186,14 @@ typedef struct cw_superposed_args {
 
 int cw_version(void);
 const char *cw_last_error(void);
+
+/* 0: every launch of a sweep goes on the caller's stream (the frame launches
+ * of a 2D adjoint level then follow its core launch instead of running beside
+ * it; the results are the same bits); 1 (the default): side streams as
+ * above. Process-wide; returns the previous setting. No reference
+ * counterpart: the reference's graphs and side stream are its own host
+ * code's (cuwave/wave.py:929-936, cuwave/sensitivity.py:670-689). */
+int cw_set_side_stream(int32_t enabled);
 
 /* the tiles a step kernel walks, for bucketing points */
 int cw_tile_count(const cw_grid *grid, int32_t *tiles);

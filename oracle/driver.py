@@ -208,18 +208,26 @@ class Sim:
             g.op[i] = table.flat[i]
         return g
 
-    # hooks (cuwave/scalar.py:875-950)
+    # hooks (cuwave/scalar.py:875-950); "BulkDensityWave" has no reference class:
+    # the reference's PressureWave hooks (cuwave/scalar.py:140-160, 739-745) fed a
+    # (kappa, rho) indicator stack, stiff = 1 / rho and minv = kappa
     def parametrization(self, indicator):
         if self.kind == "ScalarWave":
             return indicator, None
+        if self.kind == "BulkDensityWave":
+            kappa, rho = indicator[0], indicator[1]
+            return 1 / rho, kappa
         rho1, rho2, kappa1, kappa2 = self.materials
         rho_inv = 1 / rho1 + indicator * (1 / rho2 - 1 / rho1)
         kappa_inv = 1 / kappa1 + indicator * (1 / kappa2 - 1 / kappa1)
         return rho_inv, 1 / kappa_inv
 
-    def parametrization_jacobian(self):
+    def parametrization_jacobian(self, indicator=None):
         if self.kind == "ScalarWave":
             return 1.0, 1.0
+        if self.kind == "BulkDensityWave":
+            kappa, rho = np.asarray(indicator, dtype=self.dtype)
+            return -1 / kappa**2, -1 / rho**2
         rho1, rho2, kappa1, kappa2 = self.materials
         return 1 / kappa2 - 1 / kappa1, 1 / rho2 - 1 / rho1
 

@@ -22,16 +22,17 @@ from __future__ import annotations
 import math
 import os
 import sys
+import tempfile
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
 REFERENCE = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
-CACHE = HERE.parents[1] / "_work" / "refcache"
+CACHE = tempfile.mkdtemp(prefix="cuwave-golden-")  # the reference's CPU build cache
 
 os.environ["CUWAVE_DEVICE"] = "cpu"
-os.environ["XDG_CACHE_HOME"] = str(CACHE)
+os.environ["XDG_CACHE_HOME"] = CACHE
 sys.path.insert(0, str(REFERENCE))
 
 from cuwave import stencils  # noqa: E402

@@ -185,10 +185,12 @@ def test_at_full_size_the_batched_gradient_sums_the_shots_and_matches_the_oracle
 
 
 def test_on_a_caller_stream_the_frame_launches_join_it():
-    # a 2D sweep runs its frame kernels on a stream of the library's own: it
-    # must start after the caller stream's earlier work (the inputs it fills)
-    # and end before that stream's next work (here a copy of the outputs,
-    # enqueued with no synchronisation between)
+    # a 2D adjoint sweep runs its frame kernels on a stream of the library's
+    # own: it must start after the caller stream's earlier work (the inputs it
+    # fills) and end before that stream's next work (here a copy of the
+    # outputs, enqueued with no synchronisation between). The forward half
+    # runs on the caller's stream alone; it checks that a caller's stream
+    # other than the null stream carries the fused steps unchanged.
     import sys
 
     import torch
@@ -222,3 +224,33 @@ def test_on_a_caller_stream_the_frame_launches_join_it():
     side.synchronize()
     assert cost2 == cost
     np.testing.assert_array_equal(host(g2c), host(g))
+
+
+def test_side_by_side_adjoint_levels_equal_the_serialised_ones():
+    # the core and frame launches of an adjoint level run on two streams; with
+    # side streams off both run on the caller's stream, one after the other.
+    # A race between them (a frame kernel reading a level the core has not
+    # finished) would make the two gradients differ.
+    import sys
+
+    from helpers import ROOT
+
+    from cuwave_amd import _abi
+
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    sim, true, start, sources, nodes = bench.workload(32, 120)
+    print("case side streams on/off 1090x1090 32 shots 120 steps", flush=True)
+    sensors = utils.Sensors(sim, nodes.T * np.array(sim.dx))
+    observed = utils.measure(sim, sources, true, sensors)
+    lib = _abi.load()
+    cost, g = utils.misfit_gradient(sim, sources, start, sensors, observed)
+    previous = lib.cw_set_side_stream(0)
+    try:
+        cost_s, g_s = utils.misfit_gradient(sim, sources, start, sensors, observed)
+    finally:
+        lib.cw_set_side_stream(previous)
+    assert previous == 1
+    assert cost_s == cost
+    np.testing.assert_array_equal(host(g_s), host(g))

@@ -215,3 +215,60 @@ def test_a_3d_order_8_float32_adjoint_matches_the_oracle():
     assert abs(cost - ocost) <= F32_FIELD * abs(ocost)
     assert rel_max(g["mass"], og["mass"]) < F32_GRAD
     assert rel_max(g["stiff"], og["stiff"]) < F32_GRAD
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_3d_order_8_bulk_density_reconstruction_with_checkpoints_matches_the_oracle(precision):
+    # the shape of the 3D acoustic benchmark at a size the oracle finishes in
+    # seconds: independent bulk modulus and density (smoothed noise plus a low
+    # density void), order 8, a CPML on every face, the reverse march with its
+    # strip replay and two treeverse checkpoints
+    from scipy.ndimage import gaussian_filter
+
+    from cuwave_amd import boundary, scalar
+    from oracle import driver
+
+    n, width, N, dt = 40, 7, 120, 2.0e-7
+    dx = 1e-3
+    rng = np.random.default_rng(11)
+    npad = (n, n, 64)
+
+    def field(lo, hi):
+        f = gaussian_filter(rng.standard_normal(npad), 3)
+        return lo + (hi - lo) * (f - f.min()) / (f.max() - f.min())
+
+    rho, vp = field(900.0, 1300.0), field(1400.0, 1800.0)
+    kappa = rho * vp**2
+    c = (np.arange(n) - n / 2) ** 2
+    void = (c[:, None, None] + c[None, :, None] + c[None, None, :] <= 16)
+    void = np.pad(void, ((0, 0), (0, 0), (0, npad[2] - n)))
+    rho[void], kappa[void] = 1.204, 1.419e5
+    dtype = np.float32 if precision == "float32" else np.float64
+    stack = np.stack([kappa, rho]).astype(dtype)
+    pml = boundary.Pml(width, 1800.0, 1e-6)
+    sim = scalar.BulkDensityWave((n, n, n), (dx,) * 3, N, dt, precision=precision,
+                                 space_order=8, pml=pml)
+    osim = driver.Sim("BulkDensityWave", (n, n, n), (dx,) * 3, N, dt, precision, 8,
+                      pml=driver.Pml(width, 1800.0, 1e-6))
+    t = np.arange(N) * dt
+    from cuwave_amd.signals import ricker
+
+    sig = (ricker(t, 1.0, 1e5) / dx**3)[:, None].astype(dtype)
+    src_pos = np.array([[width + 2], [n // 2], [n // 2]])
+    g = np.arange(width + 2, n - width - 2, 4)
+    sensors = np.stack([np.full(len(g) ** 2, width + 2), np.repeat(g, len(g)), np.tile(g, len(g))])
+    print(f"case 3d order 8 bulk-density reconstruction checkpoints=2 {precision}", flush=True)
+    _, observed = driver.simulate(osim, src_pos, sig, stack, sensors)
+    observed = (0.9 * observed).astype(dtype)
+    cost, grads, traces, info = S.reconstruction_sensitivity(
+        sim, wave.Source(src_pos, sig), stack, sensors, S.l2_misfit(observed), checkpoints=2)
+    ocost, ograds, otraces, oinfo = driver.reconstruction_sensitivity(
+        osim, src_pos, sig, stack, sensors, driver.l2_misfit(observed), checkpoints=2)
+    assert info["segment"] < N and info["sweeps"] > 1.0  # the checkpointed schedule ran
+    assert info["segment"] == oinfo["segment"] and info["sweeps"] == oinfo["sweeps"]
+    assert info["strip"] == oinfo["strip"]
+    bar = F64 if precision == "float64" else F32_GRAD
+    assert rel_l2(traces, otraces) < (F64 if precision == "float64" else F32_FIELD)
+    assert abs(cost - ocost) <= bar * abs(ocost)
+    for key in ("mass", "stiff"):
+        assert rel_max(grads[key], ograds[key]) < bar, key

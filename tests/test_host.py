@@ -77,6 +77,13 @@ def test_checkpoint_schedule_matches_the_oracle(segments, slots):
     assert S.checkpoint_schedule(segments, slots) == D.checkpoint_schedule(segments, slots)
 
 
+def test_checkpoint_schedule_matches_the_oracle_over_a_range():
+    # the product's stack formulation against the oracle's recursive restatement
+    for segments in range(1, 41):
+        for slots in range(0, 6):
+            assert S.checkpoint_schedule(segments, slots) == D.checkpoint_schedule(segments, slots)
+
+
 def test_every_segment_is_reversed_once():
     for segments, slots in ((9, 2), (20, 3)):
         leaves = [a[1] for a in S.checkpoint_schedule(segments, slots) if a[0] == "leaf"]

@@ -249,3 +249,85 @@ def test_the_illumination_after_one_step_is_the_source_injection_squared():
     # two shots add
     two = D.illumination(sim, [(z["src_pos"], z["src_sig"])] * 2, z["indicator"])
     np.testing.assert_array_equal(two, 2 * expect)
+
+
+# ------------------------------------------------- independent bulk modulus and density
+def _bulk_sim(kind, ndim=2, N=70, order=4, precision="float64", **kw):
+    shape = (40, 36) if ndim == 2 else (22, 21, 20)
+    return D.Sim(kind, shape, (1.0,) * ndim, N, 0.25, precision, order,
+                 pml=D.Pml(6, 1.5, 1e-6), **kw)
+
+
+def _bulk_points(sim):
+    mid = [n // 2 for n in sim.Nx]
+    src = np.array([[m] for m in mid])
+    src[0, 0] = 9
+    sensors = np.array([[10, 11, 12], [10, 17, 24]] if sim.ndim == 2
+                       else [[10, 11, 12], [9, 10, 11], [8, 10, 12]])
+    t = np.arange(sim.N) * sim.dt
+    sig = np.exp(-((t - 3.0) ** 2))[:, None]
+    return src, sig, sensors
+
+
+def test_bulk_density_reduces_to_the_acoustic_two_phase_medium():
+    # kappa and rho following one binary indicator: kappa in {1, 4}, rho in {1, 2},
+    # every reciprocal exact, so both parametrizations hand the kernels the same
+    # material; the traces then agree bit for bit and the chained gradients agree
+    # through d kappa / d gamma = 0.75 kappa^2 and d rho / d gamma = 0.5 rho^2
+    acoustic = _bulk_sim("AcousticWave", materials=(1.0, 2.0, 1.0, 4.0))
+    bulk = _bulk_sim("BulkDensityWave")
+    gamma = np.zeros(acoustic.Nx_padded)
+    a0, a1 = np.meshgrid(*(np.arange(n) for n in acoustic.Nx_padded), indexing="ij")
+    gamma[(a0 - 20) ** 2 + (a1 - 18) ** 2 <= 36] = 1.0
+    stack = np.stack([np.where(gamma > 0, 4.0, 1.0), np.where(gamma > 0, 2.0, 1.0)])
+    src, sig, sensors = _bulk_points(acoustic)
+    _, tr_a = D.simulate(acoustic, src, sig, gamma, sensors)
+    _, tr_b = D.simulate(bulk, src, sig, stack, sensors)
+    np.testing.assert_array_equal(tr_a, tr_b)
+    objective = D.l2_misfit(0.7 * tr_a)
+    _, ga, _, _ = D.reconstruction_sensitivity(acoustic, src, sig, gamma, sensors, objective)
+    _, gb, _, _ = D.reconstruction_sensitivity(bulk, src, sig, stack, sensors, objective)
+    jm, js = acoustic.parametrization_jacobian()
+    dgamma = jm * ga["mass"] + js * ga["stiff"]
+    km, ks = bulk.parametrization_jacobian(stack)
+    kappa, rho = stack
+    chained = km * gb["mass"] * (0.75 * kappa**2) + ks * gb["stiff"] * (0.5 * rho**2)
+    scale = np.abs(dgamma).max()
+    assert scale > 0
+    assert np.abs(chained - dgamma).max() <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_bulk_density_gradients_match_finite_differences(ndim):
+    from scipy.ndimage import gaussian_filter
+
+    sim = _bulk_sim("BulkDensityWave", ndim=ndim, N=60 if ndim == 2 else 40)
+    rng = np.random.default_rng(7)
+    fields = [1.0 + 0.3 * np.tanh(gaussian_filter(rng.standard_normal(sim.Nx_padded), 3) * 8)
+              for _ in range(2)]
+    stack = np.stack(fields)
+    src, sig, sensors = _bulk_points(sim)
+    _, tr = D.simulate(sim, src, sig, stack, sensors)
+    objective = D.l2_misfit(0.8 * tr)
+    _, grads, _, _ = D.reconstruction_sensitivity(sim, src, sig, stack, sensors, objective,
+                                                  checkpoints=2)
+    # the stored-history adjoint agrees (round-off of the reverse march only)
+    _, ref, _, _ = D.sensitivity(sim, src, sig, stack, sensors, objective)
+    for key in ("mass", "stiff"):
+        assert np.abs(grads[key] - ref[key]).max() <= 1e-10 * np.abs(ref[key]).max()
+    km, ks = sim.parametrization_jacobian(stack)
+    _, valid = D.reconstruction_nodes(sim)
+    for which, g in ((0, km * grads["mass"]), (1, ks * grads["stiff"])):
+        # a smooth direction over the reversible interior (the gradient is zeroed
+        # in the layer): the directional derivative against central differences
+        direction = gaussian_filter(rng.standard_normal(sim.Nx_padded), 2) * valid
+        h = 1e-5
+
+        def cost(step):
+            trial = stack.copy()
+            trial[which] += step * direction
+            return objective(D.simulate(sim, src, sig, trial, sensors)[1])[0]
+
+        fd = (cost(h) - cost(-h)) / (2 * h)
+        adj = float(np.sum(g * direction))
+        assert abs(adj - fd) <= 1e-6 * abs(fd), (which, adj, fd)
