@@ -172,6 +172,7 @@ class Sim:
     density: float = None
     materials: tuple = None  # rho1, rho2, kappa1, kappa2
     damping: np.ndarray | None = None
+    pml_margin: int = 0  # undamped layer nodes before the front (elastic: radius - 1)
     ndim: int = field(init=False)
 
     def __post_init__(self):
@@ -298,7 +299,7 @@ class Sim:
 
     def pml_profiles(self):
         layer = self.pml
-        L = layer.width - 1
+        L = layer.width - 1 - self.pml_margin  # cuwave/boundary.py:335-336
         out = []
         for d, (lo, hi) in enumerate(self.pml_widths()):
             if not lo + hi:
