@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 5
+ABI_VERSION = 6
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -142,6 +142,9 @@ class SuperposedArgs(ctypes.Structure):
     ]
 
 
+# the bytes of the library's own scratch on each GPU (CW_STATIC_SCRATCH_BYTES)
+STATIC_SCRATCH_BYTES = 64 << 20
+
 # every symbol include/cuwave_amd.h declares, with its argument types
 SIGNATURES = {
     "cw_version": ([], I32),
@@ -152,6 +155,7 @@ SIGNATURES = {
     "cw_scalar_prepare": ([ctypes.POINTER(Grid), ctypes.POINTER(Materials), VP], I32),
     "cw_scalar_forward": ([ctypes.POINTER(ForwardArgs), I32, I32, VP], I32),
     "cw_scalar_backward": ([ctypes.POINTER(BackwardArgs), I32, I32, VP], I32),
+    "cw_scalar_forward_static": ([ctypes.POINTER(ForwardArgs), I32, I32, VP], I32),
     "cw_scalar_superposed": ([ctypes.POINTER(SuperposedArgs), VP], I32),
 }
 

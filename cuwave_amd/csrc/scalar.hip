@@ -216,6 +216,43 @@ template <typename T, int NDIM, int R> struct Superposed {
 
 int check_forward(const cw_forward_args &a, int32_t t0, int32_t t1);
 
+// ------------------------------------------------------- static scratch
+// The workspace of cw_scalar_forward_static: one array on every GPU the
+// library runs on, held until the process ends. Host code never names it
+// past hipGetSymbolAddress: `scratch` on the host is a stand-in, not a device
+// address, so every launch and fill takes the address that call returned for
+// the current device.
+static __device__ __align__(256) unsigned char scratch[CW_STATIC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mutex;
+unsigned char *g_scratch_of[64] = {};
+
+// the scratch of the current device, looked up once per device
+int static_scratch(unsigned char **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+    return fail("static scratch: no current device");
+  std::lock_guard<std::mutex> hold(g_scratch_mutex);
+  if (!g_scratch_of[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(std::string("static scratch: hipGetSymbolAddress: ") +
+                  hipGetErrorString(e));
+    g_scratch_of[dev] = (unsigned char *)p;
+  }
+  *out = g_scratch_of[dev];
+  return 0;
+}
+
+// zero `words` 16-byte words (a fill kernel, so a caller may capture it)
+__global__ void clear_words(uint4 *p, int64_t words) {
+  const uint4 z = {0u, 0u, 0u, 0u};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z;
+}
+
 } // namespace
 
 extern "C" {
@@ -379,6 +416,57 @@ int cw_scalar_backward(const cw_backward_args *args, int32_t n0, int32_t n1,
   if (n0 == n1)
     return 0;
   return dispatch<Backward>(args->grid, *args, n0, n1, (hipStream_t)stream);
+}
+
+int cw_scalar_forward_static(const cw_forward_args *args, int32_t t0,
+                             int32_t t1, void *stream) {
+  if (!args)
+    return fail("args is NULL");
+  if (int e = check_grid(&args->grid))
+    return e;
+  cw_forward_args a = *args;
+  const int64_t elem = a.grid.dtype ? 8 : 4;
+  int64_t nodes = 1;
+  for (int d = 0; d < a.grid.ndim; ++d)
+    nodes *= a.grid.np[d];
+  // the layout of the memories in the scratch: each lined axis' (3, slab,
+  // lanes) block, rounded up to 256 bytes
+  int64_t offset[3] = {0, 0, 0}, used = 0;
+  for (int d = 0; d < a.grid.ndim; ++d) {
+    if (a.pml.lo[d] < 0 || a.pml.hi[d] < 0)
+      return fail("pml widths must be non-negative");
+    if (!(a.pml.lo[d] + a.pml.hi[d]))
+      continue;
+    const int64_t slab = (nodes / a.grid.np[d]) * (a.pml.lo[d] + a.pml.hi[d]);
+    offset[d] = used;
+    used += (3 * slab * a.grid.lanes * elem + 255) / 256 * 256;
+  }
+  if (used > (int64_t)CW_STATIC_SCRATCH_BYTES)
+    return fail("the layer memories need " + std::to_string(used) +
+                " bytes, past the static scratch of " +
+                std::to_string((int64_t)CW_STATIC_SCRATCH_BYTES) + " bytes");
+  unsigned char *base = nullptr;
+  if (used) {
+    if (int e = static_scratch(&base))
+      return e;
+  }
+  for (int d = 0; d < a.grid.ndim; ++d)
+    a.pml.aux[d] = (a.pml.lo[d] + a.pml.hi[d]) ? base + offset[d] : nullptr;
+  if (!a.mat.prep)
+    return fail("forward needs the prepared materials");
+  if (int e = check_forward(a, t0, t1))
+    return e;
+  if (t0 == t1)
+    return 0;
+  if (t0 == 0 && used) {
+    const int64_t words = used / 16;
+    const int blocks = (int)std::min<int64_t>((words + cw::kThreads - 1) / cw::kThreads, 4096);
+    hipLaunchKernelGGL(clear_words, dim3(blocks), dim3(cw::kThreads), 0,
+                       (hipStream_t)stream, (uint4 *)base, words);
+    if (int e = hip_status("clear_words"))
+      return e;
+  }
+  return dispatch<Forward>(a.grid, a, t0, t1, (hipStream_t)stream);
 }
 
 int cw_scalar_superposed(const cw_superposed_args *args, void *stream) {

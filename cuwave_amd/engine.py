@@ -200,11 +200,19 @@ class Propagator:
 
         return bracket()
 
-    def forward(self, u: torch.Tensor, t0: int, t1: int, *, offset: int = 0, aux=None,
-                sources: PointList | None = None, signal: torch.Tensor | None = None,
-                sensors: PointList | None = None, record: torch.Tensor | None = None,
-                strip: dict | None = None, illum: torch.Tensor | None = None) -> None:
-        """Steps t0 .. t1 - 1 into the levels of `u` (cw_scalar_forward); `illum`, a
+    def forward(self, u: torch.Tensor, t0: int, t1: int, **kw) -> None:
+        """Steps t0 .. t1 - 1 into the levels of `u` (cw_scalar_forward); keywords as
+        `forward_args` takes them."""
+        a = self.forward_args(u, **kw)
+        with self._timed("forward", t1 - t0):
+            _abi.call("cw_scalar_forward", ctypes.byref(a), int(t0), int(t1), _abi.stream())
+
+    def forward_args(self, u: torch.Tensor, *, offset: int = 0, aux=None,
+                     sources: PointList | None = None, signal: torch.Tensor | None = None,
+                     sensors: PointList | None = None, record: torch.Tensor | None = None,
+                     strip: dict | None = None,
+                     illum: torch.Tensor | None = None) -> _abi.ForwardArgs:
+        """The argument block of a forward sweep over the levels of `u`; `illum`, a
         per-node accumulator, gathers the illumination sum over steps and shots of u^2."""
         a = _abi.ForwardArgs()
         a.grid = self.grid
@@ -225,8 +233,7 @@ class Propagator:
             self._require_shape("illumination", illum, tuple(self.sim.Nx_padded))
             a.illum = _abi.ptr(illum)
         a.steps = self.sim.N
-        with self._timed("forward", t1 - t0):
-            _abi.call("cw_scalar_forward", ctypes.byref(a), int(t0), int(t1), _abi.stream())
+        return a
 
     def backward(self, lam: torch.Tensor, u: torch.Tensor, n0: int, n1: int, *, eta,
                  g_mass: torch.Tensor, g_stiff: torch.Tensor | None = None, u_offset: int = 0,

@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 5
+#define CW_ABI_VERSION 6
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -213,6 +213,23 @@ int cw_scalar_forward(const cw_forward_args *args, int32_t t0, int32_t t1,
 
 int cw_scalar_backward(const cw_backward_args *args, int32_t n0, int32_t n1,
                        void *stream);
+
+/* No counterpart in the reference: cw_scalar_forward for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The CPML memories, the only workspace of
+ * a forward sweep, come from a scratch array the library holds on every GPU
+ * it runs on, CW_STATIC_SCRATCH_BYTES of it, until the process ends;
+ * `args->pml.aux` is ignored. A call with t0 == 0 clears the memories first;
+ * a call with t0 > 0 continues from what the previous call on that GPU left
+ * in them. Calls on one GPU share the array, so calls that overlap on one GPU
+ * (two streams, two host threads, or one sweep split into calls with another
+ * simulation's calls between them) race. A sweep whose memories need more
+ * than CW_STATIC_SCRATCH_BYTES (the sum over lined axes of 3 x slab nodes x
+ * lanes x element size, each axis rounded up to 256 bytes) is refused before
+ * any launch. Nothing here allocates: no hipMalloc, no library allocation. */
+#define CW_STATIC_SCRATCH_BYTES (64u << 20)
+int cw_scalar_forward_static(const cw_forward_args *args, int32_t t0,
+                             int32_t t1, void *stream);
 
 int cw_scalar_superposed(const cw_superposed_args *args, void *stream);
 
