@@ -142,6 +142,93 @@ class SuperposedArgs(ctypes.Structure):
     ]
 
 
+class ElasticGrid(ctypes.Structure):
+    _fields_ = [
+        ("ndim", I32),
+        ("order", I32),
+        ("n", I32 * 3),
+        ("np", I32 * 3),
+        ("nbatch", I32),
+        ("lanes", I32),
+        ("dtype", I32),
+        ("clamped", I32),
+        ("closure", I32 * 4),
+        ("stag", F64 * 4),
+        ("lame", F64),
+        ("shear", F64),
+        ("density", F64),
+        ("inv_dx", F64 * 3),
+        ("step", F64 * 3),
+    ]
+
+
+class ElasticMaterial(ctypes.Structure):
+    _fields_ = [("gamma", VP), ("prep", VP), ("rows", VP)]
+
+
+class ElasticLayer(ctypes.Structure):
+    _fields_ = [
+        ("lo", I32 * 3),
+        ("hi", I32 * 3),
+        ("profile", VP * 3),
+        ("memory", VP * 3),
+        ("phase", I32),
+    ]
+
+
+class ElasticStepArgs(ctypes.Structure):
+    _fields_ = [
+        ("grid", ElasticGrid),
+        ("mat", ElasticMaterial),
+        ("pml", ElasticLayer),
+        ("sigma", VP),
+        ("u0", VP),
+        ("u1", VP),
+        ("u2", VP),
+        ("g_mass", VP),
+        ("g_normal", VP),
+        ("g_shear", VP),
+        ("v1", VP),
+        ("mass_factor", F64),
+    ]
+
+
+class ElasticGradientArgs(ctypes.Structure):
+    _fields_ = [
+        ("grid", ElasticGrid),
+        ("rows", VP),
+        ("g_mass", VP),
+        ("g_normal", VP),
+        ("g_shear", VP),
+        ("u0", VP),
+        ("u1", VP),
+        ("u2", VP),
+        ("l1", VP),
+        ("mass_factor", F64),
+    ]
+
+
+class ElasticPointArgs(ctypes.Structure):
+    _fields_ = [
+        ("starts", VP),
+        ("lin", VP),
+        ("shot", VP),
+        ("col", VP),
+        ("runs", I32),
+        ("kind", I32),
+        ("row_stride", I64),
+        ("col_stride", I64),
+        ("shot_stride", I64),
+        ("row", I64),
+        ("field", VP),
+        ("table", VP),
+        ("weight", VP),
+        ("g_mass", VP),
+        ("u1", VP),
+        ("mass_factor", F64),
+    ]
+
+
 # the bytes of the library's own scratch on each GPU (CW_STATIC_SCRATCH_BYTES)
 STATIC_SCRATCH_BYTES = 64 << 20
 
@@ -157,6 +244,10 @@ SIGNATURES = {
     "cw_scalar_backward": ([ctypes.POINTER(BackwardArgs), I32, I32, VP], I32),
     "cw_scalar_forward_static": ([ctypes.POINTER(ForwardArgs), I32, I32, VP], I32),
     "cw_scalar_superposed": ([ctypes.POINTER(SuperposedArgs), VP], I32),
+    "cw_elastic_prepare": ([ctypes.POINTER(ElasticGrid), ctypes.POINTER(ElasticMaterial), VP], I32),
+    "cw_elastic_step": ([ctypes.POINTER(ElasticStepArgs), VP], I32),
+    "cw_elastic_gradient": ([ctypes.POINTER(ElasticGradientArgs), VP], I32),
+    "cw_elastic_points": ([ctypes.POINTER(ElasticGrid), ctypes.POINTER(ElasticPointArgs), VP], I32),
 }
 
 _LIB: ctypes.CDLL | None = None

@@ -34,10 +34,17 @@ class BoundaryCondition:
 
 Neumann = BoundaryCondition("neumann")
 Dirichlet = BoundaryCondition("dirichlet")
+# the faces of the elastic family (cuwave/boundary.py:30-78): a traction-free
+# face zeroes the normal stress on its wall node, a clamped one holds the
+# displacement there at zero
+Traction = BoundaryCondition("traction")
+Clamped = BoundaryCondition("clamped")
 
 
-def canonical_boundary(boundary, ndim: int, default: BoundaryCondition = Neumann) -> tuple:
-    """((low, high),) * ndim from a condition, a per-axis one, or None for `default`."""
+def canonical_boundary(boundary, ndim: int, default: BoundaryCondition = Neumann,
+                       allowed: tuple = (Neumann, Dirichlet)) -> tuple:
+    """((low, high),) * ndim from a condition, a per-axis one, or None for `default`;
+    every condition one of `allowed`."""
     if boundary is None:
         boundary = default
     if isinstance(boundary, BoundaryCondition):
@@ -51,8 +58,9 @@ def canonical_boundary(boundary, ndim: int, default: BoundaryCondition = Neumann
         if len(pair) != 2:
             raise ValueError("a boundary axis is a (low, high) pair of conditions")
         for c in pair:
-            if c not in (Neumann, Dirichlet):
-                raise ValueError(f"a pressure face is Neumann or Dirichlet, not {c!r}")
+            if c not in allowed:
+                names = " or ".join(map(repr, allowed))
+                raise ValueError(f"a face of this equation is {names}, not {c!r}")
         pairs.append(tuple(pair))
     return tuple(pairs)
 
@@ -106,9 +114,22 @@ def _canonical_faces(ndim: int, faces: Sequence[int] | None) -> tuple[int, ...]:
 def validate_pml(sim: Simulation) -> None:
     """Raise if `sim.pml` cannot be stepped: too thin, too wide, or ill-posed."""
     layer = sim.pml
-    if layer.width < 2 or layer.wavespeed <= 0.0 or not 0.0 < layer.reflection < 1:
-        raise ValueError(f"a pml needs width >= 2, a speed and 0 < reflection < 1: {layer}")
+    least = 2 + sim.pml_margin
+    if layer.width < least or layer.wavespeed <= 0.0 or not 0.0 < layer.reflection < 1:
+        raise ValueError(
+            f"a pml needs width >= {least}, a speed and 0 < reflection < 1: {layer}")
+    if sim.pml_shifted and layer.frequency <= 0.0:
+        raise ValueError(
+            f"{type(sim).__name__} keeps a static mode in an unshifted pml: set frequency, "
+            f"near the low end of the band")
     faces = _canonical_faces(sim.ndim, layer.faces)
+    if sim.pml_walls is not None:
+        for face in faces:
+            wall = sim.boundary[face // 2][face % 2]
+            if wall not in sim.pml_walls:
+                raise ValueError(
+                    f"face {face} ends the pml on {wall!r}, which traps a growing wave: make it "
+                    f"{' or '.join(map(repr, sim.pml_walls))}")
     width = layer.width + sim.reach  # the transposed taps reach past the layer
     for d in range(sim.ndim):
         sides = sum(2 * d + side in faces for side in (0, 1))
@@ -129,12 +150,13 @@ def pml_profiles(sim: Simulation) -> list[np.ndarray | None]:
     """Per lined axis the (4, Nx[d]) recursion coefficients a, b at nodes and half points.
 
     In float64 then rounded to the simulation's precision; the depth below the
-    front is (L - distance) / L from the wall node, L = width - 1, so the front
-    node is undamped; d0 = -3 c ln(R) / (2 L dx) depth^2, b = exp(-(d0 + alpha) dt),
+    front is (L - distance) / L from the wall node, L = width - 1 - margin, so
+    the front node and the `sim.pml_margin` nodes before it are undamped;
+    d0 = -3 c ln(R) / (2 L dx) depth^2, b = exp(-(d0 + alpha) dt),
     a = d0 / (d0 + alpha) (b - 1).
     """
     layer = sim.pml
-    L = layer.width - 1
+    L = layer.width - 1 - sim.pml_margin
     out = []
     for d, (lo, hi) in enumerate(pml_widths(sim)):
         if not lo + hi:

@@ -1,8 +1,10 @@
 """Build `libcuwave_amd.so` from the HIP sources for gfx950: `python -m cuwave_amd.build`.
 
-Every translation unit (the C ABI in `csrc/scalar.hip`, the step kernels of
-one precision and radius in each `csrc/steps_*.hip`) compiles in its own
-process, all at once, and the objects link into one shared library.
+Every translation unit (the C ABI of the pressure family in `csrc/scalar.hip`
+and of the elastic family in `csrc/elastic.hip`, the step kernels of one
+precision and radius in each `csrc/steps_*.hip` and `csrc/elastic_*.hip`)
+compiles in its own process, all at once, and the objects link into one
+shared library.
 
 Flags: no contraction (the kernels round as the reference's IEEE path does, an
 fma only where written), denormals kept, correctly rounded division (hipcc's
@@ -34,7 +36,8 @@ FLAGS = [
 
 
 def sources() -> list[Path]:
-    return [CSRC / "scalar.hip", *sorted(CSRC.glob("steps_*.hip"))]
+    return [CSRC / "scalar.hip", *sorted(CSRC.glob("steps_*.hip")), CSRC / "elastic.hip",
+            *sorted(CSRC.glob("elastic_*.hip"))]
 
 
 def _run(cmd: list[str], verbose: bool) -> None:

@@ -76,6 +76,11 @@ def adjoint_signal(sim: Simulation, dphi, sensors, scale: float = 1.0, delay: in
 
 
 # --------------------------------------------------------------------- helpers
+def _elastic(sim: Simulation) -> bool:
+    """Whether `sim` is of the elastic family, whose drivers live in `elastic.py`."""
+    return sim.ncomp > 1 or type(sim).__name__ == "ElasticWave"
+
+
 def _require(sim: Simulation, source: Source, sensors) -> None:
     """The reference's checks before a sensitivity run: points off the ghosts
     (cuwave/sensitivity.py:430-432, 546-548)."""
@@ -180,6 +185,10 @@ def sensitivity(sim: Simulation, source: Source, indicator, sensors, objective):
     transpose with both gradient densities fused in and the transposed PML, the
     layer then frozen out of the gradient (cuwave/sensitivity.py:384-497).
     """
+    if _elastic(sim):
+        from . import elastic
+
+        return elastic.sensitivity(sim, source, indicator, sensors, objective)
     _require(sim, source, sensors)
     mat = sim.build_materials(indicator)
     prop = Propagator(sim, mat, 1)
@@ -365,6 +374,11 @@ def reconstruction_sensitivity(sim: Simulation, source: Source, indicator, senso
     `strip` node count, the `drift` of the rebuilt initial state, the steps per
     `segment` and the forward `sweeps`.
     """
+    if _elastic(sim):
+        from . import elastic
+
+        return elastic.reconstruction_sensitivity(sim, source, indicator, sensors, objective,
+                                                  strip_dtype, checkpoints)
     _require(sim, source, sensors)
     if checkpoints is not None and checkpoints < 0:
         raise ValueError(f"checkpoints must be non-negative: {checkpoints}")
@@ -411,6 +425,9 @@ def superposition_sensitivity(sim: Simulation, source: Source, indicator, sensor
     running the closed lossless domain backwards (cuwave/sensitivity.py:753-893).
     `info` carries `scale` and the `cancellation` the subtraction cost.
     """
+    if _elastic(sim):
+        raise NotImplementedError("superposition_sensitivity is built for the pressure family; "
+                                  "use sensitivity or reconstruction_sensitivity")
     if sim.pml is not None:
         raise NotImplementedError(
             "superposition_sensitivity runs the field backwards, which a pml does not allow; "

@@ -174,6 +174,11 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
     if devices is not None:
         raise NotImplementedError("devices= shot dealing is not part of this build (one GPU)")
     sources = list(sources)
+    if type(sim).__name__ == "ElasticWave" and adjoint in (sensitivity, reconstruction_sensitivity):
+        from . import elastic
+
+        return elastic.misfit_gradient(sim, sources, indicator, sensors, observed, objective,
+                                       batch, checkpoints)
     if adjoint not in (sensitivity, reconstruction_sensitivity):
         cost, gradient = 0.0, None
         d_mass, d_stiff = sim.parametrization_jacobian(indicator)

@@ -77,6 +77,28 @@ class Simulation:
     devices: tuple[int, ...] | None = None
 
     default_boundary = Neumann
+    face_conditions = (Neumann, Dirichlet)  # the conditions a face may carry
+    pml_shifted = False  # whether a pml needs a frequency shift (a static mode)
+    pml_walls = None  # the conditions a lined face may end on, None for any
+
+    @property
+    def pml_margin(self) -> int:
+        """Undamped layer nodes before the front (cuwave/boundary.py:335-336)."""
+        return 0
+
+    @property
+    def ncomp(self) -> int:
+        """Field components: one for a pressure field."""
+        return 1
+
+    @property
+    def node_rows(self) -> int:
+        """Rows of a position: the axes, plus a leading component row for a vector field."""
+        return self.ndim + (self.ncomp > 1)
+
+    def grid_rows(self, position) -> np.ndarray:
+        """The `ndim` spatial rows of `position`, a leading component row dropped."""
+        return np.asarray(as_numpy(position))[-self.ndim:]
 
     def __post_init__(self) -> None:
         self.Nx = tuple(int(n) for n in self.Nx)
@@ -95,7 +117,8 @@ class Simulation:
             raise ValueError(f"precision is float32 or float64, not {self.precision!r}")
         self.np_dtype = np.float32 if self.precision == "float32" else np.float64
         self.dtype = torch.float32 if self.precision == "float32" else torch.float64
-        self.boundary = canonical_boundary(self.boundary, self.ndim, self.default_boundary)
+        self.boundary = canonical_boundary(self.boundary, self.ndim, self.default_boundary,
+                                           self.face_conditions)
         if self.space_order % 2 != 0 or not 2 <= self.space_order <= 8:
             raise ValueError("space_order must be 2, 4, 6 or 8")
         if self.stencil not in STENCILS:
@@ -142,7 +165,7 @@ class Simulation:
 
     def require_interior(self, position, what: str) -> None:
         """Raise if a node of `position` is a ghost (cuwave/sensitivity.py:103-118)."""
-        rows = np.asarray(as_numpy(position))
+        rows = self.grid_rows(position)
         if rows.size == 0:
             return
         lo = int(rows.min())
@@ -220,6 +243,12 @@ def simulate_shots(sim: Simulation, sources, indicator, sensors=None, record_eve
     """
     from .engine import Propagator, sensor_points, source_points
 
+    if sim.ncomp > 1 or type(sim).__name__ == "ElasticWave":
+        from . import elastic
+
+        if record_every is not None:
+            raise NotImplementedError("record_every is built for the pressure family")
+        return elastic.simulate_shots(sim, sources, indicator, sensors)
     sources = list(sources)
     for s in sources:
         sim.require_interior(s.position, "source")

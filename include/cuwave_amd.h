@@ -233,6 +233,112 @@ int cw_scalar_forward_static(const cw_forward_args *args, int32_t t0,
 
 int cw_scalar_superposed(const cw_superposed_args *args, void *stream);
 
+/* ------------------------------------------------------------ elastic
+ * The staggered isotropic elastic family (cuwave/elastic.py:154-794): the
+ * displacement component c half a node up axis c, gamma scaling density and
+ * stiffness alike. Fields are (ncomp, *np, lanes), ncomp = ndim; the stress
+ * scratch (nvoigt, *np, lanes), nvoigt = ndim (ndim + 1) / 2; the gradient
+ * accumulators per node: mass (ncomp, *np), normal (*np), shear
+ * (nvoigt - ndim, *np). */
+
+/* the grid, operator and material constants (cuwave/elastic.py:154-345;
+ * the closure of the staggered SBP pair, cuwave/stencils.py:60-213) */
+typedef struct cw_elastic_grid {
+  int32_t ndim;       /* 1, 2 or 3 */
+  int32_t order;      /* space order: 2, 4, 6 or 8 */
+  int32_t n[3];       /* logical extents, ghosts included */
+  int32_t np[3];      /* padded extents */
+  int32_t nbatch, lanes, dtype;
+  int32_t clamped;    /* faces held at zero (Clamped), bit 2 axis + side; the
+                         others are traction-free */
+  int32_t closure[4]; /* node rows, half rows, node span, half span */
+  double stag[4];     /* the full-radius staggered weights, stag[k - 1] */
+  double lame, shear, density;
+  double inv_dx[3];   /* the strain factors 1 / dx */
+  double step[3];     /* the update factors dt^2 / dx */
+} cw_elastic_grid;
+
+/* the material: gamma and what cw_elastic_prepare makes of it */
+typedef struct cw_elastic_material {
+  const void *gamma; /* nodal design field */
+  void *prep;        /* (nvoigt, field): the inverse inertia of each
+                        component point (rows 0 .. ndim - 1), then the shear
+                        modulus scale of each pair point (the pair's stress
+                        row); zeroed before prepare */
+  const void *rows;  /* the closure rows in the grid's dtype: node rows
+                        (closure[0] x closure[2]) then half rows
+                        (closure[1] x closure[3]) */
+} cw_elastic_material;
+
+/* the PML folded into both launches of the pair (cuwave/elastic.py:597-612) */
+typedef struct cw_elastic_layer {
+  int32_t lo[3], hi[3];   /* layer rows behind the low / high face, 0 none */
+  const void *profile[3]; /* (4, n[d]): a, b at nodes and at half points */
+  void *memory[3];        /* (rows, slab nodes, lanes), zeroed: 2 ndim rows
+                             forward (strain, then divergence memory), 3 ndim
+                             in the adjoint (the strain memory twice) */
+  int32_t phase;          /* 0 forward; 1 or 2 adjoint, reading its strain
+                             memory off row 0 or 2 ndim, alternating a step */
+} cw_elastic_layer;
+
+/* one step of the pair over a batch (cuwave/kernels/elastic.cu:754-853):
+ * the stresses of u1 into sigma, then u2 = -u0 + 2 u1 + minv div(sigma);
+ * u2 may be u0. With g_mass set it is the adjoint pair: both gradient
+ * densities of u1 against the forward field v1, the shots summed into the
+ * accumulators (cuwave/elastic.py:436-488). */
+typedef struct cw_elastic_step_args {
+  cw_elastic_grid grid;
+  cw_elastic_material mat;
+  cw_elastic_layer pml;
+  void *sigma;
+  const void *u0, *u1;
+  void *u2;
+  void *g_mass, *g_normal, *g_shear; /* NULL for a forward step */
+  const void *v1;
+  double mass_factor;                /* 1 / dt^2 (cuwave/wave.py:597-599) */
+} cw_elastic_step_args;
+
+/* the densities of a forward triplet against the adjoint l1
+ * (cuwave/kernels/elastic_sensitivity.cu:115-158) */
+typedef struct cw_elastic_gradient_args {
+  cw_elastic_grid grid;
+  const void *rows;
+  void *g_mass, *g_normal, *g_shear;
+  const void *u0, *u1, *u2, *l1;
+  double mass_factor;
+} cw_elastic_gradient_args;
+
+/* points of a field: `starts` (runs + 1) CSR groups the entries into runs of
+ * one node each (a run's entries keep their order); entry e touches element
+ * lin[e] lanes + b of the field for shot lane b (every lane when shot[e] < 0)
+ * and element row row_stride + col[e] col_stride + b shot_stride of the
+ * table. kind 0 injects (field += weight table, cuwave/kernels/common.cuh:
+ * 456-464), 1 records (table = field, :483-490), 2 replays (field = table,
+ * :493-503), 3 injects the adjoint load and its inertia density
+ * (g_mass[lin] += -mass_factor u1 load, :467-480). */
+typedef struct cw_elastic_point_args {
+  const int32_t *starts, *lin, *shot, *col;
+  int32_t runs;
+  int32_t kind;
+  int64_t row_stride, col_stride, shot_stride;
+  int64_t row;
+  void *field;
+  void *table;
+  const void *weight;
+  void *g_mass;
+  const void *u1;
+  double mass_factor;
+} cw_elastic_point_args;
+
+/* the prepared material (point inverse inertia, cuwave/kernels/elastic.cu:
+ * 200-211; pair shear scale, :189-197) of every interior node */
+int cw_elastic_prepare(const cw_elastic_grid *grid, const cw_elastic_material *mat,
+                       void *stream);
+int cw_elastic_step(const cw_elastic_step_args *args, void *stream);
+int cw_elastic_gradient(const cw_elastic_gradient_args *args, void *stream);
+int cw_elastic_points(const cw_elastic_grid *grid, const cw_elastic_point_args *args,
+                      void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,26 @@ def product_sim(z, **override):
     return scalar.AcousticWave(*args, rho1=r1, rho2=r2, kappa1=k1, kappa2=k2, **common)
 
 
+def elastic_sim(z, **override):
+    """The `cuwave_amd` elastic simulation an elastic golden case stores (faces coded
+    0 traction, 1 clamped)."""
+    from cuwave_amd import boundary, elastic
+
+    bmap = {0: boundary.Traction, 1: boundary.Clamped}
+    bnd = tuple((bmap[int(a)], bmap[int(b)]) for a, b in z["boundary"])
+    pml = None
+    if "pml" in z:
+        w, c, r, f = (float(v) for v in z["pml"])
+        pml = boundary.Pml(int(w), c, r, f, tuple(int(x) for x in z["pml_faces"]))
+    rho, cp, cs = (float(v) for v in z["elastic"])
+    kw = dict(precision=str(z["precision"]), space_order=int(z["space_order"]),
+              stencil=str(z["stencil"]), boundary=bnd, pml=pml, density=rho, wavespeed_p=cp,
+              wavespeed_s=cs, plane=str(z["plane"]))
+    kw.update(override)
+    return elastic.ElasticWave(tuple(int(n) for n in z["Nx"]), tuple(float(d) for d in z["dx"]),
+                               int(z["N"]), float(z["dt"]), **kw)
+
+
 def oracle_sim(z):
     from oracle import driver
 
