@@ -62,7 +62,7 @@ def models(roi=ROI, seed_noise=2, seed_ellipses=3):
     return true, start
 
 
-def workload(shots=SHOTS, steps=STEPS, roi=ROI, receivers=RECEIVERS):
+def workload(shots=SHOTS, steps=STEPS, roi=ROI, receivers=RECEIVERS, precision="float32"):
     """The simulation, the true and start indicators, the shots and the receiver nodes."""
     from cuwave_amd import boundary, scalar, wave
     from cuwave_amd.signals import ricker
@@ -71,7 +71,7 @@ def workload(shots=SHOTS, steps=STEPS, roi=ROI, receivers=RECEIVERS):
     width = LAYER + 1  # the wall node is part of the layer
     n = roi + 2 * LAYER + 2
     dt = 0.4 * DX / float(true.max())
-    sim = scalar.AcousticWave((n, n), (DX, DX), steps, dt, precision="float32", space_order=8,
+    sim = scalar.AcousticWave((n, n), (DX, DX), steps, dt, precision=precision, space_order=8,
                               rho1=1.0, rho2=1.0, kappa1=C1**2, kappa2=C2**2,
                               pml=boundary.Pml(width, float(true.max()), 1e-6))
 
@@ -88,31 +88,51 @@ def workload(shots=SHOTS, steps=STEPS, roi=ROI, receivers=RECEIVERS):
     return sim, indicator(true), indicator(start), sources, sensors
 
 
+class _GridSensors:
+    """Receivers on grid nodes, as `utils.Sensors` (the record is the traces)."""
+
+    def __new__(cls, nodes):
+        from cuwave_amd import utils
+
+        class NodeSensors(utils.Sensors):
+            def __init__(self, nodes):
+                self.nodes = nodes
+                self.count = nodes.shape[1]
+                self.weights = np.ones((self.count, 1), dtype=np.float32)
+
+        return NodeSensors(nodes)
+
+
+def velocity_of(sim, indicator):
+    """The velocity field an AcousticWave(rho1 = rho2 = 1) indicator encodes."""
+    m = 1.0 / C1**2 + np.asarray(indicator, dtype=np.float64) * (1.0 / C2**2 - 1.0 / C1**2)
+    return 1.0 / np.sqrt(m)
+
+
 def gpu_run(args):
+    """The config-2 gradient: velocity -> indicator -> traces (the autograd Function over
+    every shot in one batch) -> L2 misfit -> backward to the velocity."""
     import torch
 
-    from cuwave_amd import utils, wave
+    from cuwave_amd import autograd, wave
     from cuwave_amd.engine import Propagator
 
     sim, true, start, sources, sensors = workload(args.shots, args.nt, args.roi)
     n_int = int(np.prod([n - 2 for n in sim.Nx]))
     B = len(sources)
-
-    class NodeSensors(utils.Sensors):
-        """Receivers on grid nodes: the record is the traces."""
-
-        def __init__(self, nodes):
-            self.nodes = nodes
-            self.count = nodes.shape[1]
-            self.weights = np.ones((self.count, 1), dtype=np.float32)
-
-    rec = NodeSensors(sensors)
+    rec = _GridSensors(sensors)
     _, observed = wave.simulate_shots(sim, sources, true, sensors)
+    observed = observed.contiguous()
+    c_start = torch.as_tensor(velocity_of(sim, start), dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
 
     def step():
-        return utils.misfit_gradient(sim, sources, start, rec, list(observed),
-                                     adjoint=utils.reconstruction_sensitivity)
+        c = c_start.clone().requires_grad_(True)
+        gamma = (1.0 / c**2 - 1.0 / C1**2) / (1.0 / C2**2 - 1.0 / C1**2)
+        traces = autograd.traces(gamma, sim, sources, rec)
+        loss = 0.5 * torch.sum((traces - observed) ** 2)
+        loss.backward()
+        return float(loss), c.grad
 
     for _ in range(args.warmup):
         step()
@@ -137,6 +157,7 @@ def gpu_run(args):
     bytes_bwd = (24.0 * B + 16.0) * n_int  # algorithmic bytes of one adjoint level
     bytes_fwd = (12.0 * B + 8.0) * n_int
     achieved = bytes_bwd / (bwd_ms * 1e-3)
+    measured_at = B == SHOTS and args.roi == ROI
     return {
         "value": work / elapsed / 1e9,
         "ms_per_step": elapsed / args.steps * 1e3,
@@ -157,39 +178,81 @@ def gpu_run(args):
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": traffic_per_launch(),
+            # measured with the profiler at the default 32 shots only
+            "traffic": traffic_per_launch() if measured_at else None,
         },
     }
 
 
-def traffic_per_launch():
-    """HBM bytes per adjoint level from the committed rocprofv3 PMC summary, if any."""
-    path = ROOT / "profiles" / "traffic_adjoint_level_scalar2d_b32.json"
+def one_shot_checks(args, shots: int = 4):
+    """Per the workload: the one-shot-per-launch rate over `shots` shots run one after
+    another (W / wall time, as the batched line), and for the first shot the
+    reconstruction drift and the float32 gradient's error against the float64 run of
+    the same scheme."""
+    import torch
+
+    from cuwave_amd import sensitivity, utils, wave
+
+    sim, true, start, sources, sensors = workload(args.shots, args.nt, args.roi)
+    n_int = int(np.prod([n - 2 for n in sim.Nx]))
+    rec = _GridSensors(sensors)
+    part = sources[:shots]
+    _, observed = wave.simulate_shots(sim, part, true, sensors)
+    observed = [o for o in observed]
+    utils.misfit_gradient(sim, part[:1], start, rec, observed[:1], batch=1)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    utils.misfit_gradient(sim, part, start, rec, observed, batch=1)
+    torch.cuda.synchronize()
+    b1 = 2.0 * len(part) * n_int * sim.N / (time.perf_counter() - t0) / 1e9
+    grads = {}
+    drift = None
+    for precision in ("float32", "float64"):
+        s64 = workload(1, args.nt, args.roi, precision=precision)[0]
+        obs = wave.simulate(s64, part[0], true.astype(s64.np_dtype), sensors)[1]
+        _, g, _, info = sensitivity.reconstruction_sensitivity(
+            s64, part[0], start.astype(s64.np_dtype), sensors, sensitivity.l2_misfit(obs.cpu()))
+        jm, js = s64.parametrization_jacobian()
+        grads[precision] = (jm * g["mass"].double() + js * g["stiff"].double()).cpu()
+        if precision == "float32":
+            drift = info["drift"]
+    g32, g64 = grads["float32"], grads["float64"]
+    err = float(torch.linalg.vector_norm(g32 - g64) / torch.linalg.vector_norm(g64))
+    return {"b1_value": b1, "b1_shots": len(part), "fp64_rel_l2": err, "drift": drift}
+
+
+def traffic_per_launch(name: str = "traffic_adjoint_level_scalar2d_b32.json"):
+    """HBM bytes per adjoint level from a committed rocprofv3 PMC summary, if any."""
+    path = ROOT / "profiles" / name
     if not path.exists():
         return None
     return json.loads(path.read_text()).get("bytes_per_launch")
 
 
-def cpu_baseline(sample_steps: int, shots: int = 1):
+def cpu_baseline(sample_steps: int, runs: int = 3):
     """The oracle (C/OpenMP restatement of the reference's CPU path) on the host cores.
 
-    A bounded sample of the same workload: `shots` shot(s), the full grid, the
-    first `sample_steps` steps, through the oracle's reconstruction adjoint.
+    A bounded sample of the same workload: one shot, the full grid, the first
+    `sample_steps` steps, forward + reconstruction adjoint through the oracle;
+    the median of `runs` timed runs after one warm-up run.
     """
     from oracle import driver as D
 
-    sim, true, start, sources, sensors = workload(shots, sample_steps)
+    sim, true, start, sources, sensors = workload(1, sample_steps)
     osim = D.Sim("AcousticWave", sim.Nx, sim.dx, sim.N, sim.dt, "float32", 8,
                  materials=(1.0, 1.0, C1**2, C2**2),
                  pml=D.Pml(sim.pml.width, sim.pml.wavespeed, sim.pml.reflection))
-    _, observed = D.simulate(osim, sources[0].position, sources[0].signal, true, sensors)
-    t0 = time.perf_counter()
-    for s in sources:
+    s = sources[0]
+    _, observed = D.simulate(osim, s.position, s.signal, true, sensors)
+    times = []
+    for _ in range(runs + 1):
+        t0 = time.perf_counter()
         D.reconstruction_sensitivity(osim, s.position, s.signal, start, sensors,
                                      D.l2_misfit(observed))
-    elapsed = time.perf_counter() - t0
+        times.append(time.perf_counter() - t0)
+    median = float(np.median(times[1:]))
     n_int = int(np.prod([n - 2 for n in sim.Nx]))
-    return 2.0 * shots * n_int * sample_steps / elapsed / 1e9, elapsed
+    return 2.0 * n_int * sample_steps / median / 1e9, median
 
 
 def probe_copy(mib: int = 256, reps: int = 5) -> None:
@@ -303,6 +366,235 @@ def gpu_run_3d(args):
     }
 
 
+def workload_forward(nt: int = 2000):
+    """The plain 2D forward: ROI 256^2 behind a 20-cell PML (Nx = 298 x 298), dx = 1 mm,
+    order 8, float32; 1500 m/s with a 2000 m/s disk of radius 30 cells (centre
+    seed 1); one Ricker source (100 kHz) at ROI (0, 128), 16 receivers on ROI row
+    0; dt = 0.4 dx / 2000."""
+    from cuwave_amd import boundary, scalar, wave
+    from cuwave_amd.signals import ricker
+
+    roi, layer = 256, 20
+    width = layer + 1
+    n = roi + 2 * layer + 2
+    dt = 0.4 * DX / 2000.0
+    sim = scalar.AcousticWave((n, n), (DX, DX), nt, dt, precision="float32", space_order=8,
+                              rho1=1.0, rho2=1.0, kappa1=1500.0**2, kappa2=2000.0**2,
+                              pml=boundary.Pml(width, 2000.0, 1e-6))
+    centre = np.random.default_rng(1).integers(64, 192, 2)
+    a0, a1 = np.meshgrid(np.arange(sim.Nx_padded[0]), np.arange(sim.Nx_padded[1]), indexing="ij")
+    disk = (a0 - width - centre[0]) ** 2 + (a1 - width - centre[1]) ** 2 <= 30**2
+    indicator = disk.astype(np.float32)
+    wavelet = ricker(np.arange(nt) * dt, 1.0, 1e5)[:, None]
+    source = wave.Source(np.array([[width], [width + 128]]), wavelet)
+    cols = np.round(np.linspace(0, roi - 1, 16)).astype(int) + width
+    sensors = np.stack([np.full(16, width), cols])
+    return sim, indicator, source, sensors
+
+
+def forward_line(args) -> dict:
+    """Gcell-updates/s of the plain forward (W = n_int N per run), timed over whole runs."""
+    import torch
+
+    from cuwave_amd import wave
+    from cuwave_amd.engine import Propagator
+
+    nt = args.nt if args.nt != STEPS else 2000
+    sim, indicator, source, sensors = workload_forward(nt)
+    n_int = int(np.prod([n - 2 for n in sim.Nx]))
+    for _ in range(args.warmup):
+        wave.simulate(sim, source, indicator, sensors)
+    torch.cuda.synchronize()
+    Propagator.timings = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        field, traces = wave.simulate(sim, source, indicator, sensors)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    timings, Propagator.timings = Propagator.timings, None
+    launch_ms = sum(s.elapsed_time(e) for _, _, s, e in timings) / sum(n for _, n, _, _ in timings)
+    achieved = 20.0 * n_int / (launch_ms * 1e-3)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import driver as D
+
+        osim = D.Sim("AcousticWave", sim.Nx, sim.dx, sim.N, sim.dt, "float32", 8,
+                     materials=(1.0, 1.0, 1500.0**2, 2000.0**2),
+                     pml=D.Pml(sim.pml.width, sim.pml.wavespeed, sim.pml.reflection))
+        times = []
+        for _ in range(4):
+            c0 = time.perf_counter()
+            D.simulate(osim, source.position, source.signal, indicator, sensors)
+            times.append(time.perf_counter() - c0)
+        med = float(np.median(times[1:]))
+        cores = len(os.sched_getaffinity(0))
+        cpu = {"value": n_int * nt / med / 1e9, "unit": "Gcell-updates/s",
+               "cores": min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))),
+               "kind": "port",
+               "sample": f"the whole {nt}-step forward through the C oracle; median of 3 runs "
+                         f"after a warm-up ({med:.2f} s a run)"}
+    return {
+        "metric": "Gcell-updates/s (2D scalar forward, one shot)",
+        "value": n_int * nt * args.steps / elapsed / 1e9, "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2D scalar forward: traces and final field, one shot",
+                   "grid": "298x298 (ROI 256^2 + 20-cell PML)", "space_order": 8,
+                   "shots": 1, "receivers": 16, "time_steps": nt, "interior_nodes": n_int,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None},
+        "cpu_baseline": cpu,
+        "kernels": {"step_us": launch_ms * 1e3,
+                    "note": "a step's launches on the stream, averaged over every step run"},
+    }
+
+
+EL_ROI, EL_LAYER, EL_DX, EL_FREQ = (1024, 2048), 32, 5e-4, 5e5
+EL_SHOTS, EL_RECEIVERS, EL_STEPS = 16, 512, 3000
+EL_RHO0, EL_VP0 = 2200.0, 3000.0
+
+
+def workload_elastic(shots=EL_SHOTS, nt=EL_STEPS, roi=EL_ROI):
+    """The 2D elastic case in the reference's parametrization.
+
+    Depth roi[0] on axis 0 under a traction-free top face, width roi[1] on
+    axis 1, a PML of EL_LAYER cells on the other three faces (clamped walls,
+    CFS shift 1e5 Hz), dx = 0.5 mm, order 4, float32. The reference's
+    ElasticWave holds vp and vs fixed (vp0 = 3000 m/s, vs = vp / 1.8) and lets
+    gamma scale density and both Lame parameters alike, so the model's 8
+    layers (boundaries at sorted seed-6 depths) are density layers, gamma =
+    rho / rho0 with rho ~ U[1800, 2600] and rho0 = 2200, and the 10 % anomaly
+    a seed-7 rectangle of gamma x 1.1; the start model has no anomaly. 16
+    vertical forces (Ricker, 500 kHz) on the free surface at ROI columns
+    linspace(64, 1984), 512 receivers of the vertical displacement every 4
+    columns on the same row.
+    """
+    from cuwave_amd import boundary, elastic, wave
+    from cuwave_amd.signals import ricker
+
+    width = EL_LAYER + 1  # the wall node is part of the layer
+    n0, n1 = roi[0] + EL_LAYER + 2, roi[1] + 2 * EL_LAYER + 2
+    dt = 0.4 * EL_DX / EL_VP0
+    sim = elastic.ElasticWave(
+        (n0, n1), (EL_DX, EL_DX), nt, dt, precision="float32", space_order=4,
+        density=EL_RHO0, wavespeed_p=EL_VP0, wavespeed_s=EL_VP0 / 1.8,
+        boundary=((boundary.Traction, boundary.Clamped), (boundary.Clamped, boundary.Clamped)),
+        pml=boundary.Pml(width, EL_VP0, 1e-6, 1e5, (1, 2, 3)))
+    rng = np.random.default_rng(6)
+    tops = np.sort(rng.integers(1, roi[0], 7))
+    rho = rng.uniform(1800.0, 2600.0, 8)
+    depth = np.arange(sim.Nx_padded[0]) - 1
+    layer = np.searchsorted(tops, depth, side="right")
+    start = np.repeat((rho[layer] / EL_RHO0)[:, None], sim.Nx_padded[1], axis=1)
+    box = np.random.default_rng(7)
+    r0 = int(box.integers(roi[0] // 4, roi[0] // 2))
+    c0 = int(box.integers(roi[1] // 4, roi[1] // 2))
+    true = start.copy()
+    true[1 + r0:1 + r0 + roi[0] // 8, width + c0:width + c0 + roi[1] // 8] *= 1.1
+    t = np.arange(nt) * dt
+    wavelet = (ricker(t, 1.0, EL_FREQ) / (EL_DX * EL_DX))[:, None]
+    cols = np.round(np.linspace(64, roi[1] - 64, shots)).astype(int) + width
+    sources = [wave.Source(np.array([[0], [1], [c]]), wavelet) for c in cols]
+    rec = np.arange(0, roi[1], max(roi[1] // EL_RECEIVERS, 1))[:EL_RECEIVERS] + width
+    sensors = np.stack([np.zeros(len(rec), int), np.ones(len(rec), int), rec])
+    return sim, true.astype(np.float32), start.astype(np.float32), sources, sensors
+
+
+class _NodeReceivers:
+    """Receivers on unknowns: the record is the traces."""
+
+    def __init__(self, nodes):
+        self.nodes = nodes
+
+    def objective(self, objective):
+        return objective
+
+    def traces(self, record):
+        return record
+
+
+def gpu_run_elastic(args):
+    """The elastic FWI gradient: every shot's forward, the L2 misfit, the reverse sweep."""
+    import torch
+
+    from cuwave_amd import elastic, utils, wave
+
+    sim, true, start, sources, sensors = workload_elastic(args.shots, args.nt)
+    n_int = int(np.prod([n - 2 for n in sim.Nx]))
+    B = len(sources)
+    _, observed = wave.simulate_shots(sim, sources, true, sensors)
+    observed = list(observed)
+    rec = _NodeReceivers(sensors)
+
+    def step():
+        return utils.misfit_gradient(sim, sources, start, rec, observed,
+                                     adjoint=utils.reconstruction_sensitivity)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    elastic._Reconstruction.timings = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cost, gradient = step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    timings, elastic._Reconstruction.timings = elastic._Reconstruction.timings, None
+    per = {}
+    for kind, steps, s, e in timings:
+        tot = per.setdefault(kind, [0.0, 0])
+        tot[0] += s.elapsed_time(e)
+        tot[1] += steps
+    bwd_ms = per["backward"][0] / per["backward"][1]
+    fwd_ms = per["forward"][0] / per["forward"][1]
+    # a forward step: u1 (2 components) read, u0 read and u2 written over it,
+    # gamma and the prepared material once a node; an adjoint level: the
+    # reverse and adjoint steps, the triplet and lambda read by the gradient,
+    # three accumulators read and written (SURVEY's per-unit budget)
+    bytes_fwd = (24.0 * B + 12.0) * n_int
+    bytes_bwd = (48.0 * B + 36.0) * n_int
+    achieved = bytes_bwd / (bwd_ms * 1e-3)
+    return {
+        "value": 2.0 * B * n_int * sim.N * args.steps / elapsed / 1e9,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "cost": cost, "B": B, "n_int": n_int, "sim": sim,
+        "grad_norm": float(torch.linalg.vector_norm(gradient)),
+        "kernels": {"backward_ms": bwd_ms, "forward_ms": fwd_ms,
+                    "backward_GBps": achieved / 1e9,
+                    "forward_GBps": bytes_fwd / (fwd_ms * 1e-3) / 1e9},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": traffic_per_launch("traffic_adjoint_level_elastic2d_b16.json")},
+    }
+
+
+def cpu_baseline_elastic(sample_steps: int, runs: int = 3):
+    """The elastic oracle on the host cores: one shot, the full grid, the first
+    `sample_steps` steps of forward + reconstruction adjoint; the median of
+    `runs` timed runs after one warm-up run."""
+    from oracle import driver as D
+    from oracle import elastic as E
+
+    sim, true, start, sources, sensors = workload_elastic(1, sample_steps)
+    osim = E.ElSim(sim.Nx, sim.dx, sim.N, sim.dt, sim.precision, sim.space_order, sim.stencil,
+                   ((0, 1), (1, 1)), sim.density, sim.wavespeed_p, sim.wavespeed_s, "strain",
+                   D.Pml(sim.pml.width, sim.pml.wavespeed, sim.pml.reflection,
+                         sim.pml.frequency, sim.pml.faces))
+    src = sources[0]
+    _, observed = E.simulate(osim, src.position, src.signal, true, sensors)
+    times = []
+    for _ in range(runs + 1):
+        t0 = time.perf_counter()
+        E.reconstruction_sensitivity(osim, src.position, src.signal, start, sensors,
+                                     D.l2_misfit(observed))
+        times.append(time.perf_counter() - t0)
+    median = float(np.median(times[1:]))
+    n_int = int(np.prod([n - 2 for n in sim.Nx]))
+    return 2.0 * n_int * sample_steps / median / 1e9, median
+
+
 def main():
     p = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     p.add_argument("--gpus", type=int, default=1)
@@ -311,15 +603,32 @@ def main():
     p.add_argument("--shots", type=int, default=SHOTS)
     p.add_argument("--nt", type=int, default=STEPS)
     p.add_argument("--roi", type=int, default=ROI)
-    p.add_argument("--cpu-sample", type=int, default=1500, help="oracle steps for the CPU baseline")
+    p.add_argument("--cpu-sample", type=int, default=300,
+                   help="oracle steps of the CPU baseline's sample (median of 3 after a warm-up)")
+    p.add_argument("--no-checks", action="store_true",
+                   help="skip the one-shot-per-launch rate, the drift and the float64 error")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--probe-copy", action="store_true",
                    help="run only a 256 MiB device copy (counter calibration)")
-    p.add_argument("--workload", choices=("2d", "3d"), default="2d",
-                   help="3d: the 3D acoustic gradient on a 354^3 grid instead (its own line)")
+    p.add_argument("--workload", choices=("2d", "3d", "elastic", "forward"), default="2d",
+                   help="3d: the 3D acoustic gradient on a 354^3 grid instead (its own line); "
+                        "elastic: the 2D elastic gradient on 1058 x 2114, 16 shots; "
+                        "forward: the plain 298 x 298 forward, one shot")
     args = p.parse_args()
     if args.probe_copy:
         probe_copy()
+        return
+    if args.workload == "forward":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU"}))
+            sys.exit(2)
+        print(json.dumps(forward_line(args)))
+        return
+    if args.workload == "elastic":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU"}))
+            sys.exit(2)
+        print(json.dumps(elastic_line(args)))
         return
     if args.workload == "3d":
         if args.gpus != 1:
@@ -332,6 +641,7 @@ def main():
                                    "GPUs by design only; run with --gpus 1"}))
         sys.exit(2)
     res = gpu_run(args)
+    checks = None if args.no_checks else one_shot_checks(args)
     cpu = None
     if not args.no_cpu:
         cores = len(os.sched_getaffinity(0))
@@ -339,7 +649,8 @@ def main():
         cpu = {"value": value, "unit": "Gcell-updates/s", "cores": min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))),
                "kind": "port",
                "sample": f"1 shot, full 1090x1090 grid, first {args.cpu_sample} of {STEPS} steps, "
-                         f"forward + reconstruction adjoint through the C oracle ({seconds:.1f} s)"}
+                         f"forward + reconstruction adjoint through the C oracle; median of 3 "
+                         f"runs after a warm-up ({seconds:.1f} s a run)"}
     line = {
         "metric": "Gcell-updates/s (shot-batched 2D scalar forward+adjoint)",
         "value": res["value"],
@@ -354,7 +665,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "2D scalar FWI gradient: forward + reconstruction adjoint, shot-batched",
+            "workload": "2D scalar FWI gradient wrt velocity: velocity -> indicator -> traces "
+                        "(torch.autograd.Function) -> L2 -> backward; forward + reconstruction "
+                        "adjoint, shot-batched",
             "grid": f"{res['sim'].Nx[0]}x{res['sim'].Nx[1]} (ROI {args.roi}^2 + {LAYER}-cell PML)",
             "space_order": 8,
             "shots": res["B"],
@@ -367,7 +680,47 @@ def main():
         "cpu_baseline": cpu,
         "kernels": res["kernels"],
     }
+    if checks is not None:
+        line["one_shot_per_launch"] = {
+            "value": checks["b1_value"], "unit": "Gcell-updates/s",
+            "sample": f"{checks['b1_shots']} shots one after another, each its own launches",
+            "speedup": res["value"] / checks["b1_value"]}
+        line["accuracy"] = {"fp32_vs_fp64_gradient_rel_l2": checks["fp64_rel_l2"],
+                            "reconstruction_drift": checks["drift"],
+                            "sample": "the first shot, the same scheme in float32 and float64"}
     print(json.dumps(line))
+
+
+def elastic_line(args) -> dict:
+    shots = args.shots if args.shots != SHOTS else EL_SHOTS
+    nt = args.nt if args.nt != STEPS else EL_STEPS
+    args.shots, args.nt = shots, nt
+    res = gpu_run_elastic(args)
+    cpu = None
+    if not args.no_cpu:
+        cores = len(os.sched_getaffinity(0))
+        sample = 20
+        value, seconds = cpu_baseline_elastic(sample)
+        cpu = {"value": value, "unit": "Gcell-updates/s",
+               "cores": min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))),
+               "kind": "port",
+               "sample": f"1 shot, full {res['sim'].Nx[0]}x{res['sim'].Nx[1]} grid, the first "
+                         f"{sample} steps, forward + reconstruction adjoint through the C oracle; "
+                         f"median of 3 runs after a warm-up ({seconds:.1f} s a run)"}
+    sim = res["sim"]
+    return {
+        "metric": "Gcell-updates/s (shot-batched 2D elastic forward+adjoint)",
+        "value": res["value"], "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2D elastic FWI gradient: forward + reconstruction adjoint, "
+                               "shot-batched, gamma-layered (the reference's parametrization)",
+                   "grid": f"{sim.Nx[0]}x{sim.Nx[1]} (ROI 1024x2048, free top, 32-cell PML)",
+                   "space_order": 4, "shots": res["B"], "receivers": EL_RECEIVERS,
+                   "time_steps": nt, "interior_nodes": res["n_int"],
+                   "parallelism": "single GPU, shots batched per launch"},
+        "roofline": res["roofline"], "cpu_baseline": cpu, "kernels": res["kernels"],
+    }
 
 
 if __name__ == "__main__":

@@ -229,6 +229,43 @@ class ElasticPointArgs(ctypes.Structure):
     ]
 
 
+class Conv3x3Args(ctypes.Structure):
+    _fields_ = [
+        ("n", I32),
+        ("h", I32),
+        ("w", I32),
+        ("cin", I32),
+        ("cout", I32),
+        ("x", VP),
+        ("weight", VP),
+        ("weight_flipped", VP),
+        ("bias", VP),
+        ("relu", I32),
+        ("y", VP),
+        ("dy", VP),
+        ("dx", VP),
+        ("dweight", VP),
+        ("workspace", VP),
+        ("splits", I32),
+    ]
+
+
+class AdamWArgs(ctypes.Structure):
+    _fields_ = [
+        ("count", I64),
+        ("param", VP),
+        ("grad", VP),
+        ("m", VP),
+        ("v", VP),
+        ("lr", ctypes.c_float),
+        ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float),
+        ("eps", ctypes.c_float),
+        ("weight_decay", ctypes.c_float),
+        ("step", I32),
+    ]
+
+
 # the bytes of the library's own scratch on each GPU (CW_STATIC_SCRATCH_BYTES)
 STATIC_SCRATCH_BYTES = 64 << 20
 
@@ -237,6 +274,7 @@ SIGNATURES = {
     "cw_version": ([], I32),
     "cw_last_error": ([], ctypes.c_char_p),
     "cw_set_side_stream": ([I32], I32),
+    "cw_set_stream_threshold": ([I32], I32),
     "cw_tile_count": ([ctypes.POINTER(Grid), ctypes.POINTER(I32)], I32),
     "cw_point_lists": ([ctypes.POINTER(Grid), VP, I32, VP, VP], I32),
     "cw_scalar_prepare": ([ctypes.POINTER(Grid), ctypes.POINTER(Materials), VP], I32),
@@ -248,6 +286,11 @@ SIGNATURES = {
     "cw_elastic_step": ([ctypes.POINTER(ElasticStepArgs), VP], I32),
     "cw_elastic_gradient": ([ctypes.POINTER(ElasticGradientArgs), VP], I32),
     "cw_elastic_points": ([ctypes.POINTER(ElasticGrid), ctypes.POINTER(ElasticPointArgs), VP], I32),
+    "cw_conv3x3_bf16_fwd": ([ctypes.POINTER(Conv3x3Args), VP], I32),
+    "cw_conv3x3_bf16_bwd_data": ([ctypes.POINTER(Conv3x3Args), VP], I32),
+    "cw_conv3x3_bf16_bwd_weight": ([ctypes.POINTER(Conv3x3Args), VP], I32),
+    "cw_conv3x3_weights": ([I32, I32, VP, VP, VP, VP], I32),
+    "cw_adamw": ([ctypes.POINTER(AdamWArgs), VP], I32),
 }
 
 _LIB: ctypes.CDLL | None = None

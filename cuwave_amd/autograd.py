@@ -10,8 +10,9 @@ One adjoint sweep over the batch then rebuilds the forward field backwards,
 as `reconstruction_sensitivity` does. The mass and stiffness densities it
 sums are chained onto the indicator by the simulation's
 `parametrization_jacobian`, as the reference's caller does by hand
-(cuwave/utils.py:19-24). That jacobian may be constant or field-valued, as
-for `BulkDensityWave`.
+(cuwave/utils.py:19-24). That jacobian may be constant, when the kernels
+chain it into one accumulator as they sum (and skip a density whose factor
+is zero), or field-valued, as for `BulkDensityWave`.
 
 The forward half runs inside the autograd forward and keeps the boundary
 strip and the checkpoints. The adjoint half runs only if a gradient is asked
@@ -56,8 +57,16 @@ class Traces(torch.autograd.Function):
         sim.require_interior(sensors.nodes, "sensor")
         mat = sim.build_materials(ind)
         run = _Reconstruction(sim, mat, list(sources), sensors.nodes, checkpoints=checkpoints)
-        g_mass, g_stiff = run.prop.accumulator(), run.prop.accumulator()
-        sweep = run.sweep(dict(g_mass=g_mass, g_stiff=g_stiff, skip_layer=True))
+        jm, js = sim.parametrization_jacobian(ind)
+        if np.isscalar(jm) and np.isscalar(js):
+            # a constant Jacobian chains in the kernel: one accumulator, and no
+            # stiffness density at all where its factor is zero
+            g_mass, g_stiff = run.prop.accumulator(), None
+            mode = dict(g_mass=g_mass, combine=(jm, js), skip_layer=True)
+        else:
+            g_mass, g_stiff = run.prop.accumulator(), run.prop.accumulator()
+            mode = dict(g_mass=g_mass, g_stiff=g_stiff, skip_layer=True)
+        sweep = run.sweep(mode)
         record = next(sweep)
         out = torch.stack([sensors.traces(record[b]) for b in range(run.B)])
         ctx.state = (sim, ind, run, sweep, g_mass, g_stiff, sensors, indicator.device,
@@ -75,6 +84,9 @@ class Traces(torch.autograd.Function):
             sweep.send(dphi)
         except StopIteration:
             pass
+        if g_stiff is None:
+            grads = _zero_outside(_finalize(sim, {"g": g_mass}, run.dev), ~run.valid, run.dev)
+            return grads["g"].to(device, dtype), None, None, None, None
         grads = _finalize(sim, {"mass": g_mass, "stiff": g_stiff}, run.dev)
         grads = _zero_outside(grads, ~run.valid, run.dev)
         gi = _chain(sim, ind, grads["mass"], grads["stiff"])

@@ -539,11 +539,14 @@ __device__ __forceinline__ void settle_memory(const ElGeom<T> &g, const ElLayer<
 
 // ---------------------------------------------------------------- kernels
 // thread -> (interior node, first lane): a node's lanes / V threads are
-// neighbours in the wave; false past the last node (whole node groups)
+// neighbours in the wave; false past the last node (whole node groups).
+// Blocks go to the XCDs in contiguous runs of nodes (cw::xcd_tile), so the
+// rows a block's taps read along the slow axes were fetched into the same
+// XCD's L2 by the blocks before it.
 template <int NDIM, int V, typename T>
 __device__ __forceinline__ bool node_of(const ElGeom<T> &g, int *A, int64_t *idx, int *lane0) {
   const int per = g.lanes / V;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)cw::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int64_t i = t / per;
   if (i >= g.interior)
     return false;

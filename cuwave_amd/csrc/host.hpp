@@ -169,6 +169,10 @@ template <typename T> cw::Strip<T> make_strip(const cw_strip &s) {
 constexpr int kChunk = 32; // rows a streaming thread marches
 static_assert(kChunk <= 32, "a chunk's rows with points are one 32-bit mask");
 
+// the fewest blocks a stream box must have for the stream kernels to run
+// (cw_set_stream_threshold); below it the general kernels take every node
+int stream_threshold();
+
 // the stream box of a 2D grid: the nodes order / 2 + 2 off the walls (out of
 // reach of the wall mirror, the refill, the held nodes and the adjoint's
 // reflected taps) that neither lie in a layer slab nor reach one with their
@@ -205,6 +209,10 @@ inline cw::Box stream_box(const cw_grid &g, const cw_pml &pml) {
   box.c1 = (box.y1 - box.y0 + box.chunk - 1) / box.chunk;
   box.k0 = 0;
   box.k1 = (box.x1 - xa + C - 1) / C;
+  // a box of few blocks marches few long chunks, each one row at a time: a
+  // small grid steps faster with one independent thread a node everywhere
+  if ((box.c1 - box.c0) * (box.k1 - box.k0) < stream_threshold())
+    return cw::Box{0, 0, 0, 0, kChunk};
   return box;
 }
 

@@ -73,6 +73,11 @@ void return_side(Side *s) {
 int side_enabled() { return g_side_enabled.load(); }
 void set_side_enabled(int on) { g_side_enabled.store(on ? 1 : 0); }
 
+// two blocks a CU: a box that cannot fill the GPU once over marches too few
+// chunks to pay for its row-by-row latency
+std::atomic<int> g_stream_threshold{512};
+int stream_threshold() { return g_stream_threshold.load(); }
+
 } // namespace cwh
 
 namespace cwh {
@@ -263,6 +268,10 @@ int cw_set_side_stream(int32_t enabled) {
   const int was = cwh::side_enabled();
   cwh::set_side_enabled(enabled);
   return was;
+}
+
+int cw_set_stream_threshold(int32_t blocks) {
+  return cwh::g_stream_threshold.exchange(blocks < 0 ? 0 : blocks);
 }
 
 const char *cw_last_error(void) { return cwh::g_error.c_str(); }

@@ -625,7 +625,32 @@ class _Reconstruction:
 
     Three forward slots hold the triplet the gradient reads; a forward step t
     writes u^t into slot (t + 2) % 3 (cuwave/sensitivity.py:500-750).
+
+    Set `timings` to a list to have every forward segment and every segment's
+    reverse levels bracketed by HIP events on the launch stream: entries
+    (kind, steps, start, end), kind "forward" or "backward".
     """
+
+    timings: list | None = None
+
+    def _timed(self, kind: str, steps: int):
+        import contextlib
+
+        timings = _Reconstruction.timings
+        if timings is None or steps <= 0:
+            return contextlib.nullcontext()
+
+        @contextlib.contextmanager
+        def bracket():
+            stream = torch.cuda.current_stream()
+            start = torch.cuda.Event(enable_timing=True)
+            end = torch.cuda.Event(enable_timing=True)
+            start.record(stream)
+            yield
+            end.record(stream)
+            timings.append((kind, steps, start, end))
+
+        return bracket()
 
     def __init__(self, sim: ElasticWave, mat: dict, sources, sensors, checkpoints=None):
         self.sim, self.mat, self.sensors = sim, mat, sensors
@@ -679,8 +704,9 @@ class _Reconstruction:
         first = a == self.recorded
         if first:
             self.recorded = b
-        for k in range(b - a):
-            self.forward(a + k, first, buffer, 2 + k)
+        with self._timed("forward", b - a):
+            for k in range(b - a):
+                self.forward(a + k, first, buffer, 2 + k)
         self.steps += b - a
 
     def store(self, k):
@@ -730,12 +756,13 @@ class _Reconstruction:
         m0 = sim.N - e
         if not s:
             self.seed = self._valid_norm(self.triplet(m0)[2])
-        eng.gradient(self.grads, *self.triplet(m0), self.adjoint_level(m0, m0), self.mf)
-        for j in range(e - s - 1):
-            m = m0 + j
-            self.reverse(m, e - 2 - j, e - s - 2 - j, buffer)
-            p = self.adjoint_level(m + 1, m0 + 1 + j)
-            eng.gradient(self.grads, *self.triplet(m + 1), p, self.mf)
+        with self._timed("backward", e - s):
+            eng.gradient(self.grads, *self.triplet(m0), self.adjoint_level(m0, m0), self.mf)
+            for j in range(e - s - 1):
+                m = m0 + j
+                self.reverse(m, e - 2 - j, e - s - 2 - j, buffer)
+                p = self.adjoint_level(m + 1, m0 + 1 + j)
+                eng.gradient(self.grads, *self.triplet(m + 1), p, self.mf)
 
     def sweep(self):
         from .sensitivity import checkpoint_schedule

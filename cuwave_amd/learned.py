@@ -5,9 +5,11 @@ generator, not a preconditioner. The pieces:
 
 - `layered_model`: the velocity models, a layered background (3-8 layers,
   1500-2500 m/s) plus 1-6 elliptical inclusions, seeded.
-- `UNet`: three levels of 3x3 convolutions, 32/64/128 channels, about 0.47 M
-  parameters. It runs in bfloat16 with float32 accumulation under torch
-  autocast on the GPU, where the convolutions go to the ROCm library.
+- `UNet` (`unet.py`): three levels of 3x3 convolutions, 32/64/128 channels,
+  about 0.47 M parameters, NHWC bfloat16 activations; every 3x3 convolution
+  (forward and both gradients) runs in the library's hand-written MFMA
+  kernels with float32 accumulation, and training steps with its AdamW
+  kernel.
 - `normalised_gradient`: the misfit gradient divided by the illumination map
   (`wave.illumination`), floored.
 - `training_sample` / `train`: inputs (the normalised gradient and the
@@ -26,9 +28,9 @@ from typing import Sequence
 
 import numpy as np
 import torch
-from torch import nn
 
 from . import utils
+from .unet import AdamW, UNet
 from .wave import Simulation, Source, illumination
 
 
@@ -57,43 +59,6 @@ def layered_model(seed: int, shape: tuple[int, int], vmin: float = 1500.0,
         w = -x * np.sin(ang) + y * np.cos(ang)
         v[(u / ax[0]) ** 2 + (w / ax[1]) ** 2 <= 1.0] = c
     return v
-
-
-# ------------------------------------------------------------------- U-Net
-class _Block(nn.Sequential):
-    def __init__(self, cin: int, cout: int):
-        super().__init__(nn.Conv2d(cin, cout, 3, padding=1), nn.ReLU(inplace=True),
-                         nn.Conv2d(cout, cout, 3, padding=1), nn.ReLU(inplace=True))
-
-
-class UNet(nn.Module):
-    """Three-level U-Net: (B, cin, H, W) -> (B, cout, H, W); H and W divisible by 4."""
-
-    def __init__(self, cin: int = 2, cout: int = 1, base: int = 32):
-        super().__init__()
-        c1, c2, c3 = base, 2 * base, 4 * base
-        self.enc1 = _Block(cin, c1)
-        self.enc2 = _Block(c1, c2)
-        self.mid = _Block(c2, c3)
-        self.up2 = nn.ConvTranspose2d(c3, c2, 2, stride=2)
-        self.dec2 = _Block(2 * c2, c2)
-        self.up1 = nn.ConvTranspose2d(c2, c1, 2, stride=2)
-        self.dec1 = _Block(2 * c1, c1)
-        self.out = nn.Conv2d(c1, cout, 1)
-        self.pool = nn.AvgPool2d(2)
-
-    def forward(self, x: torch.Tensor) -> torch.Tensor:
-        e1 = self.enc1(x)
-        e2 = self.enc2(self.pool(e1))
-        m = self.mid(self.pool(e2))
-        d2 = self.dec2(torch.cat([self.up2(m), e2], 1))
-        d1 = self.dec1(torch.cat([self.up1(d2), e1], 1))
-        return self.out(d1)
-
-
-def _autocast(device: torch.device):
-    kind = "cuda" if device.type == "cuda" else "cpu"
-    return torch.autocast(kind, dtype=torch.bfloat16)
 
 
 # --------------------------------------------------------------- features
@@ -126,14 +91,18 @@ def features(gradient: torch.Tensor, illum: torch.Tensor, floor: float = 1e-3) -
     return torch.stack([g, il]).float()
 
 
+def _nhwc(x: torch.Tensor) -> torch.Tensor:
+    """(B, C, H, W) -> (B, H, W, C)."""
+    return x.permute(0, 2, 3, 1).contiguous()
+
+
 def direction(net: UNet, gradient: torch.Tensor, illum: torch.Tensor,
               floor: float = 1e-3) -> torch.Tensor:
     """The network's model step for a gradient, on the gradient's grid and device."""
     x, (h, w) = _fit(features(gradient, illum, floor)[None])
-    net = net.to(x.device)
-    with torch.no_grad(), _autocast(x.device):
-        y = net(x)
-    return y[0, 0, :h, :w].to(gradient.dtype)
+    with torch.no_grad():
+        y = net(_nhwc(x))
+    return y[0, :h, :w, 0].to(gradient.dtype)
 
 
 # ---------------------------------------------------------------- training
@@ -151,22 +120,26 @@ def training_sample(sim: Simulation, sources: Sequence[Source], sensors: utils.S
 
 
 def train(net: UNet, samples, epochs: int = 50, lr: float = 1e-3,
-          weight_decay: float = 1e-4) -> list[float]:
-    """Fit the network to (input, target) samples with AdamW; returns the loss per epoch."""
-    device = samples[0][0].device
-    net = net.to(device)
-    opt = torch.optim.AdamW(net.parameters(), lr=lr, weight_decay=weight_decay)
-    xs = torch.stack([_fit(x)[0] for x, _ in samples])
-    ys = torch.stack([_fit(y)[0] for _, y in samples])
+          weight_decay: float = 1e-4, batch: int = 16, seed: int = 0) -> list[float]:
+    """Fit the network to (input, target) samples: MSE, the library's AdamW, shuffled
+    minibatches of `batch`; returns the mean loss per epoch."""
+    xs = _nhwc(torch.stack([_fit(x)[0] for x, _ in samples]))
+    ys = _nhwc(torch.stack([_fit(y)[0] for _, y in samples]))
+    net = net.to(xs.device)
+    opt = AdamW(net.parameters(), lr=lr, weight_decay=weight_decay)
+    order = torch.Generator().manual_seed(seed)
     losses = []
     for _ in range(epochs):
-        opt.zero_grad(set_to_none=True)
-        with _autocast(device):
-            pred = net(xs)
-        loss = torch.mean((pred.float() - ys) ** 2)
-        loss.backward()
-        opt.step()
-        losses.append(loss.item())
+        perm = torch.randperm(len(xs), generator=order)
+        total = 0.0
+        for i in range(0, len(xs), batch):
+            idx = perm[i:i + batch].to(xs.device)
+            opt.zero_grad()
+            loss = torch.mean((net(xs[idx]) - ys[idx]) ** 2)
+            loss.backward()
+            opt.step()
+            total += loss.item() * len(idx)
+        losses.append(total / len(xs))
     return losses
 
 

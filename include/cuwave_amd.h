@@ -195,6 +195,15 @@ const char *cw_last_error(void);
  * code's (cuwave/wave.py:929-936, cuwave/sensitivity.py:670-689). */
 int cw_set_side_stream(int32_t enabled);
 
+/* The fewest blocks the stream box of a 2D sweep must hold for the stream
+ * kernels to march it (default 512, two a CU); a smaller box, as on small
+ * grids or with few lanes, leaves every node to the general kernels, one
+ * independent thread a node (the same bits either way). Process-wide;
+ * returns the previous setting; 0 marches every box. No reference
+ * counterpart: the reference picks its tiers from device attributes
+ * (cuwave/scalar.py:422-737). */
+int cw_set_stream_threshold(int32_t blocks);
+
 /* the tiles a step kernel walks, for bucketing points */
 int cw_tile_count(const cw_grid *grid, int32_t *tiles);
 
@@ -338,6 +347,55 @@ int cw_elastic_step(const cw_elastic_step_args *args, void *stream);
 int cw_elastic_gradient(const cw_elastic_gradient_args *args, void *stream);
 int cw_elastic_points(const cw_elastic_grid *grid, const cw_elastic_point_args *args,
                       void *stream);
+
+/* ------------------------------------------------------------ U-Net
+ * The learned preconditioner's kernels (no reference counterpart: the
+ * reference has no learned preconditioner). 3x3 convolutions, stride 1, zero
+ * padding 1, as implicit GEMMs on the bfloat16 matrix cores with float32
+ * accumulation. Activations are NHWC bfloat16; weights (cout, 9 cin), row
+ * k = tap cin + ci, tap = 3 (dh + 1) + (dw + 1). */
+typedef struct cw_conv3x3_args {
+  int32_t n, h, w;             /* batch, height, width */
+  int32_t cin, cout;           /* cin a multiple of 16; cout 32, 64 or 128 */
+  const void *x;               /* (n, h, w, cin) bfloat16 */
+  const void *weight;          /* (cout, 9 cin) bfloat16 */
+  const void *weight_flipped;  /* (cin, 9 cout) bfloat16: weight flipped in space,
+                                  transposed in channels (cw_conv3x3_weights) */
+  const void *bias;            /* (cout) float32, or NULL */
+  int32_t relu;                /* 1: max(0, .) after the bias */
+  void *y;                     /* (n, h, w, cout) bfloat16 */
+  const void *dy;              /* (n, h, w, cout) bfloat16: the gradient at y
+                                  before the activation */
+  void *dx;                    /* (n, h, w, cin) bfloat16 */
+  void *dweight;               /* (cout, 9 cin) float32 */
+  void *workspace;             /* (splits, 9 cin, cout) float32 partial sums */
+  int32_t splits;              /* pixel runs the weight gradient sums apart */
+} cw_conv3x3_args;
+
+/* y = act(bias + conv(x, weight)) */
+int cw_conv3x3_bf16_fwd(const cw_conv3x3_args *args, void *stream);
+/* dx = conv(dy, weight_flipped): the gradient at x (cin 32, 64 or 128) */
+int cw_conv3x3_bf16_bwd_data(const cw_conv3x3_args *args, void *stream);
+/* dweight = sum over pixels of dy (x) x shifted by each tap; deterministic:
+ * each of `splits` runs of pixels sums into its own workspace slice, then
+ * the slices add in order */
+int cw_conv3x3_bf16_bwd_weight(const cw_conv3x3_args *args, void *stream);
+/* the kernels' bfloat16 weights from a float32 (cout, 9 cin) master: wt as
+ * is (round to nearest even), wd flipped and transposed (NULL to skip) */
+int cw_conv3x3_weights(int32_t cout, int32_t cin, const void *w, void *wt, void *wd,
+                       void *stream);
+
+/* one AdamW step over `count` float32 parameters (decoupled weight decay,
+ * bias-corrected moments; the update torch.optim.AdamW makes) */
+typedef struct cw_adamw_args {
+  int64_t count;
+  void *param;
+  const void *grad;
+  void *m, *v;      /* first and second moments, zeroed before step 1 */
+  float lr, beta1, beta2, eps, weight_decay;
+  int32_t step;     /* 1 for the first step */
+} cw_adamw_args;
+int cw_adamw(const cw_adamw_args *args, void *stream);
 
 #ifdef __cplusplus
 }

@@ -9,6 +9,9 @@ outside the core rectangle) and compare with the oracle on the same inputs:
 
 - five shots (eight lanes: blocks 32 columns wide) on a 134 x 200 grid;
 - one shot (one lane: blocks 256 columns wide) on a 100 x 560 grid.
+
+Boxes this small hold fewer blocks than the library marches by default (it
+leaves them to the general kernels), so the tests lower that threshold to 0.
 """
 
 from __future__ import annotations
@@ -26,6 +29,18 @@ S = importlib.import_module("cuwave_amd.sensitivity")
 from cuwave_amd import utils, wave  # noqa: E402
 
 F32_GRAD, F64 = 1e-4, 1e-10
+
+
+@pytest.fixture(autouse=True)
+def _march_every_box():
+    # these grids are sized to hold core blocks; by default a box this small is
+    # left to the general kernels, so have the stream kernels march it
+    from cuwave_amd import _abi
+
+    lib = _abi.load()
+    previous = lib.cw_set_stream_threshold(0)
+    yield
+    lib.cw_set_stream_threshold(previous)
 
 
 def _case(Nx, precision, N, seed):

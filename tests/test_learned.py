@@ -1,37 +1,53 @@
-"""The learned-FWI pieces that run without a GPU: the U-Net, its training, the features.
+"""The learned-FWI pieces that run without a GPU: the network's shape, the features,
+the models.
 
-The network and its optimiser are torch's own (the CPU run is the library
-oracle); these tests pin the shapes, the parameter count of the 3-level
-32/64/128 network, that training fits a target, and the feature scaling.
+The network's forward and backward run on the GPU's matrix cores only
+(tests/test_gpu_unet.py checks them against the oracle network); here the
+product's parameters go through the oracle network (`oracle/unet.py`, torch's
+CPU library) to pin the shapes and the parameter count of the 3-level
+32/64/128 U-Net, and the product refuses to run on the CPU.
 """
 
 from __future__ import annotations
 
 import numpy as np
+import pytest
 import torch
 
-from cuwave_amd import learned
+from cuwave_amd import learned, unet
+from oracle import unet as O
 
 
 def test_the_unet_has_the_three_level_shape():
-    net = learned.UNet()
+    net = unet.UNet()
     count = sum(p.numel() for p in net.parameters())
     assert 440_000 < count < 500_000
-    x = torch.randn(2, 2, 24, 36)
-    assert net(x).shape == (2, 1, 24, 36)
-    with torch.autocast("cpu", dtype=torch.bfloat16):
-        assert net(x).dtype == torch.bfloat16
-    net(x).square().mean().backward()
-    assert all(p.grad is not None for p in net.parameters())
+    params = {k: v.detach() for k, v in net.named_parameters()}
+    x = torch.randn(2, 24, 36, 2)
+    assert O.unet(params, x, net.cin_k).shape == (2, 24, 36, 1)
+    assert net.cin_k == 16
 
 
-def test_training_fits_a_target():
-    torch.manual_seed(0)
-    x = torch.randn(4, 2, 16, 16)
-    y = 0.5 * x[:, :1] - 0.25 * x[:, 1:]  # a linear map the network can learn
-    losses = learned.train(learned.UNet(base=8), list(zip(x, y)), epochs=60, lr=3e-3)
-    assert np.isfinite(losses).all()
-    assert losses[-1] < 0.3 * losses[0]
+def test_the_unet_runs_on_the_gpu_only():
+    with pytest.raises(RuntimeError, match="GPU"):
+        unet.UNet()(torch.randn(1, 8, 8, 2))
+
+
+def test_the_oracle_network_reads_the_weight_layouts_as_stated():
+    # a 3x3 kernel with one non-zero tap moves the image by that tap's offset
+    cin = 16
+    w = torch.zeros(1, 9 * cin)
+    tap, ci = 3 * 0 + 2, 5  # dh = -1, dw = +1
+    w[0, tap * cin + ci] = 1.0
+    x = torch.randn(1, 6, 7, cin)
+    y = O.conv3x3(x, w, None, False)[0, :, :, 0]
+    np.testing.assert_allclose(y[1:, :-1].numpy(), x[0, :-1, 1:, ci].numpy())
+    # the up-convolution: column (2 a + b) cout + co feeds output pixel (2i + a, 2j + b)
+    wu = torch.zeros(3, 4 * 2)
+    wu[1, (2 * 1 + 0) * 2 + 1] = 1.0
+    xu = torch.randn(1, 2, 2, 3)
+    yu = O.upconv(xu, wu, torch.zeros(2))
+    np.testing.assert_allclose(yu[0, 1::2, 0::2, 1].numpy(), xu[0, :, :, 1].numpy())
 
 
 def test_features_are_the_normalised_gradient_and_illumination():
