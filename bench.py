@@ -451,6 +451,95 @@ def forward_line(args) -> dict:
     }
 
 
+def learned_line(args) -> dict:
+    """Config 5: the U-Net preconditioner trained on solver data, then plain against
+    learned FWI on held-out models, timed to the plain run's final misfit.
+
+    ROI `roi`^2 behind a 32-cell PML, dx = 1 mm, order 8, float32, `shots`
+    Ricker shots (100 kHz) along the top row, `receivers` receivers every
+    roi / receivers cells; velocity 1500-2500 m/s layered models with 1-6
+    ellipses (seeds 0.. for training, 1000.. held out), the start model each
+    one's Gaussian smoothing (sigma roi / 32). Training: one sample a model
+    (the features at the start model against the step to the true one),
+    AdamW lr 1e-3, batch 16.
+    """
+    import torch
+    from scipy.ndimage import gaussian_filter
+
+    from cuwave_amd import boundary, learned, scalar, utils, wave
+    from cuwave_amd.signals import ricker
+
+    roi, shots, nt = args.roi, args.shots, args.nt
+    width = LAYER + 1
+    n = roi + 2 * LAYER + 2
+    dt = 0.4 * DX / 2500.0
+    sim = scalar.AcousticWave((n, n), (DX, DX), nt, dt, precision="float32", space_order=8,
+                              rho1=1.0, rho2=1.0, kappa1=1500.0**2, kappa2=2500.0**2,
+                              pml=boundary.Pml(width, 2500.0, 1e-6))
+
+    def indicator(v):
+        full = np.pad(v, ((width, n - width - roi), (width, sim.Nx_padded[1] - width - roi)),
+                      mode="edge")
+        return sim.indicator_of_velocity(full)
+
+    wavelet = (ricker(np.arange(nt) * dt, 1.0, FREQ) / (DX * DX))[:, None]
+    cols = np.round(np.linspace(roi / (2 * shots), roi - roi / (2 * shots), shots)).astype(int)
+    sources = [wave.Source(np.array([[width], [c + width]]), wavelet) for c in cols]
+    every = max(roi // args.receivers, 1)
+    coords = np.stack([np.full(roi // every, width * DX),
+                       (np.arange(0, roi, every) + width) * DX], axis=1)
+    sensors = utils.Sensors(sim, coords)
+    sigma = roi / 32
+
+    def pair(seed):
+        true = learned.layered_model(seed, (roi, roi))
+        return true, gaussian_filter(true, sigma)
+
+    t0 = time.perf_counter()
+    samples = [learned.training_sample(sim, sources, sensors, indicator(t), indicator(s))
+               for t, s in map(pair, range(args.train_models))]
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    torch.manual_seed(0)
+    net = learned.UNet().cuda()
+    t0 = time.perf_counter()
+    losses = learned.train(net, samples, epochs=args.epochs, lr=1e-3, batch=16)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del samples
+    held = {1000 + k: pair(1000 + k) for k in range(args.eval_models)}
+    results = learned.evaluate(sim, sources, sensors, indicator, held, net, args.iterations,
+                               step=args.fwi_step)
+    ratios = [r["ratio"] for r in results]
+    reached = [x for x in ratios if x is not None]
+    return {
+        "metric": "FWI time-to-misfit, learned-preconditioned vs plain (ratio of seconds)",
+        "value": float(np.median([x if x is not None else np.inf for x in ratios])),
+        "unit": "learned seconds / plain seconds to the plain 30-iteration misfit",
+        "n_gpus": 1, "steps": args.iterations, "warmup": 0, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 solver, bf16 network",
+        "data": "synthetic",
+        "config": {"workload": "learned-preconditioned FWI vs plain normalised-gradient FWI",
+                   "grid": f"{n}x{n} (ROI {roi}^2 + {LAYER}-cell PML)", "space_order": 8,
+                   "shots": shots, "receivers": len(coords), "time_steps": nt,
+                   "train_models": args.train_models, "eval_models": args.eval_models,
+                   "iterations": args.iterations, "epochs": args.epochs},
+        "reached": f"{len(reached)} of {len(ratios)} held-out models",
+        "per_model": [{"seed": r["seed"], "ratio": r["ratio"], "plain_s": r["plain_s"],
+                       "learned_s": r["learned_s"],
+                       "plain_misfit": r["plain"]["history"][-1][1],
+                       "learned_misfit": r["learned"]["history"][-1][1],
+                       "start_misfit": r["plain"]["history"][0][1],
+                       "plain_model_error": r["plain"]["model_error"],
+                       "learned_model_error": r["learned"]["model_error"],
+                       "plain_history": [[round(t, 3), c] for t, c in r["plain"]["history"]],
+                       "learned_history": [[round(t, 3), c] for t, c in r["learned"]["history"]]}
+                      for r in results],
+        "training": {"data_s": t_data, "train_s": t_train, "first_loss": losses[0],
+                     "last_loss": losses[-1]},
+    }
+
+
 EL_ROI, EL_LAYER, EL_DX, EL_FREQ = (1024, 2048), 32, 5e-4, 5e5
 EL_SHOTS, EL_RECEIVERS, EL_STEPS = 16, 512, 3000
 EL_RHO0, EL_VP0 = 2200.0, 3000.0
@@ -610,13 +699,31 @@ def main():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--probe-copy", action="store_true",
                    help="run only a 256 MiB device copy (counter calibration)")
-    p.add_argument("--workload", choices=("2d", "3d", "elastic", "forward"), default="2d",
+    p.add_argument("--train-models", type=int, default=256)
+    p.add_argument("--eval-models", type=int, default=16)
+    p.add_argument("--iterations", type=int, default=30)
+    p.add_argument("--epochs", type=int, default=40)
+    p.add_argument("--receivers", type=int, default=128)
+    p.add_argument("--fwi-step", type=float, default=0.2,
+                   help="the first FWI step, the largest indicator change")
+    p.add_argument("--workload", choices=("2d", "3d", "elastic", "forward", "learned"),
+                   default="2d",
                    help="3d: the 3D acoustic gradient on a 354^3 grid instead (its own line); "
                         "elastic: the 2D elastic gradient on 1058 x 2114, 16 shots; "
                         "forward: the plain 298 x 298 forward, one shot")
     args = p.parse_args()
     if args.probe_copy:
         probe_copy()
+        return
+    if args.workload == "learned":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU"}))
+            sys.exit(2)
+        # config 5's sizes where the 2D line's defaults were left
+        args.roi = 512 if args.roi == ROI else args.roi
+        args.shots = 24 if args.shots == SHOTS else args.shots
+        args.nt = 2000 if args.nt == STEPS else args.nt
+        print(json.dumps(learned_line(args)))
         return
     if args.workload == "forward":
         if args.gpus != 1:

@@ -92,6 +92,17 @@ __host__ __device__ constexpr int lanes_per_thread(int lanes) {
   return lanes < (int)(16 / sizeof(T)) ? lanes : (int)(16 / sizeof(T));
 }
 
+// the lanes a thread of a scalar general kernel holds: CW_GENERAL_BYTES'
+// worth (one 16-byte access by default) or the node's lane count if smaller
+#ifndef CW_GENERAL_BYTES
+#define CW_GENERAL_BYTES 16
+#endif
+template <typename T>
+__host__ __device__ constexpr int general_lanes(int lanes) {
+  return lanes < (int)(CW_GENERAL_BYTES / sizeof(T)) ? lanes
+                                                      : (int)(CW_GENERAL_BYTES / sizeof(T));
+}
+
 // the value type of V lanes: T itself for one
 template <typename T, int V> struct LaneType { using type = Lanes<T, V>; };
 template <typename T> struct LaneType<T, 1> { using type = T; };

@@ -542,9 +542,13 @@ __global__ __launch_bounds__(kThreads) void forward_step(const FwdParams<T> p,
 // then just fit four waves a SIMD (the default packing left it one register
 // over, at three), which measured 18 % faster; every other build keeps the
 // default, as a tighter budget would make it spill.
+#ifndef CW_FRAME_BLOCKS_2D
+#define CW_FRAME_BLOCKS_2D 1
+#endif
 template <typename T, int NDIM, int R, int W, int V, bool STIFF>
 __global__ __launch_bounds__(kThreads,
-                             NDIM == 3 && V == 1 && sizeof(T) == 4 ? 4 : 1) void backward_step(
+                             NDIM == 3 && V == 1 && sizeof(T) == 4 ? 4
+                             : NDIM == 2 && sizeof(T) == 4 ? CW_FRAME_BLOCKS_2D : 1) void backward_step(
     const BwdParams<T> p,
                                                           const Box box,
                                                           const int n,

@@ -84,10 +84,17 @@ def _fit(x: torch.Tensor, mult: int = 4):
 
 
 def features(gradient: torch.Tensor, illum: torch.Tensor, floor: float = 1e-3) -> torch.Tensor:
-    """The (2, H, W) network input: the normalised gradient and the illumination, each
-    scaled to a unit maximum."""
-    g = _scaled(normalised_gradient(gradient, illum, floor))
-    il = _scaled(torch.as_tensor(illum, device=g.device, dtype=g.dtype))
+    """The (2, H, W) network input: the normalised gradient over its 99th percentile of
+    magnitude, clipped to +-3 (spikes at the sources would otherwise set the scale),
+    and the illumination on a log scale from its floor to its maximum, in [0, 1]."""
+    g = normalised_gradient(gradient, illum, floor)
+    scale = torch.quantile(g.abs().flatten().float(), 0.99).to(g.dtype)
+    g = torch.clamp(g / scale, -3.0, 3.0) if scale > 0 else g
+    il = torch.as_tensor(illum, device=g.device, dtype=g.dtype)
+    top = il.max()
+    if top > 0:
+        lo, hi = np.log(floor), np.log(1.0 + floor)
+        il = (torch.log(il / top + floor) - lo) / (hi - lo)
     return torch.stack([g, il]).float()
 
 
@@ -120,11 +127,15 @@ def training_sample(sim: Simulation, sources: Sequence[Source], sensors: utils.S
 
 
 def train(net: UNet, samples, epochs: int = 50, lr: float = 1e-3,
-          weight_decay: float = 1e-4, batch: int = 16, seed: int = 0) -> list[float]:
+          weight_decay: float = 1e-4, batch: int = 16, seed: int = 0,
+          flip: bool = True) -> list[float]:
     """Fit the network to (input, target) samples: MSE, the library's AdamW, shuffled
-    minibatches of `batch`; returns the mean loss per epoch."""
+    minibatches of `batch`, each sample also mirrored left to right (`flip`); returns
+    the mean loss per epoch."""
     xs = _nhwc(torch.stack([_fit(x)[0] for x, _ in samples]))
     ys = _nhwc(torch.stack([_fit(y)[0] for _, y in samples]))
+    if flip:  # the acquisition is symmetric under a left-right flip: twice the data
+        xs, ys = torch.cat([xs, xs.flip(2)]), torch.cat([ys, ys.flip(2)])
     net = net.to(xs.device)
     opt = AdamW(net.parameters(), lr=lr, weight_decay=weight_decay)
     order = torch.Generator().manual_seed(seed)
@@ -146,42 +157,85 @@ def train(net: UNet, samples, epochs: int = 50, lr: float = 1e-3,
 # --------------------------------------------------------------------- FWI
 def fwi(sim: Simulation, sources: Sequence[Source], sensors: utils.Sensors, observed,
         start_indicator, iterations: int, step: float, net: UNet | None = None,
-        batch: int | None = None, halvings: int = 4, floor: float = 1e-3):
-    """Plain (net None: Adam on the indicator) or preconditioned FWI.
+        batch: int | None = None, halvings: int = 3, floor: float = 1e-3, lead: int = 1):
+    """Illumination-normalised gradient FWI, or the same led by the network.
 
-    The preconditioned iteration moves along the network's direction, scaled so
-    its largest change is `step`, and accepts the step only when the exact
-    misfit decreases, halving it up to `halvings` times. Returns the final
-    indicator and the (seconds, misfit) history, the first entry the start.
+    Each iteration takes the misfit gradient and the illumination at the current
+    model. The direction is the negative normalised gradient (`net` None), or,
+    for the first `lead` iterations, the network's model step with the
+    normalised gradient as the fallback: the network learned the step from the
+    start model, which later iterates leave behind (its later directions
+    measured as tiny accepted steps), so after the lead, or once no step along
+    its direction lowers the misfit, the run goes on with the normalised
+    gradient alone, from the first step size. A direction is scaled so its
+    largest change is the current step and tried at it and `halvings` halvings;
+    the first trial that lowers the exact misfit is accepted and the step grows
+    by half, else the next direction (or none) is tried and the step halves.
+    Returns the final indicator and the (seconds, misfit) history, the first
+    entry the start model.
     """
-    gamma = torch.tensor(np.asarray(start_indicator, dtype=np.float64))
+    gamma = np.asarray(start_indicator, dtype=np.float64)
     t0 = time.perf_counter()
     history = []
-    adam = None
-    if net is None:
-        gamma.requires_grad_(True)
-        adam = torch.optim.Adam([gamma], lr=step)
-    for _ in range(iterations):
-        current = gamma.detach().numpy().astype(sim.np_dtype)
+    first = step
+    for it in range(iterations):
+        if it == lead and net is not None:
+            net, step = None, first  # the gradient starts from the first step's size
+        current = gamma.astype(sim.np_dtype)
         cost, gradient = utils.misfit_gradient(sim, sources, current, sensors, observed,
                                                batch=batch)
         history.append((time.perf_counter() - t0, cost))
-        if net is None:
-            adam.zero_grad(set_to_none=True)
-            gamma.grad = gradient.detach().cpu().double()
-            adam.step()
-            continue
         illum = illumination(sim, sources, current, batch=batch)
-        d = direction(net, gradient, illum, floor).detach().cpu().double()
-        d = d / max(float(d.abs().max()), 1e-30) * step
-        for _ in range(halvings + 1):
-            trial = (gamma + d).numpy().astype(sim.np_dtype)
-            trial_cost = utils.misfit(sim, sources, trial, sensors, observed, batch=batch)
-            if trial_cost < cost:
-                gamma = torch.tensor(trial.astype(np.float64))
+        plain = -normalised_gradient(gradient, illum, floor)
+        directions = [plain] if net is None else [direction(net, gradient, illum, floor), plain]
+        accepted = False
+        for which, d in enumerate(directions):
+            d = d.detach().cpu().double().numpy()
+            d = d / max(float(np.abs(d).max()), 1e-30)
+            for k in range(halvings + 1):
+                trial = (gamma + d * step / 2**k).astype(sim.np_dtype)
+                if utils.misfit(sim, sources, trial, sensors, observed, batch=batch) < cost:
+                    gamma = trial.astype(np.float64)
+                    step = step / 2**k * 1.5
+                    accepted = True
+                    break
+            if accepted:
                 break
-            d = d / 2
-    final = gamma.detach().numpy().astype(sim.np_dtype)
+            if net is not None and which == 0:
+                net = None  # the network's direction failed: plain from here on
+        if not accepted:
+            step /= 2.0
+    final = gamma.astype(sim.np_dtype)
     history.append((time.perf_counter() - t0,
                     utils.misfit(sim, sources, final, sensors, observed, batch=batch)))
     return final, history
+
+
+def time_to_misfit(plain: list, learned: list) -> dict:
+    """The learned run's seconds to reach the plain run's final misfit, against the
+    plain run's own seconds to it (its whole run)."""
+    target, t_plain = plain[-1][1], plain[-1][0]
+    reached = next((t for t, c in learned if c <= target), None)
+    return {"target": target, "plain_s": t_plain, "learned_s": reached,
+            "ratio": None if reached is None else reached / t_plain}
+
+
+def evaluate(sim: Simulation, sources, sensors: utils.Sensors, indicator_of, models: dict,
+             net: UNet, iterations: int, step: float, batch: int | None = None) -> list[dict]:
+    """Plain against learned FWI on each held-out (true, start) velocity pair of
+    `models`: the histories, the time-to-misfit and the final model errors."""
+    out = []
+    for seed, (true, start) in models.items():
+        t_ind, s_ind = indicator_of(true), indicator_of(start)
+        observed = utils.measure(sim, sources, t_ind, sensors, batch=batch)
+        runs = {}
+        for name, n in (("plain", None), ("learned", net)):
+            final, history = fwi(sim, sources, sensors, observed, s_ind, iterations, step,
+                                 net=n, batch=batch)
+            err = float(np.linalg.norm(final - t_ind) / np.linalg.norm(s_ind - t_ind))
+            runs[name] = {"history": history, "model_error": err}
+        out.append({"seed": seed, **runs, **time_to_misfit(runs["plain"]["history"],
+                                                           runs["learned"]["history"])})
+    return out
+
+

@@ -311,6 +311,71 @@ def illumination(sim: Simulation, sources, indicator, batch: int | None = None) 
     return illum
 
 
+def stable_timestep(sim: Simulation, indicator, iterations: int = 60,
+                    safety: float = 0.95) -> float:
+    """The largest stable timestep for `sim` under `indicator`, measured on the step itself.
+
+    The leapfrog is stable while the spectral radius of dt^2 minv L stays under 4,
+    and L is symmetric with minv diagonal, so a power iteration through the step
+    kernels (u0 = 0, u1 = v: 2 v - u2 = -dt^2 minv L v) converges to that radius
+    for any order, material and wall layout (cuwave/wave.py:56-116). The PML is
+    left out: its memory would make the iteration nonlinear. Returns `safety`
+    times the bound; a wide stencil over a strong contrast can put it well below
+    `stable_dt`, which knows only the speed and the spacing.
+    """
+    import dataclasses
+
+    sim = dataclasses.replace(sim, pml=None)
+    field = np.random.default_rng(0).standard_normal((sim.ncomp, *sim.Nx_padded))
+    mask = np.zeros(field.shape, dtype=bool)
+    for c in range(sim.ncomp):
+        # a staggered component has no unknown on its own axis' last node
+        stag = getattr(sim, "radius", None) is not None and sim.ncomp > 1
+        inner = tuple(slice(1, n - 1 - (stag and d == c)) for d, n in enumerate(sim.Nx))
+        mask[(c, *inner)] = True
+    if sim.ncomp > 1 or type(sim).__name__ == "ElasticWave":
+        from .elastic import ElasticEngine
+
+        mat = sim.build_materials(indicator)
+        eng = ElasticEngine(sim, mat, 1)
+        pair = eng.pair()
+        slots = eng.levels(3)
+
+        def apply(v):
+            slots[0].zero_()
+            slots[1].copy_(v)
+            pair.step(slots[0], slots[1], slots[2])
+            return slots[2]
+
+        shape = (*sim.field_shape, 1)
+    else:
+        from .engine import Propagator
+
+        mat = sim.build_materials(indicator)
+        prop = Propagator(sim, mat, 1)
+        slots = prop.levels_memory(3)
+
+        def apply(v):
+            # level -2 (slot 0) zero, level -1 (slot 1) v, step 0 into slot 2
+            slots[0].zero_()
+            slots[1].copy_(v)
+            prop.forward(slots, 0, 1, offset=2)
+            return slots[2]
+
+        shape = (*sim.Nx_padded, 1)
+    keep = torch.as_tensor(mask.reshape(shape), device=slots.device)
+    v = torch.as_tensor(field.reshape(shape), dtype=sim.dtype, device=slots.device) * keep
+    value = 0.0
+    for _ in range(iterations):
+        v = v / torch.linalg.vector_norm(v)
+        applied = (2.0 * v - apply(v)) * keep
+        value = float(torch.sum(v * applied))
+        v = applied
+    if value <= 0.0:
+        raise ValueError(f"the operator came back non-positive: {value}")
+    return safety * sim.dt * float(np.sqrt(4.0 / value))
+
+
 def simulate(sim: Simulation, source: Source, indicator, sensors=None, record_every=None):
     """Run `sim` forward under `source` and material `indicator` (cuwave/wave.py:1071-1163).
 
