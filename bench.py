@@ -495,27 +495,42 @@ def learned_line(args) -> dict:
         true = learned.layered_model(seed, (roi, roi))
         return true, gaussian_filter(true, sigma)
 
-    t0 = time.perf_counter()
-    samples = [learned.training_sample(sim, sources, sensors, indicator(t), indicator(s))
-               for t, s in map(pair, range(args.train_models))]
+    def note(text):
+        print(f"[learned {time.perf_counter() - start:7.1f}s] {text}", file=sys.stderr,
+              flush=True)
+
+    start = t0 = time.perf_counter()
+    samples = []
+    for seed in range(args.train_models):
+        t, s = pair(seed)
+        samples.append(learned.training_sample(sim, sources, sensors, indicator(t),
+                                               indicator(s)))
+        if (seed + 1) % 16 == 0:
+            note(f"{seed + 1} of {args.train_models} training samples")
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
     torch.manual_seed(0)
     net = learned.UNet().cuda()
     t0 = time.perf_counter()
-    losses = learned.train(net, samples, epochs=args.epochs, lr=1e-3, batch=16)
+    every = max(args.epochs // 10, 1)
+    losses = learned.train(net, samples, epochs=args.epochs, lr=1e-3, batch=16,
+                           progress=lambda e, loss: e % every or note(f"epoch {e} loss {loss:.4g}"))
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     del samples
     held = {1000 + k: pair(1000 + k) for k in range(args.eval_models)}
-    results = learned.evaluate(sim, sources, sensors, indicator, held, net, args.iterations,
-                               step=args.fwi_step)
+    results = learned.evaluate(
+        sim, sources, sensors, indicator, held, net, args.iterations, step=args.fwi_step,
+        progress=lambda r: note(f"model {r['seed']}: ratio {r['ratio']}, plain "
+                                f"{r['plain']['history'][-1][1]:.4g}, learned "
+                                f"{r['learned']['history'][-1][1]:.4g}"))
     ratios = [r["ratio"] for r in results]
     reached = [x for x in ratios if x is not None]
     return {
         "metric": "FWI time-to-misfit, learned-preconditioned vs plain (ratio of seconds)",
         "value": float(np.median([x if x is not None else np.inf for x in ratios])),
-        "unit": "learned seconds / plain seconds to the plain 30-iteration misfit",
+        "unit": f"learned seconds / plain seconds to the plain {args.iterations}-iteration "
+                "misfit",
         "n_gpus": 1, "steps": args.iterations, "warmup": 0, "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 solver, bf16 network",
         "data": "synthetic",

@@ -15,10 +15,10 @@ generator, not a preconditioner. The pieces:
 - `training_sample` / `train`: inputs (the normalised gradient and the
   normalised illumination at the start model) against the target (the
   indicator step from the start model to the true one), MSE, AdamW.
-- `fwi`: the plain baseline (Adam on the indicator, as the reference's
-  example FWI runs it) and the preconditioned iteration. The preconditioned
-  iteration accepts a step only when the exact solver's misfit decreases,
-  halving it otherwise. Both record (seconds, misfit) for time-to-misfit.
+- `fwi`: the plain baseline (the illumination-normalised gradient) and the
+  network-led iteration (the network's step first, then the same baseline),
+  both a line search that accepts a step only when the exact solver's misfit
+  decreases. Both record (seconds, misfit) for time-to-misfit.
 """
 
 from __future__ import annotations
@@ -31,7 +31,7 @@ import torch
 
 from . import utils
 from .unet import AdamW, UNet
-from .wave import Simulation, Source, illumination
+from .wave import Simulation, Source
 
 
 # ------------------------------------------------------------------ models
@@ -113,14 +113,18 @@ def direction(net: UNet, gradient: torch.Tensor, illum: torch.Tensor,
 
 
 # ---------------------------------------------------------------- training
+def _accumulator(sim: Simulation) -> torch.Tensor:
+    return torch.zeros(tuple(sim.Nx_padded), dtype=sim.dtype, device=torch.device("cuda"))
+
+
 def training_sample(sim: Simulation, sources: Sequence[Source], sensors: utils.Sensors,
                     true_indicator, start_indicator, batch: int | None = None):
     """(input, target) for one model: the features at the start model against the
     indicator step to the true one (unit-max scaled)."""
     observed = utils.measure(sim, sources, true_indicator, sensors, batch=batch)
+    illum = _accumulator(sim)
     _, gradient = utils.misfit_gradient(sim, sources, start_indicator, sensors, observed,
-                                        batch=batch)
-    illum = illumination(sim, sources, start_indicator, batch=batch)
+                                        batch=batch, illum=illum)
     step = torch.as_tensor(np.asarray(true_indicator) - np.asarray(start_indicator),
                            device=gradient.device, dtype=torch.float32)
     return features(gradient, illum), _scaled(step)[None]
@@ -128,10 +132,10 @@ def training_sample(sim: Simulation, sources: Sequence[Source], sensors: utils.S
 
 def train(net: UNet, samples, epochs: int = 50, lr: float = 1e-3,
           weight_decay: float = 1e-4, batch: int = 16, seed: int = 0,
-          flip: bool = True) -> list[float]:
+          flip: bool = True, progress=None) -> list[float]:
     """Fit the network to (input, target) samples: MSE, the library's AdamW, shuffled
     minibatches of `batch`, each sample also mirrored left to right (`flip`); returns
-    the mean loss per epoch."""
+    the mean loss per epoch (each also handed to `progress(epoch, loss)` if given)."""
     xs = _nhwc(torch.stack([_fit(x)[0] for x, _ in samples]))
     ys = _nhwc(torch.stack([_fit(y)[0] for _, y in samples]))
     if flip:  # the acquisition is symmetric under a left-right flip: twice the data
@@ -151,6 +155,8 @@ def train(net: UNet, samples, epochs: int = 50, lr: float = 1e-3,
             opt.step()
             total += loss.item() * len(idx)
         losses.append(total / len(xs))
+        if progress is not None:
+            progress(len(losses), losses[-1])
     return losses
 
 
@@ -160,7 +166,7 @@ def fwi(sim: Simulation, sources: Sequence[Source], sensors: utils.Sensors, obse
         batch: int | None = None, halvings: int = 3, floor: float = 1e-3, lead: int = 1):
     """Illumination-normalised gradient FWI, or the same led by the network.
 
-    Each iteration takes the misfit gradient and the illumination at the current
+    The misfit gradient and the illumination come from one sweep at the current
     model. The direction is the negative normalised gradient (`net` None), or,
     for the first `lead` iterations, the network's model step with the
     normalised gradient as the fallback: the network learned the step from the
@@ -169,23 +175,23 @@ def fwi(sim: Simulation, sources: Sequence[Source], sensors: utils.Sensors, obse
     its direction lowers the misfit, the run goes on with the normalised
     gradient alone, from the first step size. A direction is scaled so its
     largest change is the current step and tried at it and `halvings` halvings;
-    the first trial that lowers the exact misfit is accepted and the step grows
-    by half, else the next direction (or none) is tried and the step halves.
-    Returns the final indicator and the (seconds, misfit) history, the first
-    entry the start model.
+    each trial is the forward half of the gradient sweep at the trial model,
+    whose adjoint half runs only if the misfit came out lower. The first such
+    trial is accepted (its sweep has the next gradient) and the step grows by
+    half, else the next direction (or none) is tried and the step halves.
+    Returns the final indicator and the (seconds, misfit) history: the start
+    model, then the model after each iteration.
     """
     gamma = np.asarray(start_indicator, dtype=np.float64)
     t0 = time.perf_counter()
-    history = []
+    illum = _accumulator(sim)
+    cost, gradient = utils.misfit_gradient(sim, sources, gamma.astype(sim.np_dtype), sensors,
+                                           observed, batch=batch, illum=illum)
+    history = [(time.perf_counter() - t0, cost)]
     first = step
     for it in range(iterations):
         if it == lead and net is not None:
             net, step = None, first  # the gradient starts from the first step's size
-        current = gamma.astype(sim.np_dtype)
-        cost, gradient = utils.misfit_gradient(sim, sources, current, sensors, observed,
-                                               batch=batch)
-        history.append((time.perf_counter() - t0, cost))
-        illum = illumination(sim, sources, current, batch=batch)
         plain = -normalised_gradient(gradient, illum, floor)
         directions = [plain] if net is None else [direction(net, gradient, illum, floor), plain]
         accepted = False
@@ -194,8 +200,11 @@ def fwi(sim: Simulation, sources: Sequence[Source], sensors: utils.Sensors, obse
             d = d / max(float(np.abs(d).max()), 1e-30)
             for k in range(halvings + 1):
                 trial = (gamma + d * step / 2**k).astype(sim.np_dtype)
-                if utils.misfit(sim, sources, trial, sensors, observed, batch=batch) < cost:
-                    gamma = trial.astype(np.float64)
+                il = _accumulator(sim)
+                c, g = utils.misfit_gradient(sim, sources, trial, sensors, observed,
+                                             batch=batch, illum=il, below=cost)
+                if g is not None:
+                    gamma, cost, gradient, illum = trial.astype(np.float64), c, g, il
                     step = step / 2**k * 1.5
                     accepted = True
                     break
@@ -205,10 +214,8 @@ def fwi(sim: Simulation, sources: Sequence[Source], sensors: utils.Sensors, obse
                 net = None  # the network's direction failed: plain from here on
         if not accepted:
             step /= 2.0
-    final = gamma.astype(sim.np_dtype)
-    history.append((time.perf_counter() - t0,
-                    utils.misfit(sim, sources, final, sensors, observed, batch=batch)))
-    return final, history
+        history.append((time.perf_counter() - t0, cost))
+    return gamma.astype(sim.np_dtype), history
 
 
 def time_to_misfit(plain: list, learned: list) -> dict:
@@ -221,9 +228,11 @@ def time_to_misfit(plain: list, learned: list) -> dict:
 
 
 def evaluate(sim: Simulation, sources, sensors: utils.Sensors, indicator_of, models: dict,
-             net: UNet, iterations: int, step: float, batch: int | None = None) -> list[dict]:
+             net: UNet, iterations: int, step: float, batch: int | None = None,
+             progress=None) -> list[dict]:
     """Plain against learned FWI on each held-out (true, start) velocity pair of
-    `models`: the histories, the time-to-misfit and the final model errors."""
+    `models`: the histories, the time-to-misfit and the final model errors (each
+    model's result also handed to `progress(result)` if given)."""
     out = []
     for seed, (true, start) in models.items():
         t_ind, s_ind = indicator_of(true), indicator_of(start)
@@ -236,6 +245,8 @@ def evaluate(sim: Simulation, sources, sensors: utils.Sensors, indicator_of, mod
             runs[name] = {"history": history, "model_error": err}
         out.append({"seed": seed, **runs, **time_to_misfit(runs["plain"]["history"],
                                                            runs["learned"]["history"])})
+        if progress is not None:
+            progress(out[-1])
     return out
 
 

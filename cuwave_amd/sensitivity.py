@@ -224,8 +224,10 @@ class _Reconstruction:
     in the same kernel.
     """
 
-    def __init__(self, sim, mat, sources, sensors, strip_dtype=None, checkpoints=None):
+    def __init__(self, sim, mat, sources, sensors, strip_dtype=None, checkpoints=None,
+                 illum=None):
         self.sim, self.mat = sim, mat
+        self.illum = illum  # a per-node accumulator the first forward pass adds u^2 into
         self.prop = prop = Propagator(sim, mat, len(sources))
         self.dev = dev = prop.device
         self.B = len(sources)
@@ -276,7 +278,8 @@ class _Reconstruction:
             self.recorded = b
         self.prop.forward(self.U, a, b, aux=self.aux, sources=self.src, signal=self.signal,
                           sensors=self.rec if first else None,
-                          record=self.record if first else None, strip=strip)
+                          record=self.record if first else None, strip=strip,
+                          illum=self.illum if first else None)
         self.steps += b - a
 
     def store(self, k: int) -> None:

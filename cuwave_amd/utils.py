@@ -156,7 +156,8 @@ def misfit(sim: Simulation, sources, indicator, sensors: Sensors, observed,
 
 def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, sensors: Sensors,
                     observed, objective: Callable = l2_misfit, adjoint: Callable = sensitivity,
-                    devices=None, batch: int | None = None, checkpoints: int | None = None):
+                    devices=None, batch: int | None = None, checkpoints: int | None = None,
+                    illum: torch.Tensor | None = None, below: float | None = None):
     """Misfit summed over the shots and its derivative with respect to `indicator`.
 
     The shots advance together, `batch` of them a launch (all by default): one
@@ -169,7 +170,16 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
     for either (the stored history is never needed); any other callable runs
     shot by shot as the reference does.
 
-    Returns (cost, gradient), the gradient a tensor over the padded grid.
+    Two additions for iterative inversion, neither in the reference: `illum`, a
+    per-node accumulator over the padded grid, gathers the illumination (the
+    sum over steps and shots of u^2, as `wave.illumination`) in the forward
+    sweep; and with `below` given, the adjoint half runs only if the cost comes
+    out below it (a line-search trial that fails costs one forward sweep, and
+    one that succeeds has the next gradient), every batch of shots then held
+    until the cost is known.
+
+    Returns (cost, gradient), the gradient a tensor over the padded grid (None
+    when `below` stopped it).
     """
     if devices is not None:
         raise NotImplementedError("devices= shot dealing is not part of this build (one GPU)")
@@ -177,9 +187,13 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
     if type(sim).__name__ == "ElasticWave" and adjoint in (sensitivity, reconstruction_sensitivity):
         from . import elastic
 
+        if illum is not None or below is not None:
+            raise NotImplementedError("illum= and below= are built for the scalar equations")
         return elastic.misfit_gradient(sim, sources, indicator, sensors, observed, objective,
                                        batch, checkpoints)
     if adjoint not in (sensitivity, reconstruction_sensitivity):
+        if illum is not None or below is not None:
+            raise NotImplementedError("illum= and below= are for the batched adjoint sweep")
         cost, gradient = 0.0, None
         d_mass, d_stiff = sim.parametrization_jacobian(indicator)
         for source, data in zip(sources, observed):
@@ -198,22 +212,37 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
     mat = sim.build_materials(indicator)
     cost = 0.0
     gradient = None
+    held = []
+
+    def adjoint_half(sweep, dphi):
+        try:
+            sweep.send(dphi)
+        except StopIteration:
+            pass
+
     for start, chunk in _batches(sources, batch):
-        run = _Reconstruction(sim, mat, chunk, sensors.nodes, checkpoints=checkpoints)
+        run = _Reconstruction(sim, mat, chunk, sensors.nodes, checkpoints=checkpoints,
+                              illum=illum)
         acc = run.prop.accumulator()
         valid = run.valid
-
-        def objectives(record, start=start, chunk=chunk):
-            costs, dphis = [], []
-            for b in range(len(chunk)):
-                c, dphi = sensors.objective(objective(observed[start + b]))(record[b])
-                costs.append(c)
-                dphis.append(torch.as_tensor(dphi, dtype=sim.dtype, device=record.device))
-            return costs, torch.stack(dphis)
-
-        costs = run.run(objectives, dict(g_mass=acc, combine=(jm, js), skip_layer=True))
-        cost += sum(costs)
-        gradient = acc if gradient is None else gradient + acc
+        sweep = run.sweep(dict(g_mass=acc, combine=(jm, js), skip_layer=True))
+        record = next(sweep)
+        dphis = []
+        for b in range(len(chunk)):
+            c, dphi = sensors.objective(objective(observed[start + b]))(record[b])
+            cost += c
+            dphis.append(torch.as_tensor(dphi, dtype=sim.dtype, device=record.device))
+        if below is None:
+            adjoint_half(sweep, torch.stack(dphis))
+            gradient = acc if gradient is None else gradient + acc
+        else:
+            held.append((sweep, torch.stack(dphis), acc))
+    if below is not None:
+        if not cost < below:
+            return cost, None
+        for sweep, dphi, acc in held:
+            adjoint_half(sweep, dphi)
+            gradient = acc if gradient is None else gradient + acc
     grads = _finalize(sim, {"g": gradient}, gradient.device)
     grads = _zero_outside(grads, ~valid, gradient.device)
     return cost, grads["g"]

@@ -97,6 +97,29 @@ def test_batched_misfit_gradient_matches_the_reference(tag):
         assert rel_max(gradient, z["r_gradient"]) < tol(sim, True)
 
 
+def test_the_gradient_sweep_gathers_the_illumination_and_stops_above_a_bound():
+    # illum= is the illumination map of the same shots, whatever the checkpoints;
+    # below= runs the adjoint half only under the bound, bitwise as without it
+    z = golden("misfit_acoustic_o4_f64")
+    sim = product_sim(z)
+    sensors = utils.Sensors(sim, z["rec_coords"])
+    sources = [wave.Source(p, s) for p, s in zip(z["src_pos"], z["src_sig"])]
+    observed = list(z["observed"])
+    print("case illum and below misfit_acoustic_o4_f64", flush=True)
+    expect = wave.illumination(sim, sources, z["indicator"])
+    for kw in ({}, {"batch": 2, "checkpoints": 1}):
+        illum = torch.zeros_like(expect)
+        cost, gradient = utils.misfit_gradient(sim, sources, z["indicator"], sensors,
+                                               observed, illum=illum, **kw)
+        assert rel_l2(host(illum), host(expect)) < 1e-12, kw
+        c, g = utils.misfit_gradient(sim, sources, z["indicator"], sensors, observed,
+                                     below=cost * (1 + 1e-9), **kw)
+        assert c == cost and torch.equal(g, gradient), kw
+        c, g = utils.misfit_gradient(sim, sources, z["indicator"], sensors, observed,
+                                     below=cost, **kw)
+        assert c == cost and g is None, kw
+
+
 def test_a_batch_steps_every_shot_as_alone():
     # shots in one launch: each bitwise its single-shot run
     z = golden("fwd_pml_acoustic_o8_f32")
