@@ -132,7 +132,7 @@ def gpu_run(args):
         traces = autograd.traces(gamma, sim, sources, rec)
         loss = 0.5 * torch.sum((traces - observed) ** 2)
         loss.backward()
-        return float(loss), c.grad
+        return float(loss.detach()), c.grad
 
     for _ in range(args.warmup):
         step()
@@ -461,7 +461,7 @@ def learned_line(args) -> dict:
     ellipses (seeds 0.. for training, 1000.. held out), the start model each
     one's Gaussian smoothing (sigma roi / 32). Training: one sample a model
     (the features at the start model against the step to the true one),
-    AdamW lr 1e-3, batch 16.
+    AdamW lr 1e-3, batch 16. The shots run `shot_batch` a sweep.
     """
     import torch
     from scipy.ndimage import gaussian_filter
@@ -504,7 +504,7 @@ def learned_line(args) -> dict:
     for seed in range(args.train_models):
         t, s = pair(seed)
         samples.append(learned.training_sample(sim, sources, sensors, indicator(t),
-                                               indicator(s)))
+                                               indicator(s), batch=args.shot_batch))
         if (seed + 1) % 16 == 0:
             note(f"{seed + 1} of {args.train_models} training samples")
     torch.cuda.synchronize()
@@ -521,6 +521,7 @@ def learned_line(args) -> dict:
     held = {1000 + k: pair(1000 + k) for k in range(args.eval_models)}
     results = learned.evaluate(
         sim, sources, sensors, indicator, held, net, args.iterations, step=args.fwi_step,
+        batch=args.shot_batch,
         progress=lambda r: note(f"model {r['seed']}: ratio {r['ratio']}, plain "
                                 f"{r['plain']['history'][-1][1]:.4g}, learned "
                                 f"{r['learned']['history'][-1][1]:.4g}"))
@@ -538,7 +539,8 @@ def learned_line(args) -> dict:
                    "grid": f"{n}x{n} (ROI {roi}^2 + {LAYER}-cell PML)", "space_order": 8,
                    "shots": shots, "receivers": len(coords), "time_steps": nt,
                    "train_models": args.train_models, "eval_models": args.eval_models,
-                   "iterations": args.iterations, "epochs": args.epochs},
+                   "iterations": args.iterations, "epochs": args.epochs,
+                   "shots_a_sweep": min(args.shot_batch, shots)},
         "reached": f"{len(reached)} of {len(ratios)} held-out models",
         "per_model": [{"seed": r["seed"], "ratio": r["ratio"], "plain_s": r["plain_s"],
                        "learned_s": r["learned_s"],
@@ -717,10 +719,12 @@ def main():
     p.add_argument("--train-models", type=int, default=256)
     p.add_argument("--eval-models", type=int, default=16)
     p.add_argument("--iterations", type=int, default=30)
-    p.add_argument("--epochs", type=int, default=40)
     p.add_argument("--receivers", type=int, default=128)
     p.add_argument("--fwi-step", type=float, default=0.2,
                    help="the first FWI step, the largest indicator change")
+    p.add_argument("--shot-batch", type=int, default=16,
+                   help="learned: shots a sweep (24 shots as 16 + 8 leave no lane padding)")
+    p.add_argument("--epochs", type=int, default=100)
     p.add_argument("--workload", choices=("2d", "3d", "elastic", "forward", "learned"),
                    default="2d",
                    help="3d: the 3D acoustic gradient on a 354^3 grid instead (its own line); "

@@ -371,10 +371,20 @@ __device__ __forceinline__ bool general_node(const Region &r, const Geom<T> &g,
     a[0] = 1 + t / W;
     a[1] = 1 + t % W;
   } else {
-    const int W = g.n[2] - 2, P = (g.n[1] - 2) * W;
-    a[0] = 1 + t / P;
-    a[1] = 1 + (t % P) / W;
-    a[2] = 1 + t % W;
+    // the rows cut into kXcds stripes, a stripe's nodes in (plane, row,
+    // column) order: the run of items an XCD takes is one stripe marched
+    // plane by plane, so the planes the slow-axis taps read (R either side)
+    // are the stripe's share of them, which its L2 holds (whole planes it
+    // would not)
+    const int W = g.n[2] - 2, H = g.n[1] - 2;
+    const int hs = (H + kXcds - 1) / kXcds;
+    const int64_t full = (int64_t)(g.n[0] - 2) * hs * W;
+    const int k = (int)(t / full);
+    const int rem = (int)(t - k * full);
+    const int rows = min(hs, H - k * hs);
+    a[0] = 1 + rem / (rows * W);
+    a[1] = 1 + k * hs + (rem % (rows * W)) / W;
+    a[2] = 1 + rem % W;
   }
   return true;
 }

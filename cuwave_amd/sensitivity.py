@@ -214,6 +214,49 @@ def sensitivity(sim: Simulation, source: Source, indicator, sensors, objective):
     return cost, grads, um, {}
 
 
+# ------------------------------------------------------------ source gradient
+def source_sensitivity(sim: Simulation, source: Source, indicator, sensors, objective):
+    """Cost and d(cost)/d(source.signal): the adjoint field read at the source.
+
+    The traces are linear in the signal, so the gradient pairs no forward field
+    with the adjoint and neither history is kept (cuwave/sensitivity.py:1054-1111).
+    The adjoint marches alone in its two slots, its gradient densities switched
+    off, one level a call, and each level is read at the source nodes before
+    the next overwrites it (a launch a level, as the reference's probe is; this
+    is not a batched hot path). The rows are then weighted as `adjoint_signal`
+    divides (its transpose).
+
+    Returns (cost, gradient, traces, {}), the gradient (N, source points) as
+    `source.signal` is laid out.
+    """
+    if _elastic(sim):
+        raise NotImplementedError("source_sensitivity is built for the scalar equations")
+    _require(sim, source, sensors)
+    from .wave import simulate
+
+    _, um = simulate(sim, source, indicator, sensors)
+    cost, dphi = objective(um)
+    mat = sim.build_materials(indicator)
+    prop = Propagator(sim, mat, 1)
+    dev = prop.device
+    adjoint = prop.lane_table(adjoint_signal(sim, dphi, sensors)[None])
+    lin = sim.flatten_indices(sensors)
+    adj = sensor_points(prop, sensors, weight=sim.excitation_weights(mat, lin))
+    P = prop.levels_memory(2)
+    U = prop.levels_memory(2)  # zero: the densities the kernel adds are zero
+    acc = prop.accumulator()
+    eta = prop.eta_memory()
+    at = torch.as_tensor(sim.flatten_indices(source.position), device=dev, dtype=torch.int64)
+    slots = P.view(2, -1, prop.lanes)
+    lam = torch.empty((sim.N, len(at)), dtype=sim.dtype, device=dev)
+    for n in range(sim.N - 1, -1, -1):
+        prop.backward(P, U, n, n + 1, eta=eta, g_mass=acc, combine=(0.0, 0.0), sensors=adj,
+                      adjoint=adjoint)
+        lam[n] = slots[n % 2, at, 0]
+    w = torch.as_tensor(np.asarray(sim.adjoint_weights(source.position)), device=dev)
+    return cost, lam * w, um, {}
+
+
 # ------------------------------------------------------ reconstruction (strip)
 class _Reconstruction:
     """The forward/backward machinery of the strip-replayed reverse march over a batch.

@@ -207,8 +207,9 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
         sim.require_interior(s.position, "source")
     sim.require_interior(sensors.nodes, "sensor")
     jm, js = sim.parametrization_jacobian(indicator)
-    if not (np.isscalar(jm) and np.isscalar(js)):
-        raise NotImplementedError("a field-valued Jacobian goes through autograd (cuwave_amd.autograd)")
+    constant = np.isscalar(jm) and np.isscalar(js)
+    if not constant and tuple(sim.indicator_shape()) != tuple(sim.Nx_padded):
+        raise NotImplementedError("a stacked indicator goes through autograd (cuwave_amd.autograd)")
     mat = sim.build_materials(indicator)
     cost = 0.0
     gradient = None
@@ -223,9 +224,14 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
     for start, chunk in _batches(sources, batch):
         run = _Reconstruction(sim, mat, chunk, sensors.nodes, checkpoints=checkpoints,
                               illum=illum)
-        acc = run.prop.accumulator()
         valid = run.valid
-        sweep = run.sweep(dict(g_mass=acc, combine=(jm, js), skip_layer=True))
+        if constant:  # the kernel chains the Jacobian as it sums: one accumulator
+            acc = run.prop.accumulator()
+            mode = dict(g_mass=acc, combine=(jm, js), skip_layer=True)
+        else:  # a nodal Jacobian: both densities, chained after the sweep
+            acc = torch.stack([run.prop.accumulator(), run.prop.accumulator()])
+            mode = dict(g_mass=acc[0], g_stiff=acc[1], skip_layer=True)
+        sweep = run.sweep(mode)
         record = next(sweep)
         dphis = []
         for b in range(len(chunk)):
@@ -243,6 +249,15 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
         for sweep, dphi, acc in held:
             adjoint_half(sweep, dphi)
             gradient = acc if gradient is None else gradient + acc
+    if not constant:
+        w = torch.as_tensor(sim.cell_weights(), device=gradient.device)
+
+        def field(d):
+            return torch.as_tensor(d, dtype=gradient.dtype, device=gradient.device)
+
+        gradient = field(jm) * (gradient[0] * w) + field(js) * (gradient[1] * w)
+        gradient[torch.as_tensor(~valid, device=gradient.device)] = 0.0
+        return cost, gradient
     grads = _finalize(sim, {"g": gradient}, gradient.device)
     grads = _zero_outside(grads, ~valid, gradient.device)
     return cost, grads["g"]
