@@ -399,19 +399,32 @@ def forward_line(args) -> dict:
     from cuwave_amd import wave
     from cuwave_amd.engine import Propagator
 
+    from cuwave_amd import tuning
+
     nt = args.nt if args.nt != STEPS else 2000
     sim, indicator, source, sensors = workload_forward(nt)
     n_int = int(np.prod([n - 2 for n in sim.Nx]))
-    for _ in range(args.warmup):
-        wave.simulate(sim, source, indicator, sensors)
-    torch.cuda.synchronize()
+
+    def runs(count):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(count):
+            wave.simulate(sim, source, indicator, sensors)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    window = wave.graph_window(sim, 1)
+    runs(args.warmup)
     Propagator.timings = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        field, traces = wave.simulate(sim, source, indicator, sensors)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    elapsed = runs(args.steps)
     timings, Propagator.timings = Propagator.timings, None
+    # the same runs replayed from captured windows of 24 steps, for comparison
+    below, wave.GRAPH_BELOW = wave.GRAPH_BELOW, 1 << 40
+    try:
+        graphed_s = runs(args.steps)
+    finally:
+        wave.GRAPH_BELOW = below
+    tuned = tuning.tune(sim, 1, indicator, save=False)
     launch_ms = sum(s.elapsed_time(e) for _, _, s, e in timings) / sum(n for _, n, _, _ in timings)
     achieved = 20.0 * n_int / (launch_ms * 1e-3)
     cpu = None
@@ -448,6 +461,14 @@ def forward_line(args) -> dict:
         "cpu_baseline": cpu,
         "kernels": {"step_us": launch_ms * 1e3,
                     "note": "a step's launches on the stream, averaged over every step run"},
+        "graph_window": window,
+        "graphed": {"value": n_int * nt * args.steps / graphed_s / 1e9,
+                    "ms_per_step": graphed_s / args.steps * 1e3,
+                    "note": "the same runs replayed from one captured 24-step window"},
+        "tuner": {"stream_threshold": tuned["stream_threshold"],
+                  "graph_window": tuned["graph_window"],
+                  "us_per_step": {k: round(v * 1e3, 2) for k, v in tuned["timings"].items()},
+                  "note": "tuning.tune on this sweep: threshold/window candidates, best of 3"},
     }
 
 

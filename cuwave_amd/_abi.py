@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 6
+ABI_VERSION = 7
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -279,6 +279,7 @@ SIGNATURES = {
     "cw_point_lists": ([ctypes.POINTER(Grid), VP, I32, VP, VP], I32),
     "cw_scalar_prepare": ([ctypes.POINTER(Grid), ctypes.POINTER(Materials), VP], I32),
     "cw_scalar_forward": ([ctypes.POINTER(ForwardArgs), I32, I32, VP], I32),
+    "cw_scalar_forward_clocked": ([ctypes.POINTER(ForwardArgs), VP, I32, I32, VP], I32),
     "cw_scalar_backward": ([ctypes.POINTER(BackwardArgs), I32, I32, VP], I32),
     "cw_scalar_forward_static": ([ctypes.POINTER(ForwardArgs), I32, I32, VP], I32),
     "cw_scalar_superposed": ([ctypes.POINTER(SuperposedArgs), VP], I32),
@@ -324,6 +325,11 @@ def call(name: str, *args) -> None:
     rc = getattr(lib, name)(*args)
     if rc:
         raise RuntimeError(f"{name}: {lib.cw_last_error().decode()}")
+
+
+def call_value(name: str, *args) -> int:
+    """Call a setter that returns a value (the previous setting), not a status."""
+    return int(getattr(load(), name)(*args))
 
 
 def ptr(t: torch.Tensor | None) -> int | None:

@@ -265,7 +265,8 @@ inline int general_blocks(const cw_grid &gr, const cw::Region &r, int V) {
 // translation unit per (T, R)
 template <typename T, int NDIM, int R>
 int forward_steps(const cw_forward_args &a, int t0, int t1,
-                  hipStream_t stream);
+                  hipStream_t stream, const int *clock = nullptr,
+                  int t_end = 0);
 template <typename T, int NDIM, int R>
 int backward_steps(const cw_backward_args &a, int n0, int n1,
                    hipStream_t stream);

@@ -198,6 +198,24 @@ template <typename T, int NDIM, int R> struct Forward {
   }
 };
 
+// the clock's advance after a window of `steps` steps (one thread)
+__global__ void clock_tick(int *clock, int steps) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    clock[threadIdx.x] += steps;
+}
+
+// a window of steps 0 .. steps - 1 past the device clock, then its advance
+template <typename T, int NDIM, int R> struct Clocked {
+  static int run(const cw_forward_args &a, int32_t *clock, int steps,
+                 int t_end, hipStream_t stream) {
+    if (int e = cwh::forward_steps<T, NDIM, R>(a, 0, steps, stream, clock,
+                                               t_end))
+      return e;
+    hipLaunchKernelGGL(clock_tick, dim3(1), dim3(64), 0, stream, clock, steps);
+    return hip_status("clock_tick");
+  }
+};
+
 template <typename T, int NDIM, int R> struct Backward {
   static int run(const cw_backward_args &a, int n0, int n1,
                  hipStream_t stream) {
@@ -329,6 +347,28 @@ int cw_scalar_prepare(const cw_grid *grid, const cw_materials *mat,
   if (!mat || !mat->stiff || !mat->prep)
     return fail("prepare needs stiff and prep");
   return dispatch<Prepare>(*grid, *grid, *mat, (hipStream_t)stream);
+}
+
+int cw_scalar_forward_clocked(const cw_forward_args *args, int32_t *clock,
+                              int32_t steps, int32_t t_end, void *stream) {
+  if (!args)
+    return fail("args is NULL");
+  if (!clock)
+    return fail("clock is NULL");
+  if (steps < 0 || t_end < 0)
+    return fail("steps and t_end must be non-negative");
+  if (int e = check_grid(&args->grid))
+    return e;
+  if (!args->mat.prep)
+    return fail("forward needs the prepared materials");
+  // every step the clock can reach lies in 0 .. t_end - 1; the kernels skip
+  // the rest
+  if (int e = check_forward(*args, 0, t_end))
+    return e;
+  if (steps == 0)
+    return 0;
+  return dispatch<Clocked>(args->grid, *args, clock, (int)steps, (int)t_end,
+                           (hipStream_t)stream);
 }
 
 int cw_scalar_forward(const cw_forward_args *args, int32_t t0, int32_t t1,

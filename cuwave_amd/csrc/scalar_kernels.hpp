@@ -87,6 +87,10 @@ template <typename T> struct FwdParams {
   T *record;
   Strip<T> strip;
   T *illum; // per node: sum over steps and lanes of u^2, or null
+  // a step index offset in device memory (a launch's step t is t + *clock),
+  // or null; the steps from t_end on are skipped
+  const int *clock;
+  int t_end;
 };
 
 template <typename T> struct BwdParams {

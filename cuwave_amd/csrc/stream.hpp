@@ -523,11 +523,24 @@ __device__ __forceinline__ void backward_stream(const BwdParams<T> &p,
 // the rest take the general nodes (the 2D frame around the box, or every
 // node in 1D and 3D), V shot lanes a thread. W is the shot lanes of a node
 // for the stream blocks (0 where there are none).
+// the step a launch makes: `t`, offset by the device clock where one is given
+// (-1 outside 0 .. t_end - 1: the launch does nothing)
+template <typename T>
+__device__ __forceinline__ int clocked_step(const FwdParams<T> &p, int t) {
+  if (!p.clock)
+    return t;
+  const int tc = t + *p.clock;
+  return tc >= 0 && tc < p.t_end ? tc : -1;
+}
+
 template <typename T, int NDIM, int R, int W, int V>
 __global__ __launch_bounds__(kThreads) void forward_step(const FwdParams<T> p,
                                                          const Box box,
-                                                         const int t,
+                                                         const int step,
                                                          const int streams) {
+  const int t = clocked_step(p, step);
+  if (t < 0)
+    return;
   if constexpr (NDIM == 2 && W > 0) {
     if ((int)blockIdx.x < streams) {
       forward_stream<T, R, W, true, true>(p, box, t, blockIdx.x, streams);
@@ -569,7 +582,10 @@ __global__ __launch_bounds__(kThreads,
 template <typename T, int R, int W, bool ILLUM>
 __global__ __launch_bounds__(kThreads) void forward_core(const FwdParams<T> p,
                                                          const Box box,
-                                                         const int t) {
+                                                         const int step) {
+  const int t = clocked_step(p, step);
+  if (t < 0)
+    return;
   forward_stream<T, R, W, false, ILLUM>(p, box, t, blockIdx.x, gridDim.x);
 }
 

@@ -207,6 +207,38 @@ class Propagator:
         with self._timed("forward", t1 - t0):
             _abi.call("cw_scalar_forward", ctypes.byref(a), int(t0), int(t1), _abi.stream())
 
+    def march(self, u: torch.Tensor, t0: int, t1: int, window: int, **kw) -> None:
+        """Steps t0 .. t1 - 1 as `forward` takes them, in windows of `window` steps
+        replayed from one captured graph (cw_scalar_forward_clocked).
+
+        The step index lives in a device counter that each window advances, so
+        one capture serves every window: the first window runs eagerly (it warms
+        whatever the library sets up on first use), one is captured, the graph is
+        replayed for every further whole window, and the remainder runs eagerly.
+        A short march, or `window` < 1, is one `forward` call."""
+        if window < 1 or t1 - t0 < 3 * window:
+            return self.forward(u, t0, t1, **kw)
+        a = self.forward_args(u, **kw)
+        clock = torch.full((1,), t0, dtype=torch.int32, device=self.device)
+
+        def window_of(steps: int) -> None:
+            _abi.call("cw_scalar_forward_clocked", ctypes.byref(a), _abi.ptr(clock), int(steps),
+                      int(t1), _abi.stream())
+
+        with self._timed("forward", t1 - t0):
+            window_of(window)
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                window_of(window)
+            whole = (t1 - t0) // window - 1
+            for _ in range(whole):
+                graph.replay()
+            rest = t1 - t0 - window * (whole + 1)
+            if rest:
+                window_of(rest)
+            torch.cuda.current_stream().synchronize()  # the graph dies with this call
+        del graph
+
     def forward_args(self, u: torch.Tensor, *, offset: int = 0, aux=None,
                      sources: PointList | None = None, signal: torch.Tensor | None = None,
                      sensors: PointList | None = None, record: torch.Tensor | None = None,

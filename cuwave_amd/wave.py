@@ -234,6 +234,26 @@ def shot_lanes(nbatch: int) -> int:
 
 
 # ------------------------------------------------------------------- forward
+GRAPH_WINDOW = 24  # steps a captured window holds, the reference's chunk (cuwave/wave.py:841)
+# node-lanes of a sweep under which it replays captured windows untuned: none.
+# On a 298 x 298 one-shot sweep a step's kernel takes about 7 us, longer than
+# launching it, and the graphed march measured 9.1 us a step against 7.7 us
+# launched plainly; the tuner measures each sweep shape instead.
+GRAPH_BELOW = 0
+
+
+def graph_window(sim: Simulation, lanes: int) -> int:
+    """The steps a captured graph window of a forward march holds, 0 for none:
+    the tuned choice for the sweep's shape (`cuwave_amd.tuning`) where there is
+    one, else GRAPH_WINDOW for sweeps under GRAPH_BELOW node-lanes."""
+    from . import tuning
+
+    tuned = tuning.choice(sim, lanes, "graph_window")
+    if tuned is not None:
+        return int(tuned)
+    return GRAPH_WINDOW if int(np.prod(sim.Nx_padded)) * lanes < GRAPH_BELOW else 0
+
+
 def simulate_shots(sim: Simulation, sources, indicator, sensors=None, record_every=None):
     """Run every shot of `sources` forward in one batch under material `indicator`.
 
@@ -264,7 +284,11 @@ def simulate_shots(sim: Simulation, sources, indicator, sensors=None, record_eve
         record = prop.record_memory(rec)
     snapshots = []
     if record_every is None:
-        prop.forward(U, 0, sim.N, aux=aux, sources=src, signal=signal, sensors=rec, record=record)
+        from . import tuning
+
+        with tuning.applied(sim, prop.lanes):
+            prop.march(U, 0, sim.N, graph_window(sim, prop.lanes), aux=aux, sources=src,
+                       signal=signal, sensors=rec, record=record)
     else:
         for t in range(sim.N):
             prop.forward(U, t, t + 1, aux=aux, sources=src, signal=signal, sensors=rec,
@@ -307,7 +331,11 @@ def illumination(sim: Simulation, sources, indicator, batch: int | None = None) 
             illum = prop.accumulator()
         src, signal = source_points(prop, mat, sources[i:i + step])
         U = prop.levels_memory(2)
-        prop.forward(U, 0, sim.N, aux=prop.aux_memory(), sources=src, signal=signal, illum=illum)
+        from . import tuning
+
+        with tuning.applied(sim, prop.lanes):
+            prop.march(U, 0, sim.N, graph_window(sim, prop.lanes), aux=prop.aux_memory(),
+                       sources=src, signal=signal, illum=illum)
     return illum
 
 

@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 6
+#define CW_ABI_VERSION 7
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -219,6 +219,18 @@ int cw_scalar_prepare(const cw_grid *grid, const cw_materials *mat,
 
 int cw_scalar_forward(const cw_forward_args *args, int32_t t0, int32_t t1,
                       void *stream);
+
+/* A window of `steps` forward steps whose step index comes from device
+ * memory: step k of the window is *clock + k, and after the window *clock
+ * grows by `steps`. A window's launches are then the same on every call, so
+ * a caller can capture one call in a graph and replay it for every window of
+ * a march (what the reference does with its 24-step graphs,
+ * cuwave/wave.py:890-952; here without re-capturing). Steps the clock puts
+ * outside 0 .. t_end - 1 are skipped by the kernels; the buffers are checked
+ * for the whole range 0 .. t_end as cw_scalar_forward checks t0 .. t1.
+ * Nothing here allocates or sets memory. */
+int cw_scalar_forward_clocked(const cw_forward_args *args, int32_t *clock,
+                              int32_t steps, int32_t t_end, void *stream);
 
 int cw_scalar_backward(const cw_backward_args *args, int32_t n0, int32_t n1,
                        void *stream);
