@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 7
+ABI_VERSION = 8
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -139,6 +139,21 @@ class SuperposedArgs(ctypes.Structure):
         ("ft", F64),
         ("fs", F64),
         ("step", I32),
+    ]
+
+
+class GradientArgs(ctypes.Structure):
+    _fields_ = [
+        ("grid", Grid),
+        ("mat", Materials),
+        ("u0", VP),
+        ("u1", VP),
+        ("u2", VP),
+        ("l1", VP),
+        ("g_mass", VP),
+        ("g_stiff", VP),
+        ("inv_dt2", F64),
+        ("factor", F64 * 3),
     ]
 
 
@@ -283,6 +298,7 @@ SIGNATURES = {
     "cw_scalar_backward": ([ctypes.POINTER(BackwardArgs), I32, I32, VP], I32),
     "cw_scalar_forward_static": ([ctypes.POINTER(ForwardArgs), I32, I32, VP], I32),
     "cw_scalar_superposed": ([ctypes.POINTER(SuperposedArgs), VP], I32),
+    "cw_scalar_gradient": ([ctypes.POINTER(GradientArgs), VP], I32),
     "cw_elastic_prepare": ([ctypes.POINTER(ElasticGrid), ctypes.POINTER(ElasticMaterial), VP], I32),
     "cw_elastic_step": ([ctypes.POINTER(ElasticStepArgs), VP], I32),
     "cw_elastic_gradient": ([ctypes.POINTER(ElasticGradientArgs), VP], I32),

@@ -237,6 +237,21 @@ template <typename T, int NDIM, int R> struct Superposed {
   }
 };
 
+template <typename T, int NDIM, int R> struct Gradient {
+  static int run(const cw_gradient_args &a, hipStream_t stream) {
+    cw::Geom<T> g = make_geom<T>(a.grid);
+    for (int d = 0; d < NDIM; ++d)
+      g.f[d] = (T)a.factor[d];
+    const int blocks = cwh::general_blocks(a.grid, cw::Region{}, 1);
+    hipLaunchKernelGGL((cw::gradient_kernel<T, NDIM, R>), dim3(blocks),
+                       dim3(cw::kThreads), 0, stream, g, (const T *)a.mat.stiff,
+                       (const T *)a.u0, (const T *)a.u1, (const T *)a.u2,
+                       (const T *)a.l1, (T *)a.g_mass, (T *)a.g_stiff,
+                       (T)a.inv_dt2);
+    return hip_status("gradient_kernel");
+  }
+};
+
 int check_forward(const cw_forward_args &a, int32_t t0, int32_t t1);
 
 // ------------------------------------------------------- static scratch
@@ -369,6 +384,19 @@ int cw_scalar_forward_clocked(const cw_forward_args *args, int32_t *clock,
     return 0;
   return dispatch<Clocked>(args->grid, *args, clock, (int)steps, (int)t_end,
                            (hipStream_t)stream);
+}
+
+int cw_scalar_gradient(const cw_gradient_args *args, void *stream) {
+  if (!args)
+    return fail("args is NULL");
+  if (int e = check_grid(&args->grid))
+    return e;
+  if (args->grid.lanes != 1)
+    return fail("the field gradient takes one shot");
+  if (!args->mat.stiff || !args->u0 || !args->u1 || !args->u2 || !args->l1 ||
+      !args->g_mass || !args->g_stiff)
+    return fail("the field gradient needs stiff, u0, u1, u2, l1 and both accumulators");
+  return dispatch<Gradient>(args->grid, *args, (hipStream_t)stream);
 }
 
 int cw_scalar_forward(const cw_forward_args *args, int32_t t0, int32_t t1,

@@ -900,6 +900,64 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
   }
 }
 
+// ------------------------------------------------------ field gradients
+// The two densities of a forward triplet (u0, u1, u2) against an adjoint
+// field l1 (cuwave/kernels/scalar_sensitivity.cu:303-350), each subtracted
+// from its accumulator: mass inv_dt2 l1 (u2 - 2 u1 + u0); stiffness the
+// harmonic cell means differentiated, per axis as the adjoint kernels form
+// it (backward_general), mirrored past the walls and with l1 zero on a held
+// wall node. One lane, every interior node; g.f holds the step factors over
+// dt^2 (the operator L, not dt^2 L).
+template <typename T, int NDIM, int R>
+__global__ __launch_bounds__(kThreads) void gradient_kernel(
+    const Geom<T> g, const T *__restrict__ stiff, const T *__restrict__ u0,
+    const T *__restrict__ u1, const T *__restrict__ u2,
+    const T *__restrict__ l1, T *__restrict__ gmass, T *__restrict__ gstiff,
+    const T inv_dt2) {
+  constexpr int REACH = R + 2; // scalar_sensitivity.cu:44 WALL_REACH
+  int a[3];
+  if (!general_node<NDIM>(Region{}, g, a, blockIdx.x, gridDim.x, 0))
+    return;
+  int64_t idx = 0;
+  bool held = false, near[NDIM];
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    idx += (int64_t)a[d] * g.s[d];
+    held |= held_wall(a[d], g.n[d], g.dirichlet >> (2 * d));
+    near[d] = a[d] < REACH || a[d] > g.n[d] - 1 - REACH;
+  }
+  const T uc = u1[idx], lc = l1[idx];
+  gmass[idx] -= inv_dt2 * lc * (u2[idx] - (T)2 * uc + u0[idx]);
+  const T sc = stiff[idx];
+  const T lw = held ? (T)0 : lc;
+  T gsum = 0;
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    const int odd = g.dirichlet >> (2 * d);
+    const int64_t s = g.s[d];
+    T ut[2 * R + 1], lt[2 * R + 1], sp, sm;
+    if (near[d]) {
+      reflected_line<R>(u1, idx, s, a[d], g.n[d], odd, ut);
+      reflected_line<R>(l1, idx, s, a[d], g.n[d], odd, lt);
+      sp = reflected<T>(stiff, idx, s, a[d], g.n[d], 0, 1);
+      sm = reflected<T>(stiff, idx, s, a[d], g.n[d], 0, -1);
+      lt[R] = lw;
+    } else {
+#pragma unroll
+      for (int j = -R; j <= R; ++j) {
+        ut[j + R] = u1[idx + j * s];
+        lt[j + R] = l1[idx + j * s];
+      }
+      sp = stiff[idx + s];
+      sm = stiff[idx - s];
+    }
+    ut[R] = uc;
+    gsum += stiffness_density<T, R>(ut, lt, near[d] ? lw : lc, sc, sp, sm,
+                                    g.f[d], g.op);
+  }
+  gstiff[idx] -= gsum;
+}
+
 // --------------------------------------------------- superposition step
 // scalar_sensitivity.cu:599-668 (step = 1) and :352-385 (step = 0), one shot
 template <typename T, int NDIM, int R>

@@ -304,6 +304,27 @@ class Propagator:
         with self._timed("backward", n1 - n0):
             _abi.call("cw_scalar_backward", ctypes.byref(a), int(n0), int(n1), _abi.stream())
 
+    def field_gradient(self, u0, u1, u2, l1, g_mass, g_stiff) -> None:
+        """Subtract the densities of the forward triplet (u0, u1, u2) against the
+        adjoint field l1 from the accumulators (cw_scalar_gradient); one shot, every
+        field over the padded grid."""
+        if self.lanes != 1:
+            raise ValueError("the field gradient takes one shot")
+        sim = self.sim
+        a = _abi.GradientArgs()
+        a.grid = self.grid
+        a.mat = self.mat
+        for what, t in (("u0", u0), ("u1", u1), ("u2", u2), ("l1", l1),
+                        ("mass gradient", g_mass), ("stiffness gradient", g_stiff)):
+            self._require_shape(what, t, tuple(sim.Nx_padded))
+        a.u0, a.u1, a.u2, a.l1 = _abi.ptr(u0), _abi.ptr(u1), _abi.ptr(u2), _abi.ptr(l1)
+        a.g_mass, a.g_stiff = _abi.ptr(g_mass), _abi.ptr(g_stiff)
+        # the operator L rather than dt^2 L (cuwave/scalar.py:775-778)
+        a.inv_dt2 = float(sim.np_dtype(1.0 / sim.dt**2))
+        for d, f in enumerate(sim.step_factors()):
+            a.factor[d] = float(sim.np_dtype(float(f) / sim.dt**2))
+        _abi.call("cw_scalar_gradient", ctypes.byref(a), _abi.stream())
+
     def superposed(self, u0, u1, u2, acc_mass, acc_stiff, ft: float, fs: float, step: bool) -> None:
         """One superposition step, or the Frechet densities of a triplet (cw_scalar_superposed)."""
         if self.lanes != 1:

@@ -31,6 +31,9 @@ from .wave import Simulation, Source, as_numpy
 
 # cancellation past which B(w, w) - B(u, u) has eaten too much of the mantissa
 CANCELLATION_LIMIT = {"float32": 1e5, "float64": 1e12}
+DECAY_LIMIT = 1e-2  # the share of a spectrum the last period may still add
+FIT_LIMIT = {"float32": 1e-4, "float64": 1e-10}  # how far dcost/dtraces may sit off the sinusoids
+PULSE_PERIODS = 1.0  # the adjoint pulse's Gaussian width, in periods
 ADJOINT_DELAY = 1  # lines the superposition adjoint up with the forward triplet
 
 
@@ -556,3 +559,180 @@ def superposition_sensitivity(sim: Simulation, source: Source, indicator, sensor
     for f in accs.values():
         f.mul_(norm)
     return cost, accs, um, {"scale": scale, "cancellation": cancellation}
+
+
+# ----------------------------------------------------------- spectral gradient
+def phase_table(sim: Simulation, frequencies) -> np.ndarray:
+    """(N, 2F) float64: cos(w t) and -sin(w t) per frequency, interleaved; the
+    running DFT of a field is its sum over t against these rows
+    (cuwave/sensitivity.py:295-303). Built in double: the phase reaches 1e5 rad."""
+    w = 2.0 * np.pi * np.asarray(frequencies, dtype=np.float64)
+    phase = np.outer(np.arange(sim.N, dtype=np.float64) * sim.dt, w)
+    return np.stack((np.cos(phase), -np.sin(phase)), axis=2).reshape(sim.N, -1)
+
+
+def spectral_load(sim: Simulation, dphi, table: np.ndarray):
+    """Least-squares fit dphi = sum_f a_f cos(w_f t) + b_f sin(w_f t) per sensor
+    (cuwave/sensitivity.py:306-323). Returns the (F, sensors) complex loads
+    a + i b and the relative residual of the fit."""
+    basis = table * np.tile([1.0, -1.0], table.shape[1] // 2)  # cos, sin columns
+    target = np.asarray(as_numpy(dphi), dtype=np.float64)
+    coef = np.linalg.lstsq(basis, target, rcond=None)[0]
+    norm = float(np.linalg.norm(target))
+    fit = float(np.linalg.norm(target - basis @ coef)) / norm if norm > 0.0 else 0.0
+    return coef[0::2] + 1j * coef[1::2], fit
+
+
+def spectral_pulse(sim: Simulation, frequency: float, load) -> np.ndarray:
+    """(N, columns) Gaussian-windowed carriers whose DTFT at `frequency` is `load`
+    (cuwave/sensitivity.py:326-346): a cos and a sin carrier under one envelope
+    of width PULSE_PERIODS periods centred at four widths, mixed per column so
+    the pair's transform at w equals the complex load. Raises when eight widths
+    outlast the run."""
+    sigma = PULSE_PERIODS / frequency
+    t = np.arange(sim.N, dtype=np.float64) * sim.dt
+    if 8.0 * sigma > t[-1]:
+        raise ValueError(f"the adjoint pulse spans {8.0 * sigma:.3g}, longer than the run "
+                         f"{t[-1]:.3g}")
+    w = 2.0 * np.pi * frequency
+    window = np.exp(-0.5 * ((t - 4.0 * sigma) / sigma) ** 2)
+    carriers = np.stack((window * np.cos(w * t), window * np.sin(w * t)), axis=1)  # (N, 2)
+    at_w = np.exp(-1j * w * t) @ carriers  # each carrier's DTFT at w
+    load = np.atleast_1d(np.asarray(load))
+    mix = np.linalg.solve(np.array([at_w.real, at_w.imag]), np.stack((load.real, load.imag)))
+    return carriers @ mix
+
+
+def settled(sim: Simulation, spectrum, last, frequency: float):
+    """The DFT continued past step N as the constant `last` the run settled to: the
+    geometric tail sum_{t >= N} z^t = z^N / (1 - z) (cuwave/sensitivity.py:349-359)."""
+    z = np.exp(-2j * np.pi * frequency * sim.dt)
+    return spectrum + last * complex(z**sim.N / (1.0 - z))
+
+
+def spectra(sim: Simulation, acc, last, frequency: float, steps: int):
+    """One frequency's settled spectrum from its (4, num) accumulator (the DFT over
+    all steps, re and im, then over the last `steps`), and the share of it the
+    last `steps` add beyond the settled constant (cuwave/sensitivity.py:362-388)."""
+    whole = acc[0].to(torch.complex128) + 1j * acc[1].to(torch.float64)
+    tail = acc[2].to(torch.complex128) + 1j * acc[3].to(torch.float64)
+    z = np.exp(-2j * np.pi * frequency * sim.dt)
+    tail = tail - last * complex(np.sum(z ** np.arange(sim.N - steps, sim.N)))
+    norm = float(torch.linalg.vector_norm(whole))
+    share = float(torch.linalg.vector_norm(tail)) / norm if norm > 0.0 else 0.0
+    return settled(sim, whole, last, frequency), share
+
+
+def spectral_sensitivity(sim: Simulation, source: Source, indicator, sensors, objective,
+                         frequencies, region=None):
+    """Cost and gradients as `sensitivity`, for a cost that reads the traces only
+    through their DFT at `frequencies` (cuwave/sensitivity.py:896-1051).
+
+    Both passes run forwards in time and keep a running DFT at the nodes the
+    gradient reads (`region`, or every node, widened by the stencil's reach)
+    instead of a history. dcost/dtraces is fitted by the sinusoids at the
+    frequencies (a `ValueError` past FIT_LIMIT: the cost is not spectral); each
+    frequency then drives one causal adjoint run by a decaying pulse with the
+    same spectral load, and the gradient pairs the forward and adjoint spectra
+    (the forward triplet at z^-2, z^-1, 1 against the adjoint) through the field
+    gradient kernel, real and imaginary halves. It is exact once both runs have
+    decayed; `info["decay"]` is the share the last period still added (a
+    `RuntimeWarning` past DECAY_LIMIT), `info["fit"]` the fit residual.
+
+    A forward step and an adjoint level each make one launch here, each followed
+    by the DFT's device update: this variant trades launches for memory and is
+    not a batched path.
+    """
+    from .boundary import require_off_held_walls
+
+    if _elastic(sim):
+        raise NotImplementedError("spectral_sensitivity is built for the scalar equations")
+    _require(sim, source, sensors)
+    require_off_held_walls(sim, np.asarray(as_numpy(source.position)))
+    frequencies = np.atleast_1d(np.asarray(frequencies, dtype=np.float64))
+    logical = np.zeros(sim.Nx_padded, dtype=bool)
+    logical[tuple(slice(0, n) for n in sim.Nx)] = True
+    region = logical if region is None else np.asarray(as_numpy(region), dtype=bool)
+    # the gradient at a node reads its neighbours, ghosts included: so does the DFT
+    square = ndi.generate_binary_structure(sim.ndim, sim.ndim)
+    kept = ndi.binary_dilation(region, structure=square, iterations=sim.reach + 1) & logical
+    nodes = np.stack(np.nonzero(kept)).astype(np.int32)
+
+    mat = sim.build_materials(indicator)
+    prop = Propagator(sim, mat, 1)
+    dev = prop.device
+    at = torch.as_tensor(sim.flatten_indices(nodes), device=dev, dtype=torch.int64)
+    src, signal = source_points(prop, mat, [source])
+    rec = sensor_points(prop, sensors)
+    record = prop.record_memory(rec)
+    host_table = phase_table(sim, frequencies)
+    F = len(frequencies)
+    period = math.ceil(1.0 / (float(frequencies.min()) * sim.dt))
+    tail = host_table.copy()
+    tail[: sim.N - period] = 0.0  # the same factors over the last period alone
+    table = torch.as_tensor(np.concatenate((host_table, tail), axis=1), dtype=sim.dtype,
+                            device=dev)
+
+    # forward: the field's running DFT at the kept nodes
+    U = prop.levels_memory(2)
+    aux = prop.aux_memory()
+    slots = U.view(2, -1)
+    u_hat = torch.zeros((4 * F, len(at)), dtype=sim.dtype, device=dev)
+    for t in range(sim.N):
+        prop.forward(U, t, t + 1, aux=aux, sources=src, signal=signal, sensors=rec,
+                     record=record)
+        u_hat += table[t][:, None] * slots[t % 2, at][None, :]
+    u_last = slots[(sim.N - 1) % 2, at].to(torch.float64)
+    um = prop.shots(record)[0]
+    cost, dphi = objective(um)
+
+    load, fit = spectral_load(sim, dphi, host_table)
+    if fit > FIT_LIMIT[sim.precision]:
+        raise ValueError(f"dcost/dtraces is off the sinusoids at {frequencies} by {fit:.1e}: "
+                         f"the cost is not spectral there, use sensitivity")
+    weights = np.asarray(sim.adjoint_weights(sensors), dtype=np.float64)
+    lin = sim.flatten_indices(sensors)
+    adj = sensor_points(prop, sensors, weight=sim.excitation_weights(mat, lin))
+    acc = {k: prop.accumulator() for k in ("mass", "stiff")}
+    zero_u = prop.levels_memory(2)
+    dummy = prop.accumulator()
+    decay = 0.0
+    shape = tuple(sim.Nx_padded)
+    for f, frequency in enumerate(frequencies):
+        # the causal adjoint run driven at this frequency: level n is its step N - 1 - n
+        pulse = spectral_pulse(sim, frequency, load[f]) / weights
+        adjoint = prop.lane_table(torch.as_tensor(pulse, dtype=sim.dtype)[None])
+        P = prop.levels_memory(2)
+        ring = P.view(2, -1)
+        eta = prop.eta_memory()  # the layer memory starts from rest each run
+        cols = [2 * f, 2 * f + 1, 2 * F + 2 * f, 2 * F + 2 * f + 1]
+        lam_hat = torch.zeros((4, len(at)), dtype=sim.dtype, device=dev)
+        for t in range(sim.N):
+            n = sim.N - 1 - t
+            prop.backward(P, zero_u, n, n + 1, eta=eta, g_mass=dummy, combine=(0.0, 0.0),
+                          sensors=adj, adjoint=adjoint)
+            lam_hat += table[t, cols][:, None] * ring[n % 2, at][None, :]
+        lam_last = ring[0, at].to(torch.float64)
+        spectrum, share = spectra(sim, u_hat[cols], u_last, frequency, period)
+        adjoint_hat, adjoint_share = spectra(sim, lam_hat, lam_last, frequency, period)
+        decay = max(decay, share, adjoint_share)
+        # lambda^t pairs with (u^(t - 2), u^(t - 1), u^t): the triplet is z^-2, z^-1, 1
+        shift = complex(np.exp(-2j * np.pi * frequency * sim.dt))
+        triplet = (spectrum * shift**2, spectrum * shift, spectrum)
+        # Re(a b) = Re a Re b - Im a Im b: the bilinear form in two real halves
+        for part, sign in ((torch.real, 1.0), (torch.imag, -1.0)):
+            fields = [torch.zeros(shape, dtype=sim.dtype, device=dev) for _ in range(4)]
+            for field, value in zip(fields, (*triplet, sign * adjoint_hat)):
+                field.view(-1)[at] = part(value).to(sim.dtype)
+            prop.field_gradient(*fields, acc["mass"], acc["stiff"])
+
+    grads = _finalize(sim, acc, dev)
+    if sim.pml is not None:
+        grads = _zero_outside(grads, pml_nodes(sim), dev)
+    grads = _zero_outside(grads, ~region, dev)
+    if decay > DECAY_LIMIT:
+        warnings.warn(f"the last period still adds more than {DECAY_LIMIT:.0e} of the "
+                      f"spectrum: the gradient misses the tail the run cut off "
+                      f"(info['decay'] says how much). Lengthen the run or damp the faces.",
+                      RuntimeWarning, stacklevel=2)
+    return cost, grads, um, {"decay": decay, "fit": fit}

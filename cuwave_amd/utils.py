@@ -135,6 +135,35 @@ def _batches(items: list, batch: int | None):
         yield i, items[i : i + step]
 
 
+def intensity(sim: Simulation, frequencies, weights=None) -> Callable:
+    """Objective factory: J = sum_f sum_s w |u_hat_s(f)|^2, the spectral intensity at
+    the sensors, u_hat(f) = dt sum_t u(t) e^(-i 2 pi f t dt) (cuwave/utils.py:518-553).
+
+    `weights` broadcasts to (frequencies, sensors) (default 1). The transform is
+    linear in the record, so dJ/dtraces is the same tables read back: a sum of
+    sinusoids at `frequencies`, what `spectral_sensitivity` needs.
+    """
+    f = np.atleast_1d(np.asarray(frequencies, dtype=np.float64))
+    # the phase reaches 1e5 rad over a long run: built in double
+    phase = np.outer(np.arange(sim.N, dtype=np.float64) * sim.dt, 2.0 * np.pi * f)
+    cos_h, sin_h = np.cos(phase), np.sin(phase)
+    scale = float(sim.np_dtype(sim.dt))
+
+    def objective(traces):
+        traces = torch.as_tensor(traces)
+        dev, dtype = traces.device, traces.dtype
+        cos = torch.as_tensor(cos_h, dtype=dtype, device=dev)
+        sin = torch.as_tensor(sin_h, dtype=dtype, device=dev)
+        w = 1.0 if weights is None else torch.as_tensor(np.asarray(as_numpy(weights)),
+                                                       dtype=dtype, device=dev)
+        re = scale * (cos.T @ traces)
+        im = -scale * (sin.T @ traces)
+        cost = float(torch.sum(w * (re * re + im * im)))
+        return cost, 2.0 * scale * (cos @ (w * re) - sin @ (w * im))
+
+    return objective
+
+
 def measure(sim: Simulation, sources: Sequence[Source], indicator, sensors: Sensors,
             devices=None, batch: int | None = None) -> list[torch.Tensor]:
     """Receiver traces per shot, all shots of a batch in one launch a step."""

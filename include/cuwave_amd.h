@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 7
+#define CW_ABI_VERSION 8
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -184,6 +184,19 @@ typedef struct cw_superposed_args {
   int32_t step; /* 1: superposed step; 0: Frechet densities of (u0, u1, u2) */
 } cw_superposed_args;
 
+/* the gradient densities of a forward triplet (u0, u1, u2) against an adjoint
+ * field l1 (cuwave/kernels/scalar_sensitivity.cu:303-350 `gradient_kernel`),
+ * subtracted from the accumulators; one shot (grid.lanes 1). `factor` are the
+ * step factors over dt^2 (cuwave/scalar.py:768-784), `inv_dt2` 1 / dt^2 */
+typedef struct cw_gradient_args {
+  cw_grid grid;
+  cw_materials mat;
+  const void *u0, *u1, *u2, *l1;
+  void *g_mass, *g_stiff;
+  double inv_dt2;
+  double factor[3];
+} cw_gradient_args;
+
 int cw_version(void);
 const char *cw_last_error(void);
 
@@ -253,6 +266,8 @@ int cw_scalar_forward_static(const cw_forward_args *args, int32_t t0,
                              int32_t t1, void *stream);
 
 int cw_scalar_superposed(const cw_superposed_args *args, void *stream);
+
+int cw_scalar_gradient(const cw_gradient_args *args, void *stream);
 
 /* ------------------------------------------------------------ elastic
  * The staggered isotropic elastic family (cuwave/elastic.py:154-794): the
