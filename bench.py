@@ -679,7 +679,7 @@ def gpu_run_elastic(args):
     # a forward step: u1 (2 components) read, u0 read and u2 written over it,
     # gamma and the prepared material once a node; an adjoint level: the
     # reverse and adjoint steps, the triplet and lambda read by the gradient,
-    # three accumulators read and written (SURVEY's per-unit budget)
+    # three accumulators read and written
     bytes_fwd = (24.0 * B + 12.0) * n_int
     bytes_bwd = (48.0 * B + 36.0) * n_int
     achieved = bytes_bwd / (bwd_ms * 1e-3)
