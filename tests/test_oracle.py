@@ -251,6 +251,23 @@ def test_the_illumination_after_one_step_is_the_source_injection_squared():
     np.testing.assert_array_equal(two, 2 * expect)
 
 
+def test_the_illumination_accumulates_every_step():
+    # after k steps it is the sum over t < k of the squared final field of a
+    # t + 1 step run: an overwrite (the last step only) or a skipped step fails
+    z = golden("fwd_pml_acoustic_o8_f32")
+    sim = oracle_sim(z)
+    shots = [(z["src_pos"], z["src_sig"])]
+    inner = tuple(slice(1, n - 1) for n in sim.Nx)
+    expect = np.zeros(sim.Nx_padded)
+    for k in range(1, 7):
+        sim.N = k
+        field = D.simulate(sim, z["src_pos"], z["src_sig"], z["indicator"])
+        expect[inner] += field[inner].astype(np.float64) ** 2  # the run's logical grid
+        got = D.illumination(sim, shots, z["indicator"])
+        np.testing.assert_allclose(got, expect, rtol=1e-12, atol=0)
+    assert np.count_nonzero(expect) > np.count_nonzero(expect == 0) // 100
+
+
 # ------------------------------------------------- independent bulk modulus and density
 def _bulk_sim(kind, ndim=2, N=70, order=4, precision="float64", **kw):
     shape = (40, 36) if ndim == 2 else (22, 21, 20)
