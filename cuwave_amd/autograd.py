@@ -56,7 +56,8 @@ class Traces(torch.autograd.Function):
             sim.require_interior(s.position, "source")
         sim.require_interior(sensors.nodes, "sensor")
         mat = sim.build_materials(ind)
-        run = _Reconstruction(sim, mat, list(sources), sensors.nodes, checkpoints=checkpoints)
+        run = _Reconstruction(sim, mat, list(sources), sensors.nodes, checkpoints=checkpoints,
+                              indicator=ind)
         jm, js = sim.parametrization_jacobian(ind)
         if np.isscalar(jm) and np.isscalar(js):
             # a constant Jacobian chains in the kernel: one accumulator, and no

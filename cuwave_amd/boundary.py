@@ -199,3 +199,73 @@ def pml_nodes(sim: Simulation) -> np.ndarray:
         mask |= layer_rows.reshape(shape)
         inner &= ((a >= 1) & (a <= n - 2)).reshape(shape)
     return mask & inner
+
+
+# ------------------------------------------------------------------ sponge
+def _layer_faces(sim: Simulation, width: int, faces: Sequence[int] | None) -> tuple[int, ...]:
+    """The faces a layer of `width` nodes lines, checked to leave an interior
+    (cuwave/boundary.py:162-174)."""
+    if width < 1:
+        raise ValueError(f"width must be at least one node: {width}")
+    faces = _canonical_faces(sim.ndim, faces)
+    for d in range(sim.ndim):
+        sides = sum(2 * d + side in faces for side in (0, 1))
+        if sides * width >= sim.Nx[d] - 2:
+            raise ValueError(f"{sides} layer(s) of {width} nodes leave no interior on axis {d}: "
+                             f"Nx={sim.Nx[d]}")
+    return faces
+
+
+def _layer_ramp(sim: Simulation, width: int, faces: tuple[int, ...]) -> np.ndarray:
+    """1 on a lined wall node falling linearly to 0 `width` nodes in, the larger of
+    two faces' ramps in a corner, zero off the interior (cuwave/boundary.py:177-193)."""
+    ramp = np.zeros(sim.Nx_padded, dtype=sim.np_dtype)  # in the simulation's precision
+    interior = np.ones(sim.Nx_padded, dtype=bool)
+    for d in range(sim.ndim):
+        shape = [1] * sim.ndim
+        shape[d] = sim.Nx_padded[d]
+        index = np.arange(sim.Nx_padded[d], dtype=sim.np_dtype).reshape(shape)
+        interior &= (index >= 1) & (index <= sim.Nx[d] - 2)
+        for side in (0, 1):
+            if 2 * d + side in faces:
+                wall = 1 if side == 0 else sim.Nx[d] - 2
+                ramp = np.maximum(ramp, np.clip(1.0 - np.abs(index - wall) / width, 0.0, 1.0))
+    return ramp * interior
+
+
+def sponge(sim: Simulation, indicator, width: int, beta: float,
+           faces: Sequence[int] | None = None) -> np.ndarray:
+    """A damping field ramping quadratically to `beta` (the decay per step,
+    d dt / 2m) at the lined walls over `width` nodes, zero inside
+    (cuwave/boundary.py:238-269): d = 2 beta ramp^2 / (dt minv). Set it as
+    `Simulation.damping`. It costs losslessness: `superposition_sensitivity`
+    refuses it and `reconstruction_sensitivity` records the nodes it cannot run
+    back."""
+    from .wave import as_numpy
+
+    if beta < 0.0:
+        raise ValueError(f"beta must be non-negative: {beta}")
+    faces = _layer_faces(sim, width, faces)
+    minv = sim.inverse_inertia(np.asarray(as_numpy(indicator), dtype=sim.np_dtype))
+    ramp = _layer_ramp(sim, width, faces)
+    return np.ascontiguousarray((2.0 * beta * ramp**2 / (sim.dt * minv)).astype(sim.np_dtype))
+
+
+def pad_for_sponge(Nx, dx, thickness: float, faces: Sequence[int] | None = None):
+    """Grow a region of interest `Nx` (ghosts included) by a sponge of physical
+    `thickness` behind `faces` (cuwave/boundary.py:196-235). Returns the grown
+    extent, the layer width in nodes (off the finest axis, so no face is thinner
+    than asked), the region's physical origin per axis, and the index tuple of its
+    interior nodes in the grown grid."""
+    if thickness <= 0.0:
+        raise ValueError(f"thickness must be positive: {thickness}")
+    faces = _canonical_faces(len(Nx), faces)
+    width = round(thickness / min(dx))
+    if width < 1:
+        raise ValueError(f"thickness {thickness} is under one node at dx {min(dx)}")
+    pads = [tuple(width if 2 * d + side in faces else 0 for side in (0, 1))
+            for d in range(len(Nx))]
+    grown = tuple(n + lo + hi for n, (lo, hi) in zip(Nx, pads))
+    origin = tuple(lo * h for (lo, _), h in zip(pads, dx))
+    region = tuple(slice(lo + 1, n - 1 - hi) for (lo, hi), n in zip(pads, grown))
+    return grown, width, origin, region

@@ -438,6 +438,8 @@ int check_forward(const cw_forward_args &a0, int32_t t0, int32_t t1) {
     return fail("a bfloat16 strip needs float32 fields");
   if (t0 < 0 || t1 > args->steps || t0 > t1)
     return fail("steps t0 .. t1 must lie within 0 .. steps");
+  if (args->mat.damping && !(args->mat.dt > 0.0))
+    return fail("a damping field needs dt > 0");
   if (args->strip.map && t0 < t1 &&
       (t0 < args->strip.first || t1 > args->strip.first + args->strip.rows))
     return fail("the strip record does not hold steps t0 .. t1");
@@ -462,6 +464,8 @@ int cw_scalar_backward(const cw_backward_args *args, int32_t n0, int32_t n1,
     return e;
   if (args->lam.slots != 2)
     return fail("lambda takes two slots");
+  if ((args->mat.damping || args->mat.damping_reverse) && !(args->mat.dt > 0.0))
+    return fail("a damping field needs dt > 0");
   if (int e = check_layer(args->grid, args->pml, false))
     return e;
   if (int e = check_points(args->sensors, true, "sensors"))

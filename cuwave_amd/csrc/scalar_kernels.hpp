@@ -91,6 +91,8 @@ template <typename T> struct FwdParams {
   // or null; the steps from t_end on are skipped
   const int *clock;
   int t_end;
+  const T *damp; // nodal damping d, or null (lossless)
+  T dt;
 };
 
 template <typename T> struct BwdParams {
@@ -109,7 +111,18 @@ template <typename T> struct BwdParams {
   int combine, stiff_grad, skip_layer;
   T jm, js, mf;
   int steps;
+  const T *damp, *rdamp; // the damping, and the reverse march's (or null)
+  T dt;
 };
+
+// the damped update (cuwave/kernels/scalar.cu:455-459): the step
+// (2 u1 - u0 (1 - beta) + minv L u1) / (1 + beta), beta = minv d dt / 2,
+// rounded as the reference's expression is
+template <typename T, typename Q>
+__device__ __forceinline__ Q damped(Q uc, Q u0, Q mlap, T mi, T d, T dt) {
+  const T beta = (T)0.5 * mi * d * dt;
+  return ((T)2 * uc - ((T)1 - beta) * u0 + mlap) / ((T)1 + beta);
+}
 
 // ------------------------------------------------------------------ helpers
 // the cell flux divergence of one axis over taps t[0 .. 2R] centred on R,
@@ -467,7 +480,7 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
     r0 = p.rec.starts[tile];
     r1 = p.rec.starts[tile + 1];
   }
-  if (layer && p.strip.map)
+  if ((layer || p.damp) && p.strip.map) // a damped node may be a strip node
     col = p.strip.map[idx];
   const T *__restrict__ u1 = p.u.at(t - 1);
   Q taps[NDIM][2 * R + 1];
@@ -486,7 +499,9 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
     div[d] = flux<T, R>(taps[d], gp[d], gm[d], g.op);
   const Q lap = combine_axes<T, NDIM, R>(div, g.f);
   const Q uc = taps[0][R];
-  Q v = -lanes_at<Q>(p.u.at(t - 2), e) + (T)2 * uc + mi * lap;
+  Q v = p.damp ? damped<T, Q>(uc, lanes_at<Q>(p.u.at(t - 2), e), mi * lap, mi,
+                              p.damp[idx], p.dt)
+               : -lanes_at<Q>(p.u.at(t - 2), e) + (T)2 * uc + mi * lap;
   if (held)
     v = (T)0;
   if (layer) {
@@ -608,7 +623,7 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
   // The loads that need only the node come first, so that they are all in
   // flight together rather than one branch after another.
   int strip_col = -1;
-  if (p.reverse && layer && p.strip.map)
+  if (p.reverse && (layer || p.damp) && p.strip.map)
     strip_col = p.strip.map[idx];
   int r0 = 0, r1 = 0, s0 = 0, s1 = 0;
   if (p.rec.starts) {
@@ -759,7 +774,8 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
       }
     }
   }
-  Q ln = -lo_ + (T)2 * lc + mi * div_l;
+  Q ln = p.damp ? damped<T, Q>(lc, lo_, mi * div_l, mi, p.damp[idx], p.dt)
+                : -lo_ + (T)2 * lc + mi * div_l;
   if (held)
     ln = (T)0;
   // the inertia density by parts in time (scalar_sensitivity.cu:512-514)
@@ -876,7 +892,9 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
         div[d] = flux<T, R>(taps[d], p.cell[d * g.nodes + idx],
                             p.cell[d * g.nodes + idx - g.s[d]], g.op);
       const Q lap = combine_axes<T, NDIM, R>(div, g.f);
-      v = -lanes_at<Q>(uo, e) + (T)2 * uc + mi * lap;
+      v = p.rdamp ? damped<T, Q>(uc, lanes_at<Q>(uo, e), mi * lap, mi,
+                                 p.rdamp[idx], p.dt)
+                  : -lanes_at<Q>(uo, e) + (T)2 * uc + mi * lap;
       if (held)
         v = (T)0;
       for (int k = s0; k < s1; ++k)

@@ -52,7 +52,10 @@ class Propagator:
         # then the inverse inertia, one block the kernels read through one
         # buffer resource
         self.prep = torch.empty((sim.ndim + 1, *sim.Nx_padded), dtype=dtype, device=self.device)
-        self.mat = _abi.Materials(_abi.ptr(self.stiff), _abi.ptr(self.minv), _abi.ptr(self.prep))
+        self.damping = mat.get("damping")
+        self.damping_reverse = None
+        self.mat = _abi.Materials(_abi.ptr(self.stiff), _abi.ptr(self.minv), _abi.ptr(self.prep),
+                                  _abi.ptr(self.damping), None, float(sim.dt))
         _abi.call("cw_scalar_prepare", ctypes.byref(self.grid), ctypes.byref(self.mat), _abi.stream())
         tiles = _abi.I32()
         _abi.call("cw_tile_count", ctypes.byref(self.grid), ctypes.byref(tiles))
@@ -72,6 +75,14 @@ class Propagator:
                 self.pml.profile[d] = _abi.ptr(profile)
                 plane = self.field // sim.Nx_padded[d]
                 self.slabs.append((d, (lo + hi) * plane))
+
+    def reverse_damping(self, field: torch.Tensor | None) -> None:
+        """The damping the reverse march of a reconstruction steps with (the field
+        negated where it runs backwards), or None for the forward field."""
+        if field is not None:
+            self._require_shape("reverse damping", field, tuple(self.sim.Nx_padded))
+        self.damping_reverse = field
+        self.mat.damping_reverse = _abi.ptr(field)
 
     # ------------------------------------------------------------ memory
     # Per-shot arrays end in the lanes axis: one node's shots side by side.
@@ -342,7 +353,8 @@ class Propagator:
 
 
 # ------------------------------------------------------------------ shot points
-def source_points(prop: Propagator, mat: dict, sources) -> tuple[PointList, torch.Tensor]:
+def source_points(prop: Propagator, mat: dict, sources,
+                  damping="sim") -> tuple[PointList, torch.Tensor]:
     """Every shot's source points and one (N, columns) signal table, shot b's columns its own.
 
     The points of all shots share one list: each entry names its shot, so a
@@ -362,7 +374,7 @@ def source_points(prop: Propagator, mat: dict, sources) -> tuple[PointList, torc
         lins.append(lin)
         shots.append(np.full(len(lin), b, dtype=np.int32))
         cols.append(column + np.arange(len(lin), dtype=np.int32))
-        weights.append(sim.excitation_weights(mat, lin))
+        weights.append(sim.excitation_weights(mat, lin, damping=damping))
         signals.append(np.asarray(sig[: sim.N], dtype=sim.np_dtype))
         column += len(lin)
     table = np.ascontiguousarray(np.concatenate(signals, axis=1), dtype=sim.np_dtype)

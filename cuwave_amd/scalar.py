@@ -42,6 +42,7 @@ class PressureWave(Simulation):
 
     derive_inertia = False  # set where m == k: minv derived from stiff
     gradient_names = ("mass", "stiff")
+    accepts_damping = True
 
     def parametrization(self, indicator: np.ndarray):
         raise NotImplementedError
@@ -70,6 +71,9 @@ class PressureWave(Simulation):
             "stiff": torch.as_tensor(np.ascontiguousarray(stiff), device=device),
             "minv": None if minv is None else torch.as_tensor(np.ascontiguousarray(minv), device=device),
         }
+        if self.damping is not None:  # the field as given (cuwave/scalar.py:407-415)
+            mat["damping_host"] = self.damping
+            mat["damping"] = torch.as_tensor(self.damping, device=device)
         return mat
 
     def inverse_inertia(self, indicator) -> np.ndarray:
@@ -77,12 +81,18 @@ class PressureWave(Simulation):
         stiff, minv = self.parametrization(np.asarray(as_numpy(indicator), dtype=self.np_dtype))
         return 1.0 / stiff if minv is None else minv
 
-    def excitation_weights(self, mat: dict, lin: np.ndarray) -> np.ndarray:
-        """Source weights dt^2 source_factor / inertia at `lin` (cuwave/scalar.py:858-872)."""
+    def excitation_weights(self, mat: dict, lin: np.ndarray, damping="sim") -> np.ndarray:
+        """Source weights dt^2 source_factor / inertia at `lin`, over 1 + beta where the
+        step is damped, since the damped update divides by it (cuwave/scalar.py:857-872);
+        `damping` overrides the simulation's field (the reverse march's)."""
+        damping = self.damping if isinstance(damping, str) else damping
         field = mat["stiff_host"] if self.derive_inertia else mat["minv_host"]
         weight = field.ravel()[lin]
         if self.derive_inertia:
             weight = 1.0 / weight
+        if damping is not None:
+            beta = 0.5 * weight * np.asarray(damping).ravel()[lin] * self.dt
+            weight = weight / (1.0 + beta)
         return (self.np_dtype(self.dt**2 * self.source_factor()) * weight).astype(self.np_dtype)
 
     def adjoint_weights(self, sensors) -> np.ndarray:

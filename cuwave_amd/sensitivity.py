@@ -121,14 +121,41 @@ def reconstruction_nodes(sim: Simulation, indicator=None, span: int | None = Non
     """
     inside = np.zeros(sim.Nx_padded, dtype=bool)
     inside[tuple(slice(1, n - 1) for n in sim.Nx)] = True
-    if sim.pml is None:
+    if sim.pml is None and sim.damping is None:
         return np.zeros((sim.ndim, 0), dtype=np.int32), inside
-    exact = inside & ~pml_nodes(sim)
+    exact = inside.copy()
+    if sim.pml is not None:
+        exact &= ~pml_nodes(sim)
+    if sim.damping is not None:
+        # a damped node runs back only where its growth stays bounded over the span
+        span = sim.N if span is None else span
+        exact &= ~(sim.damping > 0) | reversible_damping(sim, indicator, span)
     # a box maximum is the dilation by the full square, taken `reach` times over
     seen = ndi.maximum_filter(exact.astype(np.uint8), size=2 * sim.reach + 1,
                               mode="constant", cval=0).astype(bool)
     record = np.argwhere(inside & ~exact & seen).T.astype(np.int32)
     return record, exact
+
+
+REVERSIBLE_GROWTH = {"float32": 1e2, "float64": 1e6}  # how far a reverse march may amplify
+
+
+def reversible_damping(sim: Simulation, indicator, span: int) -> np.ndarray:
+    """Damped nodes a reverse march of `span` steps can run backwards
+    (cuwave/sensitivity.py:139-157). Solved for its trailing slot the damped update
+    is the update with the damping negated, exactly invertible; what running it back
+    costs is the decay turned growth, (1 + beta) / (1 - beta) a step for the static
+    mode, which must stay under REVERSIBLE_GROWTH over the span. beta is taken at the
+    largest inverse inertia of the node's 3^ndim box."""
+    if sim.damping is None:
+        return np.zeros(sim.Nx_padded, dtype=bool)
+    minv = ndi.maximum_filter(sim.inverse_inertia(indicator), size=3)
+    beta = 0.5 * sim.dt * sim.damping * minv
+    tiny = np.finfo(sim.np_dtype).tiny
+    with np.errstate(over="ignore"):  # beta >= 1 overflows to inf: irreversible
+        growth = span * np.log((1.0 + beta) / np.maximum(1.0 - beta, tiny))
+    bound = math.log(REVERSIBLE_GROWTH[sim.precision])
+    return (sim.damping > 0) & (beta < 1.0) & (growth <= bound)
 
 
 def _reversible_segments(snapshots: int, replays: int) -> int:
@@ -271,7 +298,10 @@ class _Reconstruction:
     """
 
     def __init__(self, sim, mat, sources, sensors, strip_dtype=None, checkpoints=None,
-                 illum=None):
+                 illum=None, indicator=None):
+        if sim.damping is not None and indicator is None:
+            raise ValueError("a damped reconstruction needs the indicator (its inertia "
+                             "scales how far the damping can be run back)")
         self.sim, self.mat = sim, mat
         self.illum = illum  # a per-node accumulator the first forward pass adds u^2 into
         self.prop = prop = Propagator(sim, mat, len(sources))
@@ -286,20 +316,33 @@ class _Reconstruction:
             raise ValueError(f"strip_dtype {strip_dtype!r}: None or 'bfloat16'")
         if self.bf16 and sim.precision != "float32":
             raise NotImplementedError("a bfloat16 strip is built for float32 runs")
-        strip, self.valid = reconstruction_nodes(sim)
-        self.count = strip.shape[1]
-        smap = np.full(sim.Nx_padded, -1, dtype=np.int32)
-        if self.count:
-            smap.ravel()[sim.flatten_indices(strip)] = np.arange(self.count, dtype=np.int32)
-        self.strip_map = torch.as_tensor(smap, device=dev)
+        strip, self.valid = reconstruction_nodes(sim, indicator)
         self.U = prop.levels_memory(2)
         self.aux = prop.aux_memory()
         itemsize = 2 if self.bf16 else np.dtype(sim.np_dtype).itemsize
         snapshot = 2 * self.U[0].nbytes + sum(a.nbytes for a in self.aux)
         segment = sim.N
-        if checkpoints is not None and self.count:
+        if checkpoints is not None and strip.shape[1]:
             # the strip of a segment costs about one snapshot (cuwave/sensitivity.py:575-583)
-            segment = min(sim.N, max(2, snapshot // (2 * self.count * itemsize * prop.lanes)))
+            segment = min(sim.N, max(2, snapshot // (2 * strip.shape[1] * itemsize * prop.lanes)))
+        if segment < sim.N:  # a damped node reverses over a segment more easily
+            strip, self.valid = reconstruction_nodes(sim, indicator, segment)
+        if not self.valid.any():
+            raise ValueError("damping too strong to reverse covers every interior node, so "
+                             "there is nothing to reconstruct; set checkpoints, damp only the "
+                             "faces with boundary.sponge, or use sensitivity")
+        self.count = strip.shape[1]
+        smap = np.full(sim.Nx_padded, -1, dtype=np.int32)
+        if self.count:
+            smap.ravel()[sim.flatten_indices(strip)] = np.arange(self.count, dtype=np.int32)
+        self.strip_map = torch.as_tensor(smap, device=dev)
+        self.src_back = self.src
+        if sim.damping is not None:
+            # backwards a reversible damping is negated (cuwave/sensitivity.py:592-598);
+            # the reverse march's sources share its divisor
+            back = np.where(self.valid, -sim.damping, sim.damping).astype(sim.np_dtype)
+            prop.reverse_damping(torch.as_tensor(back, device=dev))
+            self.src_back, _ = source_points(prop, mat, sources, damping=back)
         self.segment = segment
         self.checkpoints = checkpoints or 0
         self.starts = list(range(0, sim.N, segment)) + [sim.N]
@@ -391,8 +434,9 @@ class _Reconstruction:
             self.adj = sensor_points(prop, self.sensors, weight=sim.excitation_weights(self.mat, lin))
         if s == 0:
             self.seed = self._valid_norm(self.U[(e - 1) % 2])
-        common = dict(eta=self.eta, sensors=self.adj, adjoint=self.adjoint, sources=self.src,
-                      signal=self.signal, strip=strip, **self.gradient_mode)
+        common = dict(eta=self.eta, sensors=self.adj, adjoint=self.adjoint,
+                      sources=self.src_back, signal=self.signal, strip=strip,
+                      **self.gradient_mode)
         # the top level reads u^(e - 1) as the forward left it; below it the
         # reverse march rebuilds u^(n - 1) one level ahead
         prop.backward(self.lam, self.U, e - 1, e, reverse=False, **common)
@@ -432,7 +476,8 @@ def reconstruction_sensitivity(sim: Simulation, source: Source, indicator, senso
     if checkpoints is not None and checkpoints < 0:
         raise ValueError(f"checkpoints must be non-negative: {checkpoints}")
     mat = sim.build_materials(indicator)
-    run = _Reconstruction(sim, mat, [source], sensors, strip_dtype, checkpoints)
+    run = _Reconstruction(sim, mat, [source], sensors, strip_dtype, checkpoints,
+                          indicator=indicator)
     dev = run.dev
     acc = {k: run.prop.accumulator() for k in ("mass", "stiff")}
 
@@ -477,10 +522,11 @@ def superposition_sensitivity(sim: Simulation, source: Source, indicator, sensor
     if _elastic(sim):
         raise NotImplementedError("superposition_sensitivity is built for the pressure family; "
                                   "use sensitivity or reconstruction_sensitivity")
-    if sim.pml is not None:
+    if sim.pml is not None or sim.damping is not None:
         raise NotImplementedError(
-            "superposition_sensitivity runs the field backwards, which a pml does not allow; "
-            "use sensitivity or reconstruction_sensitivity"
+            "superposition_sensitivity runs the field backwards, which only a lossless "
+            "operator allows (no pml, no damping); use sensitivity or "
+            "reconstruction_sensitivity"
         )
     _require(sim, source, sensors)
     mat = sim.build_materials(indicator)

@@ -252,7 +252,7 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
 
     for start, chunk in _batches(sources, batch):
         run = _Reconstruction(sim, mat, chunk, sensors.nodes, checkpoints=checkpoints,
-                              illum=illum)
+                              illum=illum, indicator=indicator)
         valid = run.valid
         if constant:  # the kernel chains the Jacobian as it sums: one accumulator
             acc = run.prop.accumulator()

@@ -57,8 +57,10 @@ class Simulation:
     """Grid, timestep and stencil shared by the wave equations.
 
     `threads` is accepted for the reference's signature (its CUDA block shape);
-    the gfx950 kernels choose their own tiles. `damping`, `domain`, `dirichlet`
-    masks and `devices` are not part of this build and raise if set.
+    the gfx950 kernels choose their own tiles. `damping` is a nodal field d over
+    the padded grid (the pressure family; `boundary.sponge` builds one).
+    `domain`, `dirichlet` masks and `devices` are not part of this build and
+    raise if set.
     """
 
     Nx: tuple[int, ...]
@@ -80,6 +82,7 @@ class Simulation:
     face_conditions = (Neumann, Dirichlet)  # the conditions a face may carry
     pml_shifted = False  # whether a pml needs a frequency shift (a static mode)
     pml_walls = None  # the conditions a lined face may end on, None for any
+    accepts_damping = False  # whether the step takes a damping field
 
     @property
     def pml_margin(self) -> int:
@@ -129,7 +132,15 @@ class Simulation:
                 f"at least {self.space_order // 2 + 3} nodes an axis"
             )
         if self.damping is not None:
-            raise NotImplementedError("a damping field (sponge) is not part of this build: use pml")
+            if not self.accepts_damping:
+                raise NotImplementedError(
+                    f"{type(self).__name__} takes no damping field in this build: use pml")
+            damping = np.asarray(as_numpy(self.damping), dtype=self.np_dtype)
+            if damping.shape != tuple(self.Nx_padded):
+                raise ValueError(f"damping of shape {damping.shape}, expected {self.Nx_padded}")
+            if not np.all(np.isfinite(damping)):
+                raise ValueError("damping must be finite")
+            self.damping = np.ascontiguousarray(damping)
         if self.domain is not None or self.dirichlet is not None:
             raise NotImplementedError("domain and dirichlet masks are not part of this build")
         if self.devices is not None:
@@ -353,7 +364,7 @@ def stable_timestep(sim: Simulation, indicator, iterations: int = 60,
     """
     import dataclasses
 
-    sim = dataclasses.replace(sim, pml=None)
+    sim = dataclasses.replace(sim, pml=None, damping=None)
     field = np.random.default_rng(0).standard_normal((sim.ncomp, *sim.Nx_padded))
     mask = np.zeros(field.shape, dtype=bool)
     for c in range(sim.ncomp):

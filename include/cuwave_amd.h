@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 8
+#define CW_ABI_VERSION 9
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -71,6 +71,15 @@ typedef struct cw_materials {
   const void *minv;  /* nodal inverse inertia, or NULL to derive 1 / stiff */
   void *prep;        /* (ndim + 1, field), from prepare: the harmonic cell
                         stiffness of every axis, then the inverse inertia */
+  /* a nodal damping field d, or NULL for a lossless step: the update becomes
+   * (2 u1 - u0 (1 - beta) + minv L u1) / (1 + beta), beta = minv d dt / 2
+   * (cuwave/kernels/scalar.cu:455-459), the adjoint likewise; the 2D stream
+   * kernels are left out (every node runs the general kernels) */
+  const void *damping;
+  /* the field the reverse march steps with (the damping negated where the
+   * march runs backwards, cuwave/sensitivity.py:592-598), or NULL */
+  const void *damping_reverse;
+  double dt;
 } cw_materials;
 
 /* the convolutional PML behind the faces (cuwave/boundary.py:272-376) */

@@ -257,6 +257,11 @@ class Sim:
             np.array(minv, dtype=self.dtype))
         return mat
 
+    def inverse_inertia(self, indicator) -> np.ndarray:
+        """Nodal 1 / m from the parametrization (cuwave/scalar.py:417-420)."""
+        stiff, minv = self.parametrization(np.asarray(indicator, dtype=self.dtype))
+        return 1.0 / stiff if minv is None else minv
+
     def kernel_minv(self, mat):
         return mat["stiff"] if self.derive_inertia else mat["minv"]
 
@@ -266,13 +271,16 @@ class Sim:
             lin += np.asarray(position[d], dtype=np.int64) * self.strides[d]
         return lin.astype(np.int32)
 
-    def excitation_weights(self, mat, lin) -> np.ndarray:
+    def excitation_weights(self, mat, lin, damping="sim") -> np.ndarray:
+        """dt^2 source_factor / m at `lin`, over 1 + beta where the step is damped
+        (cuwave/scalar.py:857-872); `damping` overrides the simulation's field."""
+        damping = self.damping if isinstance(damping, str) else damping
         fld = mat["stiff"] if self.derive_inertia else mat["minv"]
         w = fld.ravel()[lin]
         if self.derive_inertia:
             w = 1.0 / w
-        if self.damping is not None:
-            beta = 0.5 * w * self.damping.ravel()[lin] * self.dt
+        if damping is not None:
+            beta = 0.5 * w * damping.ravel()[lin] * self.dt
             w = w / (1.0 + beta)
         return (self.dtype(self.dt**2 * self.source_factor()) * w).astype(self.dtype)
 
@@ -369,8 +377,11 @@ def sim_from_npz(z) -> Sim:
 class Steps:
     """The kernels of one simulation bound to its materials, as the reference's closures."""
 
-    def __init__(self, sim: Sim, mat: dict, with_pml: bool = True):
+    def __init__(self, sim: Sim, mat: dict, with_pml: bool = True, damping="sim"):
         self.sim, self.mat = sim, mat
+        # the damping field the step reads: the simulation's, or another (the
+        # reverse march's, negated where it runs backwards)
+        self.damping = sim.damping if isinstance(damping, str) else damping
         self.lib = library()
         self.suf = "_f32" if sim.dtype == np.float32 else "_f64"
         self.g = sim.grid()
@@ -389,7 +400,7 @@ class Steps:
         s = self.sim
         self.fn("fd_step")(ctypes.byref(self.g), _ptr(u0), _ptr(u1), _ptr(u2),
                            _ptr(self.mat["stiff"]), _ptr(self.minv),
-                           ctypes.c_int(s.derive_inertia), _ptr(s.damping),
+                           ctypes.c_int(s.derive_inertia), _ptr(self.damping),
                            ctypes.c_double(s.dt), self.f, ctypes.c_int(self.dirichlet))
         for (d, lo, hi, prof, _), aux in zip(self.slabs, self.memory):
             self.fn("pml_step")(ctypes.byref(self.g), _ptr(u1), _ptr(u2),
@@ -568,13 +579,39 @@ def widen(values, dtype):
     return (values.astype(np.uint32) << 16).view(np.float32).astype(dtype)
 
 
-def reconstruction_nodes(sim: Sim):
-    """The strip and the reversible mask (cuwave/sensitivity.py:160-195), no damping."""
+REVERSIBLE_GROWTH = {np.float32: 1e2, np.float64: 1e6}
+
+
+def reversible_damping(sim: Sim, indicator, span: int) -> np.ndarray:
+    """Damped nodes a reverse march of `span` steps can run backwards
+    (cuwave/sensitivity.py:139-157): the damped update solved for its trailing
+    slot is the update with the damping negated, whose growth (1 + beta) / (1 - beta)
+    a step must stay under REVERSIBLE_GROWTH over the span; beta taken at the
+    largest inverse inertia of the node's 3^ndim box."""
+    if sim.damping is None:
+        return np.zeros(sim.Nx_padded, dtype=bool)
+    minv = ndi.maximum_filter(sim.inverse_inertia(indicator), size=3)
+    beta = 0.5 * sim.dt * sim.damping * minv
+    tiny = np.finfo(sim.dtype).tiny
+    with np.errstate(over="ignore"):  # beta >= 1 overflows to inf: irreversible
+        growth = span * np.log((1.0 + beta) / np.maximum(1.0 - beta, tiny))
+    bound = math.log(REVERSIBLE_GROWTH[sim.dtype])
+    return (sim.damping > 0) & (beta < 1.0) & (growth <= bound)
+
+
+def reconstruction_nodes(sim: Sim, indicator=None, span=None):
+    """The strip and the reversible mask (cuwave/sensitivity.py:160-195): layer
+    nodes are irreversible, damped ones where `reversible_damping` says so."""
     interior = np.zeros(sim.Nx_padded, dtype=bool)
     interior[tuple(slice(1, n - 1) for n in sim.Nx)] = True
-    if sim.pml is None:
+    if sim.pml is None and sim.damping is None:
         return np.zeros((sim.ndim, 0), dtype=np.int32), interior
-    reversible = interior & ~sim.pml_nodes()
+    reversible = interior.copy()
+    if sim.pml is not None:
+        reversible &= ~sim.pml_nodes()
+    if sim.damping is not None:
+        span = sim.N if span is None else span
+        reversible &= ~(sim.damping > 0) | reversible_damping(sim, indicator, span)
     square = ndi.generate_binary_structure(sim.ndim, sim.ndim)
     reach = ndi.binary_dilation(reversible, structure=square, iterations=sim.reach,
                                 brute_force=True)
@@ -620,7 +657,6 @@ def reconstruction_sensitivity(sim: Sim, src_pos, src_sig, indicator, sensors, o
         stored = np.float32
     mat = sim.build_materials(indicator)
     st = Steps(sim, mat)
-    back = Steps(sim, mat, with_pml=False)
     src_lin = sim.flatten(src_pos)
     weight = sim.excitation_weights(mat, src_lin)
     signal = np.ascontiguousarray(src_sig, dtype=sim.dtype)
@@ -630,11 +666,21 @@ def reconstruction_sensitivity(sim: Sim, src_pos, src_sig, indicator, sensors, o
     adj_pml = AdjointPml(st, None) if st.slabs else None
     U = np.zeros((3, *sim.Nx_padded), dtype=sim.dtype)
     snapshot_bytes = 2 * U[0].nbytes + sum(a.nbytes for a in st.memory)
-    strip, valid = reconstruction_nodes(sim)
+    strip, valid = reconstruction_nodes(sim, indicator)
     segment = sim.N
     if checkpoints is not None and strip.shape[1]:
         row_bytes = 2 * strip.shape[1] * np.dtype(stored).itemsize
         segment = min(sim.N, max(2, snapshot_bytes // row_bytes))
+    if segment < sim.N:
+        strip, valid = reconstruction_nodes(sim, indicator, segment)
+    if not valid.any():
+        raise ValueError("damping too strong to reverse covers every interior node")
+    # backwards the layer sits behind the strip and a reversible damping is negated
+    back_damping = None
+    if sim.damping is not None:
+        back_damping = np.where(valid, -sim.damping, sim.damping).astype(sim.dtype)
+    back = Steps(sim, mat, with_pml=False, damping=back_damping)
+    back_weight = sim.excitation_weights(mat, src_lin, damping=back_damping)
     num_strip = strip.shape[1]
     lin_strip = sim.flatten(strip)
     starts = list(range(0, sim.N, segment)) + [sim.N]
@@ -704,7 +750,7 @@ def reconstruction_sensitivity(sim: Sim, src_pos, src_sig, indicator, sensors, o
     def reverse(m, src_row, strip_row=None):
         a, b, c = triplet(m)
         c = back.fd(b, a, c)
-        back.excite(c, src_row, src_lin, weight)
+        back.excite(c, src_row, src_lin, back_weight)
         if strip_row is not None:
             back.replay(c, strip_row, lin_strip)
         back.refill(c)

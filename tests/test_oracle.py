@@ -348,3 +348,132 @@ def test_bulk_density_gradients_match_finite_differences(ndim):
         fd = (cost(h) - cost(-h)) / (2 * h)
         adj = float(np.sum(g * direction))
         assert abs(adj - fd) <= 1e-6 * abs(fd), (which, adj, fd)
+
+
+# ------------------------------------------------------------------ damping
+# A damping field d (cuwave/scalar.py:140-160, scalar.cu:455-459): the update
+# (2 u1 - u0 (1 - beta) + minv L u1) / (1 + beta), beta = dt minv d / 2, the
+# source sharing the 1 / (1 + beta); the adjoint transposing it; a reverse
+# march stepping reversible damped nodes with the damping negated and
+# recording the rest (cuwave/sensitivity.py:139-195, 592-598).
+def _damped_sim(beta=0.03, N=90, order=4, sponge=False, pml=None, strong=None):
+    sim = D.Sim("AcousticWave", (34, 31), (1.0, 1.0), N, 0.3, "float64", order,
+                materials=(1.0, 2.5, 1.0, 4.0), pml=pml)
+    rng = np.random.default_rng(4)
+    indicator = 0.2 + 0.6 * rng.random(sim.Nx_padded)
+    minv = sim.inverse_inertia(indicator)
+    if sponge:
+        # a quadratic ramp over the six nodes behind every face
+        taper = np.zeros(sim.Nx_padded)
+        for d in range(2):
+            idx = np.arange(sim.Nx_padded[d]).reshape([-1 if k == d else 1 for k in range(2)])
+            for wall in (1, sim.Nx[d] - 2):
+                taper = np.maximum(taper, np.clip(1.0 - np.abs(idx - wall) / 6.0, 0.0, 1.0))
+        field = 2.0 * beta * taper**2 / (sim.dt * minv)
+    else:
+        field = 2.0 * beta / (sim.dt * minv) * np.ones(sim.Nx_padded)
+    if strong is not None:
+        field[strong] = 2.0 * 2.0 / (sim.dt * minv[strong])  # beta 2: no way back
+    inner = np.zeros(sim.Nx_padded, dtype=bool)
+    inner[tuple(slice(1, n - 1) for n in sim.Nx)] = True
+    sim.damping = np.where(inner, field, 0.0).astype(sim.dtype)
+    return sim, indicator
+
+
+def test_a_uniformly_damped_closed_box_sums_to_a_damped_particle():
+    # with Neumann faces sum_W L u = 0, so S = sum W m u obeys
+    # (1 + beta) S+ = 2 S - (1 - beta) S- + dt^2 sum_k W_k s_k: a wrong sign of
+    # beta, or a source missing its 1 / (1 + beta), breaks it
+    beta = 0.03
+    sim, indicator = _damped_sim(beta)
+    mat = sim.build_materials(indicator)
+    st = D.Steps(sim, mat)
+    src = np.array([[12, 20], [9, 17]])
+    lin = sim.flatten(src)
+    t = np.arange(sim.N) * sim.dt
+    sig = np.stack([np.exp(-((t - 3.0) ** 2)), -0.5 * np.exp(-((t - 4.0) ** 2))], axis=1)
+    w = sim.excitation_weights(mat, lin)
+    W = sim.apply_cell_weights(np.ones(sim.Nx_padded))
+    inner = tuple(slice(1, n - 1) for n in sim.Nx)
+    m = 1.0 / sim.inverse_inertia(indicator)
+    wsrc = W.ravel()[lin]
+    U = np.zeros((2, *sim.Nx_padded))
+    S = [0.0, 0.0]  # u^-2, u^-1 at rest
+    for k in range(sim.N):
+        u = st.fd(U[k % 2], U[1 - k % 2], U[k % 2])
+        st.excite(u, sig[k], lin, w)
+        st.refill(u)
+        S.append(float(np.sum((W * m * u)[inner])))
+        load = sim.dt**2 * float(np.sum(wsrc * sig[k]))
+        lhs = (1 + beta) * S[-1] - 2 * S[-2] + (1 - beta) * S[-3]
+        assert abs(lhs - load) <= 1e-10 * max(abs(S[-1]), 1.0), (k, lhs, load)
+    steps = np.diff(S)  # no restoring force: the velocity decays, S settles
+    assert abs(steps[-1]) < 0.1 * np.abs(steps).max()
+
+
+def _damped_fd(sim, indicator, adjoint, nodes, **kw):
+    src = np.array([[12], [15]])
+    t = np.arange(sim.N) * sim.dt
+    sig = np.exp(-((t - 3.0) ** 2))[:, None]
+    sensors = np.array([[8, 20, 25], [9, 16, 24]])
+    obs = D.simulate(sim, src, sig, indicator, sensors)[1] * 0.7
+    objective = D.l2_misfit(obs)
+    _, grads, _, info = adjoint(sim, src, sig, indicator, sensors, objective, **kw)
+    jm, js = sim.parametrization_jacobian()
+    gradient = jm * grads["mass"] + js * grads["stiff"]
+    for node in nodes:
+        h = 1e-6
+        base = indicator[node]
+
+        def cost():
+            return objective(D.simulate(sim, src, sig, indicator, sensors)[1])[0]
+
+        indicator[node] = base + h
+        plus = cost()
+        indicator[node] = base - h
+        minus = cost()
+        indicator[node] = base
+        fd = (plus - minus) / (2 * h)
+        assert abs(gradient[node] - fd) <= 1e-6 * max(abs(fd), 1e-300), (node, gradient[node], fd)
+    return gradient, info
+
+
+def test_damped_gradients_match_finite_differences():
+    # the damping is fixed (not scaled by the perturbed indicator), so the
+    # central difference sees only the material the gradient is taken for
+    nodes = [(11, 9), (17, 14), (13, 11), (22, 15), (3, 4)]
+    sim, indicator = _damped_sim(0.02)
+    _damped_fd(sim, indicator, D.sensitivity, nodes)
+    sim, indicator = _damped_sim(0.05, sponge=True)
+    g_full, _ = _damped_fd(sim, indicator, D.sensitivity, nodes)
+    for checkpoints in (None, 2):
+        g, info = _damped_fd(sim, indicator, D.reconstruction_sensitivity, nodes[:4],
+                             checkpoints=checkpoints)
+        valid = info["valid"]
+        assert valid[11, 9] and not valid[1, 1] is None
+        np.testing.assert_allclose(g[valid], g_full[valid], rtol=1e-7,
+                                   atol=1e-10 * np.abs(g_full).max())
+
+
+def test_damping_too_lossy_to_reverse_goes_to_the_strip():
+    # a patch damped at beta = 2 cannot be marched back at all: its
+    # nodes drop out of the reversible set, the ones a reversible node reads are
+    # recorded, and the rebuilt gradient still matches where it is exact
+    patch = (slice(10, 17), slice(10, 17))
+    sim, indicator = _damped_sim(0.05, sponge=True, strong=patch)
+    strip, valid = D.reconstruction_nodes(sim, indicator)
+    assert not valid[patch].any() and strip.shape[1] > 0
+    lin = set(sim.flatten(strip).tolist())
+    assert sim.flatten(np.array([[16], [13]]))[0] in lin  # the patch's last row, read across
+    assert sim.flatten(np.array([[13], [13]]))[0] not in lin  # its centre: read by no one
+    src = np.array([[12], [15]])
+    t = np.arange(sim.N) * sim.dt
+    sig = np.exp(-((t - 3.0) ** 2))[:, None]
+    sensors = np.array([[8, 20, 25], [9, 16, 24]])
+    objective = D.l2_misfit(D.simulate(sim, src, sig, indicator, sensors)[1] * 0.7)
+    _, gs, _, _ = D.sensitivity(sim, src, sig, indicator, sensors, objective)
+    _, gr, _, info = D.reconstruction_sensitivity(sim, src, sig, indicator, sensors, objective)
+    assert info["strip"] == strip.shape[1] and info["drift"] < 1e-10
+    for k in ("mass", "stiff"):
+        np.testing.assert_allclose(gr[k][valid], gs[k][valid], rtol=1e-7,
+                                   atol=1e-10 * np.abs(gs[k]).max())
