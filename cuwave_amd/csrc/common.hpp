@@ -42,6 +42,9 @@ template <typename T> struct Geom {
   T f[3];            // step factors per axis
 };
 
+// blocks of the 2D band a chunk dealt to one XCD (general_node)
+constexpr int kBandChunk = 16;
+
 // block -> work item: contiguous runs of items per XCD, the hardware dealing
 // consecutive blocks round-robin over the XCDs
 __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {

@@ -371,14 +371,25 @@ __device__ __forceinline__ bool band_node(const Region &r, const Geom<T> &g,
 }
 
 // this thread's node, every thread of a node (2^ts of them) the same: the
-// band's, or the interior's in row-major order; block `bid` of `nb`
+// band's, or the interior's in row-major order; block `bid` of `nb`.
+// The 2D band's parts cost very different amounts a node (the wall rows and
+// the slab corners several times the slab rows), so its blocks are cut into
+// chunks of kBandChunk dealt to the XCDs in turn: every XCD gets a share of
+// each part (a contiguous run each, as xcd_tile deals the rest). Measured
+// 2 % off a 1090 x 1090 adjoint level against contiguous runs.
 template <int NDIM, typename T>
 __device__ __forceinline__ bool general_node(const Region &r, const Geom<T> &g,
                                              int *a, int bid, int nb, int ts) {
   const int per = kThreads >> ts;
+  if (NDIM == 2 && r.band) {
+    // the hardware deals block bid to XCD bid % kXcds as its (bid / kXcds)-th
+    const int x = bid % kXcds, k = bid / kXcds;
+    const int chunk = (k / kBandChunk) * kXcds + x;
+    const int tb = (chunk * kBandChunk + k % kBandChunk) * per +
+                   (int)(threadIdx.x >> ts);
+    return tb < r.count && band_node(r, g, tb, a);
+  }
   const int t = xcd_tile(bid, nb) * per + (int)(threadIdx.x >> ts);
-  if (NDIM == 2 && r.band)
-    return t < r.count && band_node(r, g, t, a);
   if (t >= g.interior)
     return false;
   if (NDIM == 1) {

@@ -53,7 +53,7 @@ def test_a_convolution_and_its_gradients_match_the_library(cin, cout, relu):
     ref = O.conv3x3(ref_x, ref_w, ref_b, relu)
     # within bfloat16's rounding of the output (relative 2^-8 per element)
     err = (y.detach().float().cpu() - ref.detach()).abs()
-    assert float((err - ref.abs() * 2.0**-8).max()) <= 1e-5
+    assert float((err - ref.detach().abs() * 2.0**-8).max()) <= 1e-5
     dy = _bf(torch.randn(n, h, w, cout, generator=g))
     y.backward(dy.cuda().to(torch.bfloat16))
     ref.backward(dy)
