@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 9
+ABI_VERSION = 10
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -42,7 +42,8 @@ class Grid(ctypes.Structure):
 
 class Materials(ctypes.Structure):
     _fields_ = [("stiff", VP), ("minv", VP), ("prep", VP), ("damping", VP),
-                ("damping_reverse", VP), ("dt", F64)]
+                ("damping_reverse", VP), ("dt", F64), ("cells", VP),
+                ("rows", (F64 * 4) * 4)]
 
 
 class Pml(ctypes.Structure):

@@ -396,6 +396,8 @@ int cw_scalar_gradient(const cw_gradient_args *args, void *stream) {
   if (!args->mat.stiff || !args->u0 || !args->u1 || !args->u2 || !args->l1 ||
       !args->g_mass || !args->g_stiff)
     return fail("the field gradient needs stiff, u0, u1, u2, l1 and both accumulators");
+  if (args->mat.cells)
+    return fail("a domain mask is forward only: the adjoint kernels grade no domain wall");
   return dispatch<Gradient>(args->grid, *args, (hipStream_t)stream);
 }
 
@@ -440,6 +442,15 @@ int check_forward(const cw_forward_args &a0, int32_t t0, int32_t t1) {
     return fail("steps t0 .. t1 must lie within 0 .. steps");
   if (args->mat.damping && !(args->mat.dt > 0.0))
     return fail("a damping field needs dt > 0");
+  if (args->mat.cells) {
+    if (!args->mat.stiff)
+      return fail("a domain mask needs the stiffness its held cells read");
+    for (int d = 0; d < args->grid.ndim; ++d)
+      if (args->pml.lo[d] || args->pml.hi[d])
+        return fail("a pml takes no domain mask");
+    if (args->strip.map)
+      return fail("a domain mask takes no strip (its reverse march is not built)");
+  }
   if (args->strip.map && t0 < t1 &&
       (t0 < args->strip.first || t1 > args->strip.first + args->strip.rows))
     return fail("the strip record does not hold steps t0 .. t1");
@@ -458,6 +469,8 @@ int cw_scalar_backward(const cw_backward_args *args, int32_t n0, int32_t n1,
     return e;
   if (!args->mat.prep || !args->mat.stiff)
     return fail("backward needs stiff and the prepared materials");
+  if (args->mat.cells)
+    return fail("a domain mask is forward only: the adjoint kernels grade no domain wall");
   if (int e = check_levels(args->lam, "lambda"))
     return e;
   if (int e = check_levels(args->u, "u"))
@@ -560,6 +573,8 @@ int cw_scalar_superposed(const cw_superposed_args *args, void *stream) {
   if (!args->mat.prep || !args->u0 || !args->u1 ||
       !args->u2 || !args->acc_mass || !args->acc_stiff)
     return fail("superposed needs prep, u0, u1, u2 and both accumulators");
+  if (args->mat.cells)
+    return fail("a domain mask is forward only: the adjoint kernels grade no domain wall");
   return dispatch<Superposed>(args->grid, *args, (hipStream_t)stream);
 }
 

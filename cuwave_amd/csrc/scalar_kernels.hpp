@@ -93,6 +93,11 @@ template <typename T> struct FwdParams {
   int t_end;
   const T *damp; // nodal damping d, or null (lossless)
   T dt;
+  // a domain mask: per node the packed cell radii (read by forward_masked
+  // only), the stiffness a held cell reads, the cell weights per radius
+  const int32_t *cells;
+  const T *stiff;
+  T rows[4][4];
 };
 
 template <typename T> struct BwdParams {
@@ -123,6 +128,11 @@ __device__ __forceinline__ Q damped(Q uc, Q u0, Q mlap, T mi, T d, T dt) {
   const T beta = (T)0.5 * mi * d * dt;
   return ((T)2 * uc - ((T)1 - beta) * u0 + mlap) / ((T)1 + beta);
 }
+
+// the cell code of a node whose cells all keep the full radius, and the
+// radius code of a cell open onto a held node (cuwave/wave.py:661-662)
+constexpr int kDeepCells = 1 << 30;
+constexpr int kHeldCell = 7;
 
 // ------------------------------------------------------------------ helpers
 // the cell flux divergence of one axis over taps t[0 .. 2R] centred on R,
@@ -341,6 +351,55 @@ __device__ __forceinline__ void add_gradient(const BwdParams<T> &p,
   }
 }
 
+// the flux divergence of a node at a domain wall (cuwave/kernels/scalar.cu:
+// 85-158): along each axis the cell to the next node at radius rp and the
+// cell to the previous one at radius rm, read from `code`; a closed cell
+// (radius 0) carries no flux, a cell open onto a held node (kHeldCell) the
+// difference to the zero halfway across at the node's own stiffness sc.
+// A cell of radius r reads the radius-r weights and no node past r, so no
+// node outside the stepped set. Plain sums, the axes added in order, as the
+// reference rounds its graded cells; gp, gm are the harmonic cell means.
+template <typename T, int NDIM, int R, typename Q>
+__device__ __forceinline__ Q graded(const T *__restrict__ u1, int64_t e,
+                                    int sh, const int *s, int code, Q uc,
+                                    T sc, const T *gp, const T *gm,
+                                    const T *f, const T (*rows)[4]) {
+  Q sum = (T)0;
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    const int64_t se = (int64_t)s[d] << sh;
+    const int rp = (code >> (6 * d)) & 7, rm = (code >> (6 * d + 3)) & 7;
+    Q Dp = (T)0, Dm = (T)0;
+    if (rp == kHeldCell) {
+      Dp = -uc * sc;
+    } else if (rp) {
+      const T *w = rows[rp - 1];
+      Dp = w[0] * (lanes_at<Q>(u1, e + se) - uc);
+#pragma unroll
+      for (int k = 2; k <= R; ++k)
+        if (k <= rp)
+          Dp = Dp + w[k - 1] * (lanes_at<Q>(u1, e + k * se) -
+                                lanes_at<Q>(u1, e - (k - 1) * se));
+      Dp = Dp * gp[d];
+    }
+    if (rm == kHeldCell) {
+      Dm = uc * sc;
+    } else if (rm) {
+      const T *w = rows[rm - 1];
+      Dm = w[0] * (uc - lanes_at<Q>(u1, e - se));
+#pragma unroll
+      for (int k = 2; k <= R; ++k)
+        if (k <= rm)
+          Dm = Dm + w[k - 1] * (lanes_at<Q>(u1, e + (k - 1) * se) -
+                                lanes_at<Q>(u1, e - k * se));
+      Dm = Dm * gm[d];
+    }
+    const Q term = f[d] * (Dp - Dm);
+    sum = d == 0 ? term : sum + term;
+  }
+  return sum;
+}
+
 // ------------------------------------------------------------ node mapping
 // the 2D band node t, or false past the band
 template <typename T>
@@ -443,8 +502,10 @@ __global__ __launch_bounds__(kThreads) void prepare_kernel(
 }
 
 // ------------------------------------------------------------ forward step
-// step t at the nodes of general block `bid` of `nb`, V shot lanes a thread
-template <typename T, int NDIM, int R, int V>
+// step t at the nodes of general block `bid` of `nb`, V shot lanes a thread;
+// MASK reads the domain mask's cells: a node outside the stepped set is left
+// as it is, a node at a domain wall takes the graded flux divergence
+template <typename T, int NDIM, int R, int V, bool MASK = false>
 __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
                                                 const int t, int bid, int nb) {
   using Q = LanesOf<T, V>;
@@ -462,6 +523,12 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
     idx += (int64_t)a[d] * g.s[d];
     held |= held_wall(a[d], g.n[d], g.dirichlet >> (2 * d));
     deep &= a[d] >= R && a[d] <= g.n[d] - 1 - R;
+  }
+  int code = kDeepCells;
+  if constexpr (MASK) {
+    code = p.cells[idx];
+    if (!code)
+      return; // outside the domain, or held: never stepped
   }
   const int64_t e = (idx << sh) + b;
   const T mi = p.inv[idx];
@@ -504,12 +571,18 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
                             : mirrored<Q>(u1, e, se, a[d], g.n[d],
                                           g.dirichlet >> (2 * d), j);
   }
-  Q div[NDIM];
-#pragma unroll
-  for (int d = 0; d < NDIM; ++d)
-    div[d] = flux<T, R>(taps[d], gp[d], gm[d], g.op);
-  const Q lap = combine_axes<T, NDIM, R>(div, g.f);
   const Q uc = taps[0][R];
+  Q lap;
+  if (MASK && !(code & kDeepCells)) {
+    lap = graded<T, NDIM, R, Q>(u1, e, sh, g.s, code, uc, p.stiff[idx], gp,
+                                gm, g.f, p.rows);
+  } else {
+    Q div[NDIM];
+#pragma unroll
+    for (int d = 0; d < NDIM; ++d)
+      div[d] = flux<T, R>(taps[d], gp[d], gm[d], g.op);
+    lap = combine_axes<T, NDIM, R>(div, g.f);
+  }
   Q v = p.damp ? damped<T, Q>(uc, lanes_at<Q>(p.u.at(t - 2), e), mi * lap, mi,
                               p.damp[idx], p.dt)
                : -lanes_at<Q>(p.u.at(t - 2), e) + (T)2 * uc + mi * lap;

@@ -551,6 +551,17 @@ __global__ __launch_bounds__(kThreads) void forward_step(const FwdParams<T> p,
                                  gridDim.x - streams);
 }
 
+// a step of a grid under a domain mask: every interior node by the general
+// kernels, each reading its cell code (see forward_general)
+template <typename T, int NDIM, int R, int V>
+__global__ __launch_bounds__(kThreads) void forward_masked(const FwdParams<T> p,
+                                                           const int step) {
+  const int t = clocked_step(p, step);
+  if (t < 0)
+    return;
+  forward_general<T, NDIM, R, V, true>(p, t, blockIdx.x, gridDim.x);
+}
+
 // The one-lane float 3D sweep is asked for four blocks a CU: its registers
 // then just fit four waves a SIMD (the default packing left it one register
 // over, at three), which measured 18 % faster; every other build keeps the

@@ -203,6 +203,7 @@ class ElasticWave(Simulation):
     def build_materials(self, indicator) -> dict:
         """The design field on the host and on the GPU: the kernels average it onto the
         component and stress points (cuwave/elastic.py:305-311)."""
+        self.check_masks()  # masks set after construction raise here
         gamma = np.ascontiguousarray(as_numpy(indicator), dtype=self.np_dtype)
         if gamma.shape != self.indicator_shape():
             raise ValueError(f"indicator of shape {gamma.shape}, expected {self.indicator_shape()}")

@@ -20,6 +20,7 @@ import torch
 
 from . import _abi
 from .boundary import pml_profiles, pml_widths
+from .stencils import cell_rows
 
 
 @dataclass
@@ -54,8 +55,13 @@ class Propagator:
         self.prep = torch.empty((sim.ndim + 1, *sim.Nx_padded), dtype=dtype, device=self.device)
         self.damping = mat.get("damping")
         self.damping_reverse = None
+        self.cells = mat.get("cells")  # a domain mask's packed cell radii, or None
         self.mat = _abi.Materials(_abi.ptr(self.stiff), _abi.ptr(self.minv), _abi.ptr(self.prep),
-                                  _abi.ptr(self.damping), None, float(sim.dt))
+                                  _abi.ptr(self.damping), None, float(sim.dt),
+                                  _abi.ptr(self.cells))
+        for r, row in enumerate(cell_rows(sim.space_order, sim.stencil)):
+            for k, w in enumerate(row):
+                self.mat.rows[r][k] = float(w)
         _abi.call("cw_scalar_prepare", ctypes.byref(self.grid), ctypes.byref(self.mat), _abi.stream())
         tiles = _abi.I32()
         _abi.call("cw_tile_count", ctypes.byref(self.grid), ctypes.byref(tiles))
@@ -360,7 +366,7 @@ def source_points(prop: Propagator, mat: dict, sources,
     The points of all shots share one list: each entry names its shot, so a
     launch adds shot b's sources to shot b's field only (cuwave/wave.py:985-993).
     """
-    from .wave import as_numpy
+    from .wave import as_numpy, require_stepped
 
     sim = prop.sim
     lins, shots, cols, weights, signals = [], [], [], [], []
@@ -368,6 +374,7 @@ def source_points(prop: Propagator, mat: dict, sources,
     for b, source in enumerate(sources):
         pos = np.asarray(as_numpy(source.position))
         lin = sim.flatten_indices(pos)
+        require_stepped(sim, pos, "a source")
         sig = as_numpy(source.signal)
         if sig.ndim != 2 or sig.shape[0] < sim.N or sig.shape[1] != len(lin):
             raise ValueError(f"shot {b}: signal needs ({sim.N}, {len(lin)}), not {sig.shape}")
@@ -389,7 +396,10 @@ def sensor_points(prop: Propagator, sensors, weight=None) -> PointList:
     They write (or, weighted, read) an (N, sensors, lanes) table: the sensor
     record, or the adjoint signal `lane_table` lays out.
     """
+    from .wave import require_stepped
+
     lin = prop.sim.flatten_indices(sensors)
+    require_stepped(prop.sim, sensors, "a sensor")
     count, lanes = len(lin), prop.lanes
     return prop.point_list(lin, -1, np.arange(count, dtype=np.int32), count,
                            (count * lanes, lanes, 1), weight=weight)

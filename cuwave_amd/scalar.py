@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from . import _abi
-from .wave import Simulation, as_numpy
+from .wave import Simulation, as_numpy, domain_cells
 
 
 def mirror_ghosts(sim: Simulation, field: np.ndarray) -> np.ndarray:
@@ -43,6 +43,7 @@ class PressureWave(Simulation):
     derive_inertia = False  # set where m == k: minv derived from stiff
     gradient_names = ("mass", "stiff")
     accepts_damping = True
+    accepts_domain = True
 
     def parametrization(self, indicator: np.ndarray):
         raise NotImplementedError
@@ -74,6 +75,8 @@ class PressureWave(Simulation):
         if self.damping is not None:  # the field as given (cuwave/scalar.py:407-415)
             mat["damping_host"] = self.damping
             mat["damping"] = torch.as_tensor(self.damping, device=device)
+        if self.masked:  # the packed cell radii the masked step reads
+            mat["cells"] = torch.as_tensor(domain_cells(self), device=device)
         return mat
 
     def inverse_inertia(self, indicator) -> np.ndarray:

@@ -91,6 +91,14 @@ def _require(sim: Simulation, source: Source, sensors) -> None:
     sim.require_interior(source.position, "source")
 
 
+def require_full_domain(sim: Simulation) -> None:
+    """Raise if `sim.domain` or `sim.dirichlet` is set: the adjoint kernels grade
+    no domain wall (cuwave/sensitivity.py:121-126)."""
+    if sim.masked:
+        raise NotImplementedError(
+            "the material gradients step every node: unset domain and dirichlet")
+
+
 def _finalize(sim: Simulation, grads: dict, device) -> dict:
     """Weight the accumulators by the cell weights W (cuwave/wave.py:336-344)."""
     w = torch.as_tensor(sim.cell_weights(), device=device)
@@ -220,6 +228,7 @@ def sensitivity(sim: Simulation, source: Source, indicator, sensors, objective):
 
         return elastic.sensitivity(sim, source, indicator, sensors, objective)
     _require(sim, source, sensors)
+    require_full_domain(sim)
     mat = sim.build_materials(indicator)
     prop = Propagator(sim, mat, 1)
     dev = prop.device
@@ -266,6 +275,16 @@ def source_sensitivity(sim: Simulation, source: Source, indicator, sensors, obje
 
     _, um = simulate(sim, source, indicator, sensors)
     cost, dphi = objective(um)
+    if sim.masked:
+        # a mask grades its cells per node, which the transposed kernel does not
+        # read: the recursion is stepped by the forward step itself, exact where
+        # the graded operator is symmetric (order 2) (cuwave/scalar.py:321-323),
+        # the adjoint load injected at the sensors and lambda read at the
+        # sources as a forward run records
+        load = Source(sensors, adjoint_signal(sim, dphi, sensors))
+        _, lam = simulate(sim, load, indicator, source.position)
+        w = torch.as_tensor(np.asarray(sim.adjoint_weights(source.position)), device=lam.device)
+        return cost, lam.flip(0) * w, um, {}
     mat = sim.build_materials(indicator)
     prop = Propagator(sim, mat, 1)
     dev = prop.device
@@ -473,6 +492,7 @@ def reconstruction_sensitivity(sim: Simulation, source: Source, indicator, senso
         return elastic.reconstruction_sensitivity(sim, source, indicator, sensors, objective,
                                                   strip_dtype, checkpoints)
     _require(sim, source, sensors)
+    require_full_domain(sim)
     if checkpoints is not None and checkpoints < 0:
         raise ValueError(f"checkpoints must be non-negative: {checkpoints}")
     mat = sim.build_materials(indicator)
@@ -529,6 +549,7 @@ def superposition_sensitivity(sim: Simulation, source: Source, indicator, sensor
             "reconstruction_sensitivity"
         )
     _require(sim, source, sensors)
+    require_full_domain(sim)
     mat = sim.build_materials(indicator)
     prop = Propagator(sim, mat, 1)
     dev = prop.device
@@ -694,6 +715,7 @@ def spectral_sensitivity(sim: Simulation, source: Source, indicator, sensors, ob
     if _elastic(sim):
         raise NotImplementedError("spectral_sensitivity is built for the scalar equations")
     _require(sim, source, sensors)
+    require_full_domain(sim)
     require_off_held_walls(sim, np.asarray(as_numpy(source.position)))
     frequencies = np.atleast_1d(np.asarray(frequencies, dtype=np.float64))
     logical = np.zeros(sim.Nx_padded, dtype=bool)

@@ -89,6 +89,21 @@ def flux_row(space_order: int, stencil: str = "optimized") -> np.ndarray:
     return row
 
 
+def cell_rows(space_order: int, stencil: str = "optimized") -> np.ndarray:
+    """(4, 4) cell weights per radius, row r - 1 the radius-r row zero padded.
+
+    A cell graded at a domain wall reads the row of its own radius: the formal-order
+    (taylor) row below the full radius, the full radius' row as `flux_row` gives it
+    (cuwave/stencils.py:186-200).
+    """
+    R = space_order // 2
+    rows = np.zeros((4, 4))
+    for r in range(1, R):
+        rows[r - 1, :r] = cell_coefficients(r, "taylor")
+    rows[R - 1] = flux_row(space_order, stencil)
+    return rows
+
+
 def stable_dt(
     dx: tuple[float, ...],
     wavespeed: float,

@@ -23,6 +23,7 @@ from .sensitivity import (
     _zero_outside,
     l2_misfit,
     reconstruction_sensitivity,
+    require_full_domain,
     sensitivity,
 )
 from .wave import MAX_BATCH, Simulation, Source, as_numpy, simulate_shots
@@ -212,6 +213,7 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
     """
     if devices is not None:
         raise NotImplementedError("devices= shot dealing is not part of this build (one GPU)")
+    require_full_domain(sim)
     sources = list(sources)
     if type(sim).__name__ == "ElasticWave" and adjoint in (sensitivity, reconstruction_sensitivity):
         from . import elastic

@@ -59,8 +59,11 @@ class Simulation:
     `threads` is accepted for the reference's signature (its CUDA block shape);
     the gfx950 kernels choose their own tiles. `damping` is a nodal field d over
     the padded grid (the pressure family; `boundary.sponge` builds one).
-    `domain`, `dirichlet` masks and `devices` are not part of this build and
-    raise if set.
+    `domain` and `dirichlet` are boolean nodal masks over the padded grid (the
+    pressure family, forward runs only): the step updates the `domain` nodes
+    (every node by default) less the `dirichlet` ones, which stay at zero, the
+    cells at their walls graded (`domain_cells`). `devices` is not part of this
+    build and raises if set.
     """
 
     Nx: tuple[int, ...]
@@ -83,6 +86,7 @@ class Simulation:
     pml_shifted = False  # whether a pml needs a frequency shift (a static mode)
     pml_walls = None  # the conditions a lined face may end on, None for any
     accepts_damping = False  # whether the step takes a damping field
+    accepts_domain = False  # whether the step takes `domain` / `dirichlet` masks
 
     @property
     def pml_margin(self) -> int:
@@ -141,8 +145,7 @@ class Simulation:
             if not np.all(np.isfinite(damping)):
                 raise ValueError("damping must be finite")
             self.damping = np.ascontiguousarray(damping)
-        if self.domain is not None or self.dirichlet is not None:
-            raise NotImplementedError("domain and dirichlet masks are not part of this build")
+        self.check_masks()
         if self.devices is not None:
             raise NotImplementedError("the devices= axis split is not part of this build (one GPU)")
         self.comp_stride = int(np.prod(self.Nx_padded))
@@ -151,6 +154,30 @@ class Simulation:
         self.field_shape = self.Nx_padded
         if self.pml is not None:
             validate_pml(self)
+
+    @property
+    def masked(self) -> bool:
+        """Whether `domain` or `dirichlet` takes nodes out of the stepped set."""
+        return self.domain is not None or self.dirichlet is not None
+
+    def check_masks(self) -> None:
+        """Raise unless the masks are boolean nodal fields this equation can step
+        (cuwave/wave.py:505-511, cuwave/boundary.py:294-295)."""
+        if not self.masked:
+            return
+        if not self.accepts_domain:
+            raise ValueError(f"{type(self).__name__} steps every node: unset domain")
+        for name in ("domain", "dirichlet"):
+            mask = getattr(self, name)
+            if mask is None:
+                continue
+            mask = np.asarray(as_numpy(mask))
+            if mask.dtype != np.bool_:
+                raise ValueError(f"{name} must be a boolean mask, not {mask.dtype}")
+            if mask.shape != tuple(self.Nx_padded):
+                raise ValueError(f"{name} of shape {mask.shape}, expected {self.Nx_padded}")
+        if self.pml is not None:
+            raise ValueError("a pml takes neither a damping field nor a domain mask")
 
     # ----------------------------------------------------------- geometry
     @property
@@ -348,6 +375,100 @@ def illumination(sim: Simulation, sources, indicator, batch: int | None = None) 
             prop.march(U, 0, sim.N, graph_window(sim, prop.lanes), aux=prop.aux_memory(),
                        sources=src, signal=signal, illum=illum)
     return illum
+
+
+# ------------------------------------------------------------ domain masks
+DEEP = 1 << 30  # the cell code of a node whose cells all keep the full radius
+HELD = 7  # the radius code of a cell open onto a node held at zero
+
+
+def stepped_nodes(sim: Simulation) -> np.ndarray:
+    """The nodes the step updates: `domain` (every node by default) less `dirichlet`
+    (cuwave/wave.py:665-672)."""
+    stepped = np.ones(sim.Nx_padded, dtype=bool)
+    if sim.domain is not None:
+        stepped &= np.asarray(as_numpy(sim.domain), dtype=bool)
+    if sim.dirichlet is not None:
+        stepped &= ~np.asarray(as_numpy(sim.dirichlet), dtype=bool)
+    return stepped
+
+
+def _runs(inside: np.ndarray, d: int) -> tuple[np.ndarray, np.ndarray]:
+    """Per node, how many nodes after it and before it along axis d are `inside`
+    without a break: the distance to the nearest node that is not, less one
+    (the array's ends count as far away)."""
+    n = inside.shape[d]
+    shape = [1] * inside.ndim
+    shape[d] = n
+    at = np.arange(n).reshape(shape)
+    far = 2 * n
+    # the first outside node at or after each position, then strictly after
+    ahead = np.flip(np.minimum.accumulate(np.flip(np.where(inside, far, at), d), axis=d), d)
+    ahead = np.concatenate([np.take(ahead, np.arange(1, n), axis=d),
+                            np.full_like(np.take(ahead, [0], axis=d), far)], axis=d)
+    behind = np.maximum.accumulate(np.where(inside, -far, at), axis=d)
+    behind = np.concatenate([np.full_like(np.take(behind, [0], axis=d), -far),
+                             np.take(behind, np.arange(0, n - 1), axis=d)], axis=d)
+    return ahead - at - 1, at - behind - 1
+
+
+def domain_cells(sim: Simulation) -> np.ndarray:
+    """Per node the int32 code the masked step reads (cuwave/wave.py:675-726).
+
+    A cell's stencil reaches only stepped nodes, so along each axis the cell to the
+    next node has radius rp = min(wall, 1 + down, up) and the cell to the previous
+    rm = min(wall, down, 1 + up), `up` / `down` the run of stepped nodes after and
+    before the node (ghosts count as stepped, so the box walls close nothing) and
+    `wall` = min(R, a, Nx - 1 - a), the radius the box leaves at node a. A cell of
+    radius 0 leaving the stepped set is closed, unless the node across is held by
+    `dirichlet`: then it is open onto the zero halfway across (`HELD`). The code is 0
+    for a node not stepped, `DEEP` where every radius is the wall's own, and else
+    sum over axes d of (rp | rm << 3) << 6 d.
+    """
+    sim.check_masks()
+    R = sim.space_order // 2
+    interior = tuple(slice(1, n - 1) for n in sim.Nx)
+    inside = np.ones(sim.Nx_padded, dtype=bool)
+    inside[interior] = stepped_nodes(sim)[interior]
+    held = np.zeros(sim.Nx_padded, dtype=bool)
+    if sim.dirichlet is not None:
+        held[interior] = np.asarray(as_numpy(sim.dirichlet), dtype=bool)[interior]
+    code = np.zeros(sim.Nx_padded, dtype=np.int64)
+    deep = inside.copy()
+    for d in range(sim.ndim):
+        n = sim.Nx_padded[d]
+        shape = [1] * sim.ndim
+        shape[d] = n
+        a = np.arange(n).reshape(shape)
+        wall = np.minimum(R, np.minimum(a, sim.Nx[d] - 1 - a))
+        ahead, behind = _runs(inside, d)
+        up, down = np.minimum(ahead, wall), np.minimum(behind, wall)
+        rp = np.minimum(wall, np.minimum(1 + down, up))
+        rm = np.minimum(wall, np.minimum(down, 1 + up))
+        # the node across each cell held?
+        nxt = np.zeros_like(held)
+        prv = np.zeros_like(held)
+        lo = [slice(None)] * sim.ndim
+        hi = [slice(None)] * sim.ndim
+        lo[d], hi[d] = slice(0, n - 1), slice(1, n)
+        nxt[tuple(lo)] = held[tuple(hi)]
+        prv[tuple(hi)] = held[tuple(lo)]
+        rp = np.where((rp == 0) & nxt, HELD, rp)
+        rm = np.where((rm == 0) & prv, HELD, rm)
+        deep &= (rp == wall) & (rm == wall)
+        code |= (np.clip(rp, 0, 7) | np.clip(rm, 0, 7) << 3) << (6 * d)
+    code = np.where(deep, DEEP, code)
+    return np.where(inside, code, 0).astype(np.int32)
+
+
+def require_stepped(sim: Simulation, position, what: str) -> None:
+    """Raise if a point of `position` sits on a node the masked step never updates
+    (cuwave/wave.py:964-967)."""
+    if not sim.masked:
+        return
+    rows = sim.grid_rows(position).astype(np.int64)
+    if rows.size and not bool(stepped_nodes(sim)[tuple(rows)].all()):
+        raise ValueError(f"{what} reaches a node that is never stepped")
 
 
 def stable_timestep(sim: Simulation, indicator, iterations: int = 60,

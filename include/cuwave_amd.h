@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 9
+#define CW_ABI_VERSION 10
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -80,6 +80,20 @@ typedef struct cw_materials {
    * march runs backwards, cuwave/sensitivity.py:592-598), or NULL */
   const void *damping_reverse;
   double dt;
+  /* a domain mask (cuwave/wave.py:661-726), or NULL for every interior node
+   * stepped. Per node: 0 where the step leaves it as it is (outside
+   * `domain`, or held at zero by `dirichlet`); 1 << 30 where every cell
+   * keeps the full radius; else, per axis d, the radii rp and rm of its cells
+   * to the next and to the previous node packed as (rp | rm << 3) << 6 d,
+   * 0 for a closed cell (no flux) and 7 for a cell open onto a held node.
+   * Forward only: the adjoint, gradient and superposition entry points
+   * refuse it (the reference's gradients refuse a mask too,
+   * cuwave/sensitivity.py:121-126), and so does a pml or a strip. The 2D
+   * stream kernels are left out (every node runs the general kernels). */
+  const int32_t *cells;
+  /* rows[r - 1][k - 1]: weight k of a radius-r cell (cuwave/stencils.py:
+   * 186-200), the rows a graded cell of `cells` reads */
+  double rows[4][4];
 } cw_materials;
 
 /* the convolutional PML behind the faces (cuwave/boundary.py:272-376) */

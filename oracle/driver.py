@@ -17,6 +17,12 @@ on the C kernels of `oracle/scalar.c` (each restating one reference kernel):
   `:160-195`, treeverse `:229-269`, bfloat16 strips `cuwave/wave.py:844-860`
 - `superposition_sensitivity`: `cuwave/sensitivity.py:753-893`
 - `misfit_gradient`, `Sensors`, `distribute`: `cuwave/utils.py:19-24, 148-459`
+- domain masks: `stepped_nodes`, `domain_cells` `cuwave/wave.py:661-726` (node by
+  node), the graded step `cuwave/kernels/scalar.cu:85-158, 434-452`, the point
+  check `cuwave/wave.py:964-967`; pinned by the order-2 symmetry of the graded
+  operator, polynomial exactness of the graded cells, the method of images for a
+  held row, a domain covering everything being the plain step, and no node
+  outside the stepped set ever being read (tests/test_domain.py)
 
 Parity pins (tests/test_oracle.py): every entry point is held to the golden
 vectors the reference produced (`tests/golden/*.npz`), bit for bit where the
@@ -173,6 +179,8 @@ class Sim:
     materials: tuple = None  # rho1, rho2, kappa1, kappa2
     damping: np.ndarray | None = None
     pml_margin: int = 0  # undamped layer nodes before the front (elastic: radius - 1)
+    domain: np.ndarray | None = None  # nodal mask of the stepped domain, None for all
+    dirichlet: np.ndarray | None = None  # nodal mask held at zero, its cells left open
     ndim: int = field(init=False)
 
     def __post_init__(self):
@@ -194,6 +202,10 @@ class Sim:
     @property
     def reach(self) -> int:
         return self.R
+
+    @property
+    def masked(self) -> bool:
+        return self.domain is not None or self.dirichlet is not None
 
     def face_mask(self, condition) -> int:
         return sum(1 << (2 * d + s) for d, pair in enumerate(self.boundary)
@@ -373,6 +385,89 @@ def sim_from_npz(z) -> Sim:
                str(z["stencil"]), tuple(map(tuple, z["boundary"])), pml, **kw)
 
 
+# ------------------------------------------------------------- domain masks
+DEEP_CELLS, HELD_CELL = 1 << 30, 7  # cuwave/wave.py:661-662
+
+
+def stepped_nodes(sim: Sim) -> np.ndarray:
+    """`domain` (every node by default) less `dirichlet` (cuwave/wave.py:665-672)."""
+    stepped = np.ones(sim.Nx_padded, dtype=bool)
+    if sim.domain is not None:
+        stepped &= np.asarray(sim.domain, dtype=bool)
+    if sim.dirichlet is not None:
+        stepped &= ~np.asarray(sim.dirichlet, dtype=bool)
+    return stepped
+
+
+def domain_cells(sim: Sim) -> np.ndarray:
+    """The masked step's per-node cell codes (cuwave/wave.py:675-726), node by node.
+
+    At every stepped interior node and along each axis: walk outwards while the
+    nodes are inside (stepped, or a ghost or padding node, or past the array) and
+    within the box's radius `wall` = min(R, a, Nx - 1 - a), giving the runs `up` and
+    `down`; the cell to the next node takes radius min(wall, 1 + down, up), the cell
+    to the previous min(wall, down, 1 + up), and a radius 0 cell whose node across is
+    held becomes `HELD_CELL`. Every radius at `wall` is `DEEP_CELLS`, else the radii
+    packed (rp | rm << 3) << 6 d. Only interior nodes are coded (the rest 0).
+    """
+    R = sim.R
+    stepped = stepped_nodes(sim)
+    held = (np.zeros(sim.Nx_padded, dtype=bool) if sim.dirichlet is None
+            else np.asarray(sim.dirichlet, dtype=bool))
+
+    def interior(b):
+        return all(1 <= b[d] <= sim.Nx[d] - 2 for d in range(sim.ndim))
+
+    def inside(b):
+        if any(not 0 <= b[d] < sim.Nx_padded[d] for d in range(sim.ndim)):
+            return True
+        return bool(stepped[b]) if interior(b) else True
+
+    def held_at(b):
+        return interior(b) and bool(held[b])
+
+    code = np.zeros(sim.Nx_padded, dtype=np.int64)
+    for a in itertools.product(*(range(1, n - 1) for n in sim.Nx)):
+        if not stepped[a]:
+            continue
+        packed, deep = 0, True
+        for d in range(sim.ndim):
+            wall = min(R, a[d], sim.Nx[d] - 1 - a[d])
+
+            def at(k, d=d):
+                b = list(a)
+                b[d] += k
+                return tuple(b)
+
+            up = 0
+            while up < wall and inside(at(up + 1)):
+                up += 1
+            down = 0
+            while down < wall and inside(at(-down - 1)):
+                down += 1
+            rp, rm = min(wall, 1 + down, up), min(wall, down, 1 + up)
+            if rp == 0 and held_at(at(1)):
+                rp = HELD_CELL
+            if rm == 0 and held_at(at(-1)):
+                rm = HELD_CELL
+            deep &= rp == wall and rm == wall
+            packed |= (rp | rm << 3) << (6 * d)
+        code[a] = DEEP_CELLS if deep else packed
+    return code.astype(np.int32)
+
+
+def require_full_domain(sim: Sim) -> None:
+    """cuwave/sensitivity.py:121-126: the material gradients refuse a mask."""
+    if sim.masked:
+        raise NotImplementedError("the material gradients step every node: unset domain")
+
+
+def require_stepped(sim: Sim, position, what: str) -> None:
+    """cuwave/wave.py:964-967: a point on a node the masked step never updates raises."""
+    if sim.masked and not stepped_nodes(sim)[tuple(np.asarray(position))].all():
+        raise ValueError(f"{what} reaches a node that is never stepped")
+
+
 # -------------------------------------------------------------------- steps
 class Steps:
     """The kernels of one simulation bound to its materials, as the reference's closures."""
@@ -392,6 +487,7 @@ class Steps:
         self.memory = [np.zeros((3, (lo + hi) * plane), dtype=sim.dtype)
                        for _, lo, hi, _, plane in self.slabs]
         self.minv = sim.kernel_minv(mat)
+        self.cells = domain_cells(sim) if sim.masked else None
 
     def fn(self, name):
         return getattr(self.lib, f"cwo_{name}{self.suf}")
@@ -401,7 +497,8 @@ class Steps:
         self.fn("fd_step")(ctypes.byref(self.g), _ptr(u0), _ptr(u1), _ptr(u2),
                            _ptr(self.mat["stiff"]), _ptr(self.minv),
                            ctypes.c_int(s.derive_inertia), _ptr(self.damping),
-                           ctypes.c_double(s.dt), self.f, ctypes.c_int(self.dirichlet))
+                           ctypes.c_double(s.dt), self.f, ctypes.c_int(self.dirichlet),
+                           _ptr(self.cells))
         for (d, lo, hi, prof, _), aux in zip(self.slabs, self.memory):
             self.fn("pml_step")(ctypes.byref(self.g), _ptr(u1), _ptr(u2),
                                 _ptr(self.mat["stiff"]), _ptr(self.minv),
@@ -427,6 +524,9 @@ class Steps:
 # ----------------------------------------------------------------- simulate
 def simulate(sim: Sim, src_pos, src_sig, indicator, sensors=None):
     """Final interior field and (N, num_sensors) record (cuwave/wave.py:1071-1163)."""
+    require_stepped(sim, src_pos, "a source")
+    if sensors is not None:
+        require_stepped(sim, sensors, "a sensor")
     mat = sim.build_materials(indicator)
     st = Steps(sim, mat)
     src_lin = sim.flatten(src_pos)
@@ -528,6 +628,7 @@ class AdjointPml:
 
 def sensitivity(sim: Sim, src_pos, src_sig, indicator, sensors, objective):
     """Stored-history adjoint, the fused branch (cuwave/sensitivity.py:384-497)."""
+    require_full_domain(sim)
     mat = sim.build_materials(indicator)
     st = Steps(sim, mat)
     src_lin = sim.flatten(src_pos)
@@ -650,6 +751,7 @@ def checkpoint_schedule(segments, slots):
 def reconstruction_sensitivity(sim: Sim, src_pos, src_sig, indicator, sensors, objective,
                                strip_dtype=None, checkpoints=None):
     """Strip-replayed reverse march (cuwave/sensitivity.py:500-750), serial order."""
+    require_full_domain(sim)
     stored = sim.dtype
     if strip_dtype == "bfloat16":
         stored = np.uint16
@@ -804,6 +906,7 @@ def reconstruction_sensitivity(sim: Sim, src_pos, src_sig, indicator, sensors, o
 def superposition_sensitivity(sim: Sim, src_pos, src_sig, indicator, sensors, objective,
                               scale=1.0):
     """Lossless three-slot variant (cuwave/sensitivity.py:753-893), fused kernels."""
+    require_full_domain(sim)
     if sim.pml is not None or sim.damping is not None:
         raise NotImplementedError("superposition needs a lossless operator")
     mat = sim.build_materials(indicator)

@@ -161,13 +161,64 @@ static real laplacian(const grid_t *g, const real *u1, const real *stiff,
   return fmar(f[2], d[2], fmar(f[0], d[0], p));
 }
 
+/* cuwave/wave.py:661-662: the code of a node whose cells all keep the full
+ * radius, and the radius code of a cell open onto a held node */
+#define DEEP_CELLS (1 << 30)
+#define HELD_CELL 7
+
+/* cuwave/kernels/scalar.cu:85-120 flux_divergence_cells, one axis of a node
+ * at a domain wall: the cell to the next node at radius rp, to the previous
+ * at rm; 0 closed, HELD_CELL open onto the zero halfway to a held node at the
+ * node's own stiffness; plain sums (no fma) */
+static real graded_axis(const grid_t *g, const real *u1, const real *stiff,
+                        long idx, long s, real uc, real sc, real factor,
+                        int rp, int rm) {
+  real Dp = 0, Dm = 0;
+  if (rp == HELD_CELL) {
+    Dp = -uc * sc;
+  } else if (rp) {
+    const real sp = stiff[idx + s];
+    Dp = OPW(g, rp, 1) * (u1[idx + s] - uc);
+    for (int k = 2; k <= rp; ++k)
+      Dp = Dp + OPW(g, rp, k) * (u1[idx + k * s] - u1[idx - (k - 1) * s]);
+    Dp = Dp * (sc * sp / (sc + sp));
+  }
+  if (rm == HELD_CELL) {
+    Dm = uc * sc;
+  } else if (rm) {
+    const real sm = stiff[idx - s];
+    Dm = OPW(g, rm, 1) * (uc - u1[idx - s]);
+    for (int k = 2; k <= rm; ++k)
+      Dm = Dm + OPW(g, rm, k) * (u1[idx + (k - 1) * s] - u1[idx - k * s]);
+    Dm = Dm * (sc * sm / (sc + sm));
+  }
+  return factor * (Dp - Dm);
+}
+
+/* cuwave/kernels/scalar.cu:132-158 wall_laplacian: the axes in order, summed */
+static real graded_laplacian(const grid_t *g, const real *u1,
+                             const real *stiff, long idx, real uc, real sc,
+                             const real *f, int code) {
+  real sum = 0;
+  for (int d = 0; d < g->ndim; ++d) {
+    const real term =
+        graded_axis(g, u1, stiff, idx, g->s[d], uc, sc, f[d],
+                    (code >> (6 * d)) & 7, (code >> (6 * d + 3)) & 7);
+    sum = d == 0 ? term : sum + term;
+  }
+  return sum;
+}
+
 /* cuwave/kernels/scalar.cu:346-466 fd_kernel: u2 = -u0 + 2 u1 + minv L u1,
  * damped variant :455-459, Dirichlet wall nodes held at zero. u2 may alias u0
- * (the step writes in place, cuwave/wave.py:1119-1120). */
+ * (the step writes in place, cuwave/wave.py:1119-1120). `cells`, when given,
+ * is a domain mask's per-node code (cuwave/wave.py:675-726): a node of code 0
+ * is left as it is, a node at a domain wall (bit 30 clear) takes the graded
+ * laplacian (scalar.cu:434-452). */
 void FN(cwo_fd_step)(const grid_t *g, const real *u0, const real *u1, real *u2,
                      const real *stiff, const real *minv, int derive_inertia,
                      const real *damping, double dt, const double *fd,
-                     int dirichlet) {
+                     int dirichlet, const int *cells) {
   const long total = total_nodes(g);
   real f[3];
   for (int d = 0; d < g->ndim; ++d)
@@ -178,8 +229,14 @@ void FN(cwo_fd_step)(const grid_t *g, const real *u0, const real *u1, real *u2,
     node_axes(g, idx, a);
     if (!interior(g, a))
       continue;
+    const int code = cells ? cells[idx] : DEEP_CELLS;
+    if (!code)
+      continue;
     const real uc = u1[idx], sc = stiff[idx];
-    const real lap = laplacian(g, u1, stiff, idx, a, uc, sc, f, dirichlet);
+    const real lap =
+        (code & DEEP_CELLS)
+            ? laplacian(g, u1, stiff, idx, a, uc, sc, f, dirichlet)
+            : graded_laplacian(g, u1, stiff, idx, uc, sc, f, code);
     const real mi = derive_inertia ? (real)1 / sc : minv[idx];
     real v;
     if (damping) {
