@@ -243,6 +243,7 @@ inline cw::Region band_of(const cw_grid &g, const cw::Box &b) {
   r.y1 = b.y1;
   r.x0 = b.x0;
   r.x1 = b.x1;
+  r.q = g.order / 2 + 1; // the box keeps R + 2 from every wall (stream_box)
   const int W = g.n[1] - 2;
   r.count = (b.y0 - 1) * W + (g.n[0] - 1 - b.y1) * W +
             (b.y1 - b.y0) * ((b.x0 - 1) + (g.n[1] - 1 - b.x1));

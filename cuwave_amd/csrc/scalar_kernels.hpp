@@ -73,6 +73,7 @@ struct Region {
   int band;
   int y0, y1, x0, x1;
   int count; // band nodes
+  int q;     // the rows and columns by each wall that its folds reach, R + 1
 };
 
 template <typename T> struct FwdParams {
@@ -401,31 +402,53 @@ __device__ __forceinline__ Q graded(const T *__restrict__ u1, int64_t e,
 }
 
 // ------------------------------------------------------------ node mapping
-// the 2D band node t, or false past the band
+// the 2D band node t, or false past the band. The nodes within q of a wall
+// come first (the rows by the two walls, then the columns by the side walls
+// in the rows between): their mirrored taps, folds and, in the layer
+// corners, two memories make them the band's slowest nodes, and dispatched
+// first they run under the rest instead of trailing it (measured 1.9 % off a
+// 32-shot 1090 x 1090 gradient against the band in row order). Then the rest
+// of the rows above and below the box, then the side columns beside it.
 template <typename T>
 __device__ __forceinline__ bool band_node(const Region &r, const Geom<T> &g,
                                           int t, int *a) {
-  const int W = g.n[1] - 2;
-  const int top = (r.y0 - 1) * W;
-  if (t < top) {
-    a[0] = 1 + t / W;
+  const int N0 = g.n[0], N1 = g.n[1], W = N1 - 2, q = r.q;
+  const int rows = 2 * q * W;
+  if (t < rows) {
+    const int k = t / W;
+    a[0] = k < q ? 1 + k : N0 - 1 - 2 * q + k;
     a[1] = 1 + t % W;
+    return true;
+  }
+  t -= rows;
+  if (t < (N0 - 2 - 2 * q) * 2 * q) {
+    const int k = t % (2 * q);
+    a[0] = q + 1 + t / (2 * q);
+    a[1] = k < q ? 1 + k : N1 - 1 - 2 * q + k;
+    return true;
+  }
+  t -= (N0 - 2 - 2 * q) * 2 * q;
+  const int Wi = W - 2 * q; // the columns off the side walls
+  const int top = (r.y0 - 1 - q) * Wi;
+  if (t < top) {
+    a[0] = q + 1 + t / Wi;
+    a[1] = q + 1 + t % Wi;
     return true;
   }
   t -= top;
-  const int bot = (g.n[0] - 1 - r.y1) * W;
+  const int bot = (N0 - 1 - q - r.y1) * Wi;
   if (t < bot) {
-    a[0] = r.y1 + t / W;
-    a[1] = 1 + t % W;
+    a[0] = r.y1 + t / Wi;
+    a[1] = q + 1 + t % Wi;
     return true;
   }
   t -= bot;
-  const int left = r.x0 - 1, right = g.n[1] - 1 - r.x1, per = left + right;
+  const int left = r.x0 - 1 - q, right = N1 - 1 - q - r.x1, per = left + right;
   if (per <= 0 || t >= per * (r.y1 - r.y0))
     return false;
   const int k = t % per;
   a[0] = r.y0 + t / per;
-  a[1] = k < left ? 1 + k : r.x1 + (k - left);
+  a[1] = k < left ? q + 1 + k : r.x1 + (k - left);
   return true;
 }
 
