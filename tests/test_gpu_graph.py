@@ -9,7 +9,9 @@ remainder. Each march captures its own graph. Plain calls run between
 the graphed ones, so every replay runs after other work and is checked
 against the plain result. A clock at or past its end launches steps that do
 nothing. The tuner's measured choices go to its file, come back in a fresh
-read, and leave the simulated bits unchanged.
+read, and leave the simulated bits unchanged. Adjoint levels whose frame
+launches fork onto the side stream and join back capture into one graph, and
+every replay from the same state gives the eager call's bits.
 """
 
 from __future__ import annotations
@@ -142,3 +144,56 @@ def test_tuned_choices_are_kept_and_change_no_bits(tmp_path, monkeypatch):
     for a, b in zip(before, after):
         assert torch.equal(a, b)
     tuning.forget()
+
+
+@pytest.mark.parametrize("combine", [True, False])
+def test_captured_adjoint_levels_replay_the_eager_ones(combine, _march_every_box):
+    """Adjoint levels whose frame launches fork onto the side stream and join back
+    (two streams, events between them) captured into one graph: every replay, from
+    the same starting state and with eager work between replays, gives the bits of
+    the eager call."""
+    sim, indicator, sources, sensors = _case("fwd_pml_acoustic_o8_f32", 40, (134, 200), 3)
+    print(f"case captured adjoint levels combine={combine}", flush=True)
+    mat = sim.build_materials(indicator)
+    prop = Propagator(sim, mat, len(sources))
+    gen = torch.Generator(device=prop.device).manual_seed(0)
+
+    def noise(t):
+        return t.copy_(torch.randn(t.shape, generator=gen, device=t.device, dtype=t.dtype))
+
+    rec = sensor_points(prop, sensors,
+                        weight=sim.excitation_weights(mat, sim.flatten_indices(sensors)))
+    adjoint = noise(prop.record_memory(rec))
+    state = {"P": noise(prop.levels_memory(2)), "U": noise(prop.levels_memory(2)),
+             "gm": noise(prop.accumulator()), "gs": noise(prop.accumulator())}
+    for k, e in enumerate(prop.eta_memory()):
+        state[f"eta{k}"] = noise(e)
+    start = {k: v.clone() for k, v in state.items()}
+    eta = [state[f"eta{k}"] for k in range(len(prop.slabs))]
+    kw = dict(eta=eta, g_mass=state["gm"], sensors=rec, adjoint=adjoint)
+    kw.update(combine=(0.7, -0.3)) if combine else kw.update(g_stiff=state["gs"])
+
+    def levels():
+        prop.backward(state["P"], state["U"], sim.N - 9, sim.N - 1, **kw)
+
+    def restore():
+        for k, v in start.items():
+            state[k].copy_(v)
+
+    levels()
+    torch.cuda.synchronize()
+    eager = {k: v.clone() for k, v in state.items()}
+    restore()
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        levels()
+    torch.cuda.synchronize()
+    for replay in range(3):
+        restore()
+        noise(prop.levels_memory(2))  # other work between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        for k, v in eager.items():
+            assert torch.equal(state[k], v), (replay, k)
