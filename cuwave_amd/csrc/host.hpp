@@ -258,7 +258,7 @@ inline int general_blocks(const cw_grid &gr, const cw::Region &r, int V) {
     for (int d = 0; d < gr.ndim; ++d)
       nodes *= gr.n[d] - 2;
   int blocks = (int)((nodes + per - 1) / per);
-  if (r.band) // whole rounds of band chunks, one chunk an XCD (general_node)
+  if (r.band && gr.ndim == 2) // whole rounds of band chunks, one chunk an XCD (general_node)
     blocks = (blocks + cw::kXcds * cw::kBandChunk - 1) /
              (cw::kXcds * cw::kBandChunk) * (cw::kXcds * cw::kBandChunk);
   return blocks;

@@ -67,13 +67,15 @@ template <typename T> struct Strip {
   int bf16;
 };
 
-// where the general kernels run: every interior node, or (2D) the band of
-// nodes outside the stream box rows [y0, y1) x columns [x0, x1)
+// where the general kernels run: every interior node, or the band of nodes
+// outside a core box: (2D) rows [y0, y1) x columns [x0, x1), (3D) planes
+// [z0, z1) x rows [y0, y1) x columns [x0, x1)
 struct Region {
   int band;
   int y0, y1, x0, x1;
   int count; // band nodes
-  int q;     // the rows and columns by each wall that its folds reach, R + 1
+  int q;     // (2D) the rows and columns by each wall that its folds reach, R + 1
+  int z0, z1;
 };
 
 template <typename T> struct FwdParams {
@@ -452,6 +454,43 @@ __device__ __forceinline__ bool band_node(const Region &r, const Geom<T> &g,
   return true;
 }
 
+// the 3D shell node t, or false past the shell: the whole planes before and
+// after the box's, then in the box's planes the whole rows above and below
+// its rows, then in its rows the columns either side of it
+template <typename T>
+__device__ __forceinline__ bool shell_node(const Region &r, const Geom<T> &g,
+                                           int t, int *a) {
+  const int H = g.n[1] - 2, W = g.n[2] - 2;
+  const int before = r.z0 - 1, planes = before + (g.n[0] - 1 - r.z1);
+  if (t < planes * H * W) {
+    const int k = t / (H * W), rest = t % (H * W);
+    a[0] = k < before ? 1 + k : r.z1 + (k - before);
+    a[1] = 1 + rest / W;
+    a[2] = 1 + rest % W;
+    return true;
+  }
+  t -= planes * H * W;
+  const int depth = r.z1 - r.z0;
+  const int above = r.y0 - 1, rows = above + (g.n[1] - 1 - r.y1);
+  if (t < depth * rows * W) {
+    const int rest = t % (rows * W), k = rest / W;
+    a[0] = r.z0 + t / (rows * W);
+    a[1] = k < above ? 1 + k : r.y1 + (k - above);
+    a[2] = 1 + rest % W;
+    return true;
+  }
+  t -= depth * rows * W;
+  const int left = r.x0 - 1, cols = left + (g.n[2] - 1 - r.x1);
+  const int span = r.y1 - r.y0;
+  if (cols <= 0 || t >= depth * span * cols)
+    return false;
+  const int rest = t % (span * cols), k = rest % cols;
+  a[0] = r.z0 + t / (span * cols);
+  a[1] = r.y0 + rest / cols;
+  a[2] = k < left ? 1 + k : r.x1 + (k - left);
+  return true;
+}
+
 // this thread's node, every thread of a node (2^ts of them) the same: the
 // band's, or the interior's in row-major order; block `bid` of `nb`.
 // The 2D band's parts cost very different amounts a node (the wall rows and
@@ -463,6 +502,10 @@ template <int NDIM, typename T>
 __device__ __forceinline__ bool general_node(const Region &r, const Geom<T> &g,
                                              int *a, int bid, int nb, int ts) {
   const int per = kThreads >> ts;
+  if (NDIM == 3 && r.band) {
+    const int t = xcd_tile(bid, nb) * per + (int)(threadIdx.x >> ts);
+    return t < r.count && shell_node(r, g, t, a);
+  }
   if (NDIM == 2 && r.band) {
     // the hardware deals block bid to XCD bid % kXcds as its (bid / kXcds)-th
     const int x = bid % kXcds, k = bid / kXcds;
