@@ -12,7 +12,13 @@ reach (the general kernels alone): both must give the oracle's bits.
   the box and in the shell, a plane of sensors across the box, and the
   illumination map;
 - order 2 in float64 on extents that are no multiple of the block's;
-- a graphed march of a grid with a core is bitwise one plain call.
+- a graphed march of a grid with a core is bitwise one plain call;
+- the adjoint levels with a core (lambda and the reverse march marched the same
+  way) give the general kernels' bits: the stored-history and the
+  reconstruction gradients of the 3D golden case, a larger bulk-density
+  reconstruction with checkpoints and points in the box (both accumulators),
+  and a misfit gradient of two one-lane shots (one combined accumulator);
+  the larger one also against the oracle.
 """
 
 from __future__ import annotations
@@ -23,13 +29,14 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import golden, host, product_sim, rel_l2
+from helpers import golden, host, product_sim, rel_l2, rel_max
 
 pytestmark = pytest.mark.gpu
 
 from oracle import driver as D  # noqa: E402
 
-from cuwave_amd import _abi, boundary, scalar, wave  # noqa: E402
+from cuwave_amd import _abi, boundary, scalar, utils, wave  # noqa: E402
+from cuwave_amd import sensitivity as S  # noqa: E402
 from cuwave_amd.engine import Propagator, sensor_points, source_points  # noqa: E402
 
 
@@ -131,3 +138,87 @@ def test_a_graphed_3d_march_with_a_core_is_one_plain_call():
         assert torch.equal(a, b)
     for a, b in zip(runs[0], runs[2]):
         assert torch.equal(a, b)
+
+
+def _same(cored, plain):
+    """Two results of a gradient run, bit for bit: tensors, arrays, dicts of them, numbers."""
+    if isinstance(cored, dict):
+        assert cored.keys() == plain.keys()
+        for k in cored:
+            _same(cored[k], plain[k])
+    elif isinstance(cored, (tuple, list)):
+        for a, b in zip(cored, plain):
+            _same(a, b)
+    elif isinstance(cored, torch.Tensor):
+        assert torch.equal(cored, plain)
+    elif isinstance(cored, np.ndarray):
+        np.testing.assert_array_equal(cored, plain)
+    else:
+        assert cored == plain
+
+
+def test_the_3d_golden_adjoints_with_a_core():
+    z = golden("adj_pml_3d_o4_f64")
+    sim = product_sim(z)
+    print("case 3d core adjoint golden adj_pml_3d_o4_f64", flush=True)
+    src = wave.Source(z["src_pos"], z["src_sig"])
+    objective = S.l2_misfit(z["observed"])
+    for run in (S.sensitivity, S.reconstruction_sensitivity):
+        cored, plain = _both(lambda: run(sim, src, z["indicator"], z["sensors"], objective))
+        _same(cored[:3], plain[:3])
+
+
+def test_a_larger_3d_bulk_density_reconstruction_with_a_core():
+    n, width, N = 60, 8, 70
+    dx = 1e-3
+    dt = 0.4 * wave.stable_dt((dx,) * 3, 1800.0, 8)
+    rng = np.random.default_rng(5)
+    npad = wave.padded_shape((n, n, n))
+    rho = 900.0 + 400.0 * rng.random(npad)
+    kappa = rho * (1400.0 + 400.0 * rng.random(npad)) ** 2
+    stack = np.stack([kappa, rho]).astype(np.float32)
+    sim = scalar.BulkDensityWave((n, n, n), (dx,) * 3, N, dt, precision="float32",
+                                 space_order=8, pml=boundary.Pml(width, 1800.0, 1e-6))
+    osim = D.Sim("BulkDensityWave", (n, n, n), (dx,) * 3, N, dt, "float32", 8,
+                 pml=D.Pml(width, 1800.0, 1e-6))
+    t = np.arange(N) * dt
+    from cuwave_amd.signals import ricker
+
+    sig = (ricker(t, 1.0, 1e5) / dx**3)[:, None].astype(np.float32)
+    src = np.array([[n // 2], [n // 2 - 3], [n // 2 + 2]])
+    g = np.arange(width + 3, n - width - 3, 5)
+    sensors = np.stack([np.full(len(g) ** 2, n // 2 + 4), np.repeat(g, len(g)), np.tile(g, len(g))])
+    print("case 3d core bulk-density reconstruction checkpoints=2", flush=True)
+    _, observed = D.simulate(osim, src, sig, stack, sensors)
+    objective = S.l2_misfit((0.9 * observed).astype(np.float32))
+
+    def run():
+        return S.reconstruction_sensitivity(sim, wave.Source(src, sig), stack, sensors,
+                                            objective, checkpoints=2)
+
+    cored, plain = _both(run)
+    _same(cored[:3], plain[:3])
+    ocost, ograds, _, _ = D.reconstruction_sensitivity(
+        osim, src, sig, stack, sensors, D.l2_misfit((0.9 * observed).astype(np.float32)),
+        checkpoints=2)
+    assert abs(cored[0] - ocost) <= 1e-4 * abs(ocost)
+    for key in ("mass", "stiff"):
+        assert rel_max(cored[1][key], ograds[key]) < 1e-4, key
+
+
+def test_a_3d_misfit_gradient_of_one_lane_shots_with_a_core():
+    sim, _, ind, _ = _large((64, 60, 62), 8, "float32", 8, 60)
+    print("case 3d core misfit gradient", flush=True)
+    t = np.arange(sim.N)
+    sig = (np.sin(0.4 * t) * np.exp(-t / 30.0))[:, None].astype(np.float32)
+    sources = [wave.Source(np.array([[32], [30 + k], [31]]), sig * (1 + k)) for k in range(2)]
+    coords = np.stack([np.full(9, 0.5), np.repeat([0.35, 0.5, 0.65], 3),
+                       np.tile([0.35, 0.5, 0.65], 3)], 1)  # (sensors, axes)
+    sensors = utils.Sensors(sim, coords)
+    observed = [np.zeros((sim.N, 9), dtype=np.float32) for _ in sources]
+
+    def run():
+        return utils.misfit_gradient(sim, sources, ind, sensors, observed, batch=1)
+
+    cored, plain = _both(run)
+    _same(cored, plain)
