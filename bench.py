@@ -255,6 +255,28 @@ def cpu_baseline(sample_steps: int, runs: int = 3):
     return 2.0 * n_int * sample_steps / median / 1e9, median
 
 
+def cpu_baseline_3d(sample_steps: int, runs: int = 3):
+    """The oracle on the host cores over a bounded sample of the 3D workload: the
+    full 354^3 grid, its first `sample_steps` steps, forward + reconstruction
+    adjoint; the median of `runs` timed runs after one warm-up run."""
+    from oracle import driver as D
+
+    sim, indicator, source, sensors = workload_3d(sample_steps)
+    osim = D.Sim("BulkDensityWave", sim.Nx, sim.dx, sim.N, sim.dt, "float32", 8,
+                 pml=D.Pml(sim.pml.width, sim.pml.wavespeed, sim.pml.reflection))
+    _, observed = D.simulate(osim, source.position, source.signal, indicator, sensors)
+    observed = 0.9 * observed
+    times = []
+    for _ in range(runs + 1):
+        t0 = time.perf_counter()
+        D.reconstruction_sensitivity(osim, source.position, source.signal, indicator, sensors,
+                                     D.l2_misfit(observed))
+        times.append(time.perf_counter() - t0)
+    median = float(np.median(times[1:]))
+    n_int = int(np.prod([n - 2 for n in sim.Nx]))
+    return 2.0 * n_int * sample_steps / median / 1e9, median
+
+
 def probe_copy(mib: int = 256, reps: int = 5) -> None:
     """A plain device copy of `mib` MiB, `reps` times: the known-size kernel that
     calibrates the FETCH_SIZE / WRITE_SIZE counters the traffic figures use."""
@@ -358,7 +380,8 @@ def gpu_run_3d(args):
                    "shots": 1, "receivers": 1024, "time_steps": args.nt,
                    "interior_nodes": n_int, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": traffic_per_launch("traffic_adjoint_level_acoustic3d_b1.json")},
         "cpu_baseline": None,
         "kernels": {"backward_ms": bwd_ms, "forward_ms": fwd_ms,
                     "backward_GBps": achieved / 1e9,
@@ -781,7 +804,19 @@ def main():
         if args.gpus != 1:
             print(json.dumps({"error": "this build runs on one GPU"}))
             sys.exit(2)
-        print(json.dumps(gpu_run_3d(args)))
+        res = gpu_run_3d(args)
+        if not args.no_cpu:
+            cores = len(os.sched_getaffinity(0))
+            sample = 8
+            value, seconds = cpu_baseline_3d(sample)
+            res["cpu_baseline"] = {
+                "value": value, "unit": "Gcell-updates/s",
+                "cores": min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))),
+                "kind": "port",
+                "sample": f"the full 354^3 grid, first {sample} of {args.nt} steps, forward + "
+                          f"reconstruction adjoint through the C oracle, its strip set-up "
+                          f"included; median of 3 runs after a warm-up ({seconds:.1f} s a run)"}
+        print(json.dumps(res))
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the shot batch shards over "
