@@ -583,12 +583,11 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
   const int sh = g.lshift;
   const int b = (int)(threadIdx.x & ((1u << ts) - 1)) * V; // first lane
   int64_t idx = 0;
-  bool held = false, deep = true;
+  bool held = false;
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
     idx += (int64_t)a[d] * g.s[d];
     held |= held_wall(a[d], g.n[d], g.dirichlet >> (2 * d));
-    deep &= a[d] >= R && a[d] <= g.n[d] - 1 - R;
   }
   int code = kDeepCells;
   if constexpr (MASK) {
@@ -633,9 +632,10 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
     const int64_t se = (int64_t)g.s[d] << sh;
 #pragma unroll
     for (int j = -R; j <= R; ++j)
-      taps[d][j + R] = deep ? lanes_at<Q>(u1, e + j * se)
-                            : mirrored<Q>(u1, e, se, a[d], g.n[d],
-                                          g.dirichlet >> (2 * d), j);
+      taps[d][j + R] = a[d] >= R && a[d] <= g.n[d] - 1 - R
+                           ? lanes_at<Q>(u1, e + j * se)
+                           : mirrored<Q>(u1, e, se, a[d], g.n[d],
+                                         g.dirichlet >> (2 * d), j);
   }
   const Q uc = taps[0][R];
   Q lap;
@@ -1022,19 +1022,16 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
     // the lossless step run backwards: u^(n - 1) from u^n and u^(n + 1)
     Q v = replay;
     if (strip_col < 0) {
-      bool deep = true;
-#pragma unroll
-      for (int d = 0; d < NDIM; ++d)
-        deep &= a[d] >= R && a[d] <= g.n[d] - 1 - R;
       Q taps[NDIM][2 * R + 1];
 #pragma unroll
       for (int d = 0; d < NDIM; ++d) {
         const int64_t se = (int64_t)g.s[d] << sh;
 #pragma unroll
         for (int j = -R; j <= R; ++j)
-          taps[d][j + R] = deep ? lanes_at<Q>(u1, e + j * se)
-                                : mirrored<Q>(u1, e, se, a[d], g.n[d],
-                                              g.dirichlet >> (2 * d), j);
+          taps[d][j + R] = a[d] >= R && a[d] <= g.n[d] - 1 - R
+                               ? lanes_at<Q>(u1, e + j * se)
+                               : mirrored<Q>(u1, e, se, a[d], g.n[d],
+                                             g.dirichlet >> (2 * d), j);
       }
       Q div[NDIM];
 #pragma unroll
