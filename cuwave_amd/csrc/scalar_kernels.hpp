@@ -583,11 +583,12 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
   const int sh = g.lshift;
   const int b = (int)(threadIdx.x & ((1u << ts) - 1)) * V; // first lane
   int64_t idx = 0;
-  bool held = false;
+  bool held = false, deep = true;
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
     idx += (int64_t)a[d] * g.s[d];
     held |= held_wall(a[d], g.n[d], g.dirichlet >> (2 * d));
+    deep &= a[d] >= R && a[d] <= g.n[d] - 1 - R;
   }
   int code = kDeepCells;
   if constexpr (MASK) {
@@ -632,7 +633,9 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
     const int64_t se = (int64_t)g.s[d] << sh;
 #pragma unroll
     for (int j = -R; j <= R; ++j)
-      taps[d][j + R] = a[d] >= R && a[d] <= g.n[d] - 1 - R
+      // mirrored only along an axis near a wall (the same bits as along
+      // every axis, for fewer instructions at the walls)
+      taps[d][j + R] = deep || (a[d] >= R && a[d] <= g.n[d] - 1 - R)
                            ? lanes_at<Q>(u1, e + j * se)
                            : mirrored<Q>(u1, e, se, a[d], g.n[d],
                                          g.dirichlet >> (2 * d), j);
@@ -1022,13 +1025,17 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
     // the lossless step run backwards: u^(n - 1) from u^n and u^(n + 1)
     Q v = replay;
     if (strip_col < 0) {
+      bool deep = true;
+#pragma unroll
+      for (int d = 0; d < NDIM; ++d)
+        deep &= a[d] >= R && a[d] <= g.n[d] - 1 - R;
       Q taps[NDIM][2 * R + 1];
 #pragma unroll
       for (int d = 0; d < NDIM; ++d) {
         const int64_t se = (int64_t)g.s[d] << sh;
 #pragma unroll
         for (int j = -R; j <= R; ++j)
-          taps[d][j + R] = a[d] >= R && a[d] <= g.n[d] - 1 - R
+          taps[d][j + R] = deep || (a[d] >= R && a[d] <= g.n[d] - 1 - R)
                                ? lanes_at<Q>(u1, e + j * se)
                                : mirrored<Q>(u1, e, se, a[d], g.n[d],
                                              g.dirichlet >> (2 * d), j);
