@@ -944,16 +944,14 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
       const int N = g.n[d], L = p.pml.lo[d], H = p.pml.hi[d];
       const int64_t size = slab_size<NDIM>(g, d, L, H) << sh;
       const T *eta = p.pml.eta[d][n & 1] + b;
-      int at[3];
-#pragma unroll
-      for (int k = 0; k < NDIM; ++k)
-        at[k] = a[k];
+      // the slab index along axis d is base + row * stride (the other axes'
+      // offset is the node's own), so each tap costs one 32-bit product (a
+      // slab holds fewer nodes than a field, which stays under 2^31)
+      const int base = (int)slab_index<NDIM>(g, a, d, 0, L, H);
+      const int stride = (int)slab_index<NDIM>(g, a, d, 1, L, H) - base;
       Q sum = gather<T, R, Q>(eta, size, g.op, [&](int k) {
         const int row = slab_row(a[d] + k, N, L, H);
-        if (row < 0)
-          return (int64_t)-1;
-        at[d] = a[d] + k;
-        return slab_index<NDIM>(g, at, d, row, L, H) << sh;
+        return row < 0 ? (int64_t)-1 : (int64_t)(base + row * stride) << sh;
       });
 #pragma unroll
       for (int high = 0; high < 2; ++high) {
@@ -968,9 +966,8 @@ __device__ __forceinline__ void backward_general(const BwdParams<T> &p,
           const int ai = high ? N - 1 - k : k;
           const int row = slab_row(ai, N, L, H);
           on[k - 1] = row >= 0 && ghost - ai >= -R && ghost - ai <= R;
-          at[d] = ai;
           const int64_t i =
-              on[k - 1] ? slab_index<NDIM>(g, at, d, row, L, H) << sh : 0;
+              on[k - 1] ? (int64_t)(base + row * stride) << sh : 0;
           e3[k - 1] = lanes_at<Q>(eta, 3 * size + i);
           e4[k - 1] = lanes_at<Q>(eta, 4 * size + i);
         }
