@@ -18,7 +18,8 @@ reach (the general kernels alone): both must give the oracle's bits.
   reconstruction gradients of the 3D golden case, a larger bulk-density
   reconstruction with checkpoints and points in the box (both accumulators),
   and a misfit gradient of two one-lane shots (one combined accumulator);
-  the larger one also against the oracle.
+  the larger one also against the oracle; a float64 order-8 reconstruction
+  with Dirichlet faces on one axis behind the layer, also against the oracle.
 """
 
 from __future__ import annotations
@@ -222,3 +223,40 @@ def test_a_3d_misfit_gradient_of_one_lane_shots_with_a_core():
 
     cored, plain = _both(run)
     _same(cored, plain)
+
+
+def test_a_float64_order_8_reconstruction_with_held_faces_and_a_core():
+    # the float64 adjoint box at the widest stencil (its own register budget)
+    # beside Dirichlet faces on one axis behind the layer
+    Nx, N = (40, 38, 41), 50
+    dx = tuple(1.0 / (n - 3) for n in Nx)
+    dt = 0.4 * wave.stable_dt(dx, 1.0, 8)
+    faces = ((boundary.Neumann, boundary.Neumann), (boundary.Dirichlet, boundary.Dirichlet),
+             (boundary.Neumann, boundary.Neumann))
+    sim = scalar.AcousticWave(Nx, dx, N, dt, precision="float64", space_order=8, rho1=1.0,
+                              rho2=2.5, kappa1=1.0, kappa2=4.0, boundary=faces,
+                              pml=boundary.Pml(6, 2.0))
+    osim = D.Sim("AcousticWave", Nx, dx, N, dt, "float64", 8, materials=(1.0, 2.5, 1.0, 4.0),
+                 boundary=((D.NEUMANN, D.NEUMANN), (D.DIRICHLET, D.DIRICHLET),
+                           (D.NEUMANN, D.NEUMANN)),
+                 pml=D.Pml(6, 2.0))
+    rng = np.random.default_rng(9)
+    ind = 0.2 + 0.6 * rng.random(sim.Nx_padded)
+    src = np.array([[20], [19], [21]])
+    t = np.arange(N)
+    sig = (np.sin(0.5 * t) * np.exp(-t / 20.0))[:, None]
+    sensors = np.array([[12, 25, 20], [15, 20, 26], [14, 22, 18]])
+    print("case 3d core float64 order 8 reconstruction, dirichlet faces", flush=True)
+    observed = 0.9 * D.simulate(osim, src, sig, ind, sensors)[1]
+
+    def run():
+        return S.reconstruction_sensitivity(sim, wave.Source(src, sig), ind, sensors,
+                                            S.l2_misfit(observed))
+
+    cored, plain = _both(run)
+    _same(cored[:3], plain[:3])
+    ocost, ograds, _, _ = D.reconstruction_sensitivity(osim, src, sig, ind, sensors,
+                                                       D.l2_misfit(observed))
+    assert abs(cored[0] - ocost) <= 1e-10 * abs(ocost)
+    for key in ("mass", "stiff"):
+        assert rel_max(cored[1][key], ograds[key]) < 1e-10, key

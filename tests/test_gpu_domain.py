@@ -10,6 +10,7 @@ kernels with the per-node cell codes.
   a sponge-damped disk with Dirichlet faces, three shots in one batch, a larger
   2D grid;
 - a held row is the method of images (tests/wave_test.py:222-247 of the reference);
+- the illumination of a masked sweep is the oracle's, zero off the stepped set;
 - the material off the stepped set is never read: voided or filled there, the
   bits are the same (wave_test.py:248-254);
 - a domain covering everything is the plain sweep bit for bit, with the plain
@@ -28,7 +29,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import host
+from helpers import host, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -172,6 +173,16 @@ def test_a_held_row_is_the_method_of_images():
     scale = float(np.abs(imaged[below]).max())
     assert scale > 0
     np.testing.assert_allclose(one[below], imaged[below], atol=1e-12 * scale)
+
+
+def test_the_illumination_of_a_masked_sweep_is_the_oracles():
+    sim, osim, ind = _case("disk and obstacle o8 f32")
+    print("case masked illumination", flush=True)
+    shots = [(_inside_points(sim, 1, k), _signal(sim) * (1 + k)) for k in range(2)]
+    expect = D.illumination(osim, shots, ind)
+    got = wave.illumination(sim, [wave.Source(p, s) for p, s in shots], ind)
+    assert rel_l2(got, expect) < 1e-5
+    assert (host(got)[~wave.stepped_nodes(sim)] == 0).all()
 
 
 def test_the_material_off_the_domain_is_never_read():
