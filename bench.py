@@ -110,8 +110,9 @@ def velocity_of(sim, indicator):
 
 
 def gpu_run(args):
-    """The config-2 gradient: velocity -> indicator -> traces (the autograd Function over
-    every shot in one batch) -> L2 misfit -> backward to the velocity."""
+    """The 2D gradient (32 shots, 1090 x 1090, order 8): velocity -> indicator -> traces
+    (the autograd Function over every shot in one batch) -> L2 misfit -> backward to the
+    velocity."""
     import torch
 
     from cuwave_amd import autograd, wave
@@ -496,7 +497,7 @@ def forward_line(args) -> dict:
 
 
 def learned_line(args) -> dict:
-    """Config 5: the U-Net preconditioner trained on solver data, then plain against
+    """The learned-FWI workload: the U-Net preconditioner trained on solver data, then plain against
     learned FWI on held-out models, timed to the plain run's final misfit.
 
     ROI `roi`^2 behind a 32-cell PML, dx = 1 mm, order 8, float32, `shots`
