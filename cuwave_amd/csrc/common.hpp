@@ -233,6 +233,15 @@ __device__ __forceinline__ Q mirrored(const T *__restrict__ u, int64_t e,
   return neg ? -v : v;
 }
 
+// the node step from a to the node `mirrored` reads for tap j, and whether
+// that tap is negated
+__device__ __forceinline__ int mirror_step(int a, int N, int odd, int j,
+                                           bool &neg) {
+  const int b = a + j;
+  neg = (b < 0 && (odd & 1)) || (b > N - 1 && (odd & 2));
+  return (b < 0 ? 2 - b : b > N - 1 ? 2 * (N - 2) - b : b) - a;
+}
+
 // tap j through the image about the wall node, for a field no refill keeps
 // past it, zero on a held wall node
 template <typename Q, typename T>

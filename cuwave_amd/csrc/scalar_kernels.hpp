@@ -582,21 +582,25 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
     return;
   const int sh = g.lshift;
   const int b = (int)(threadIdx.x & ((1u << ts) - 1)) * V; // first lane
-  int64_t idx = 0;
+  // node and element indices in 32 bits for the taps (the grid check
+  // refuses a level of 2^31 elements or more)
+  int node = 0;
   bool held = false, deep = true;
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
-    idx += (int64_t)a[d] * g.s[d];
+    node += a[d] * g.s[d];
     held |= held_wall(a[d], g.n[d], g.dirichlet >> (2 * d));
     deep &= a[d] >= R && a[d] <= g.n[d] - 1 - R;
   }
+  const int64_t idx = node;
   int code = kDeepCells;
   if constexpr (MASK) {
     code = p.cells[idx];
     if (!code)
       return; // outside the domain, or held: never stepped
   }
-  const int64_t e = (idx << sh) + b;
+  const int elem = (node << sh) + b;
+  const int64_t e = elem;
   const T mi = p.inv[idx];
   T gp[NDIM], gm[NDIM];
 #pragma unroll
@@ -630,15 +634,18 @@ __device__ __forceinline__ void forward_general(const FwdParams<T> &p,
   Q taps[NDIM][2 * R + 1];
 #pragma unroll
   for (int d = 0; d < NDIM; ++d) {
-    const int64_t se = (int64_t)g.s[d] << sh;
+    const int se = g.s[d] << sh;
+    // mirrored only along an axis near a wall (the same bits as along every
+    // axis), there by moving the one load of a tap and setting its sign
+    const bool wall = !(deep || (a[d] >= R && a[d] <= g.n[d] - 1 - R));
 #pragma unroll
-    for (int j = -R; j <= R; ++j)
-      // mirrored only along an axis near a wall (the same bits as along
-      // every axis, for fewer instructions at the walls)
-      taps[d][j + R] = deep || (a[d] >= R && a[d] <= g.n[d] - 1 - R)
-                           ? lanes_at<Q>(u1, e + j * se)
-                           : mirrored<Q>(u1, e, se, a[d], g.n[d],
-                                         g.dirichlet >> (2 * d), j);
+    for (int j = -R; j <= R; ++j) {
+      bool neg = false;
+      const int k =
+          wall ? mirror_step(a[d], g.n[d], g.dirichlet >> (2 * d), j, neg) : j;
+      const Q v = lanes_at<Q>(u1, elem + k * se);
+      taps[d][j + R] = neg ? -v : v;
+    }
   }
   const Q uc = taps[0][R];
   Q lap;
