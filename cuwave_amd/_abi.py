@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 10
+ABI_VERSION = 11
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -225,6 +225,37 @@ class ElasticGradientArgs(ctypes.Structure):
     ]
 
 
+class MaxwellGrid(ctypes.Structure):
+    _fields_ = [
+        ("ndim", I32),
+        ("order", I32),
+        ("n", I32 * 3),
+        ("np", I32 * 3),
+        ("nbatch", I32),
+        ("lanes", I32),
+        ("dtype", I32),
+        ("conductor", I32),
+        ("stag", F64 * 4),
+        ("nu", F64),
+        ("inv_dx", F64 * 3),
+        ("step", F64 * 3),
+    ]
+
+
+class MaxwellStepArgs(ctypes.Structure):
+    _fields_ = [
+        ("grid", MaxwellGrid),
+        ("minv", VP),
+        ("curl", VP),
+        ("u0", VP),
+        ("u1", VP),
+        ("u2", VP),
+        ("g_mass", VP),
+        ("v1", VP),
+        ("mass_factor", F64),
+    ]
+
+
 class ElasticPointArgs(ctypes.Structure):
     _fields_ = [
         ("starts", VP),
@@ -305,6 +336,8 @@ SIGNATURES = {
     "cw_elastic_step": ([ctypes.POINTER(ElasticStepArgs), VP], I32),
     "cw_elastic_gradient": ([ctypes.POINTER(ElasticGradientArgs), VP], I32),
     "cw_elastic_points": ([ctypes.POINTER(ElasticGrid), ctypes.POINTER(ElasticPointArgs), VP], I32),
+    "cw_maxwell_step": ([ctypes.POINTER(MaxwellStepArgs), VP], I32),
+    "cw_maxwell_points": ([ctypes.POINTER(MaxwellGrid), ctypes.POINTER(ElasticPointArgs), VP], I32),
     "cw_conv3x3_bf16_fwd": ([ctypes.POINTER(Conv3x3Args), VP], I32),
     "cw_conv3x3_bf16_bwd_data": ([ctypes.POINTER(Conv3x3Args), VP], I32),
     "cw_conv3x3_bf16_bwd_weight": ([ctypes.POINTER(Conv3x3Args), VP], I32),

@@ -39,6 +39,11 @@ Dirichlet = BoundaryCondition("dirichlet")
 # displacement there at zero
 Traction = BoundaryCondition("traction")
 Clamped = BoundaryCondition("clamped")
+# the faces of the vector Maxwell family (cuwave/boundary.py:30-78): a
+# conductor holds the tangential electric field at zero on its wall node, a
+# magnetic wall holds the tangential magnetic field (the curl) there
+Conductor = BoundaryCondition("conductor")
+Magnetic = BoundaryCondition("magnetic")
 
 
 def canonical_boundary(boundary, ndim: int, default: BoundaryCondition = Neumann,

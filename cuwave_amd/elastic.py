@@ -313,7 +313,8 @@ class _Points:
         a.kind, a.row = kind, int(row)
         a.field, a.table = _abi.ptr(field), _abi.ptr(table)
         a.g_mass, a.u1, a.mass_factor = _abi.ptr(g_mass), _abi.ptr(u1), float(mf)
-        _abi.call("cw_elastic_points", ctypes.byref(self.eng.grid), ctypes.byref(a), _abi.stream())
+        entry = getattr(self.eng, "points_entry", "cw_elastic_points")
+        _abi.call(entry, ctypes.byref(self.eng.grid), ctypes.byref(a), _abi.stream())
 
     def inject(self, field, table, row):
         self._call(0, field, table, row)

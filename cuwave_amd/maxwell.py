@@ -10,19 +10,43 @@ coefficient fields assigned differently (cuwave/maxwell.py:72-160):
 - `MagneticWave` carries H_z: mu is its inertia and 1/eps its stiffness, so the
   design enters the stiffness as 1/eps(gamma), with a nodal Jacobian.
 
-Both step with the factors 2 dt^2 / dx^2 and an unscaled source. The 3D vector
-form (`MaxwellWave`, the staggered curl-curl) is not part of this build.
+Both step with the factors 2 dt^2 / dx^2 and an unscaled source.
+
+`MaxwellWave` is the vector form 3D needs (cuwave/maxwell.py:162-708): the Yee
+lattice written as -C^T nu C with a diagonal permittivity, electric component
+c half a node up axis c and the curl of the pair (k, l) half a node up both.
+A step is the pair of launches `cw_maxwell_step` makes (the curls, then the
+update); `DielectricWave` interpolates the permittivity between two phases.
+The drivers play the part of the reference's closures over a batch of shots,
+the shots innermost in every per-shot array as in the other families:
+
+- `simulate_shots` / `simulate`: the forward march (cuwave/wave.py:1071-1163);
+- `sensitivity`: the stored-history adjoint, the adjoint step being the
+  forward step itself (no pml breaks its symmetry) with the inertia density
+  fused in (cuwave/maxwell.py:278-282, 400-445), chained through the harmonic
+  point means onto the nodal permittivity (:601-613).
+
+The host-side parametrization (the point inertia, the excitation weights,
+the chain rule) runs in numpy in the simulation's precision, as the reference
+computes it. Not built here: a pml, damping, a designed permeability
+(`MagnetodielectricWave`), the fused and blocked launch tiers.
 """
 
 from __future__ import annotations
 
+import ctypes
 import math
 from dataclasses import dataclass
 
 import numpy as np
+import torch
 
+from . import _abi
+from .boundary import Conductor, Magnetic, face_mask, faces_with
+from .elastic import ElasticEngine, _require, adjoint_signal
 from .scalar import PressureWave
-from .wave import as_numpy
+from .stencils import staggered_weights
+from .wave import Simulation, Source, as_numpy, shot_lanes
 
 
 @dataclass
@@ -85,3 +109,317 @@ class MagneticWave(PolarizedWave):
         ind = np.asarray(as_numpy(indicator), dtype=self.np_dtype)
         contrast = self.permittivity2 - self.permittivity1
         return 0.0, -contrast / self.permittivity(ind) ** 2
+
+
+# ------------------------------------------------------------ vector Maxwell
+@dataclass
+class MaxwellWave(Simulation):
+    """Curl-curl Maxwell on the Yee lattice (cuwave/maxwell.py:162-260).
+
+    Component c of the electric field lives half a node up axis c and the curl
+    of the pair (k, l) half a node up both of its axes. The operator is
+    -C^T nu C, symmetric by construction and its own transpose at every order;
+    the walls read images (odd across a conductor for a tangential component,
+    the curl odd across a magnetic wall), and a conductor holds the tangential
+    field on its wall node at zero through a zeroed inertia.
+    """
+
+    permeability: float = 1.0  # uniform mu: the permeability is not designed here
+
+    default_boundary = Conductor
+    face_conditions = (Conductor, Magnetic)
+
+    def __post_init__(self) -> None:
+        super().__post_init__()
+        if self.ndim < 2:
+            raise ValueError("a single in-plane component has no curl; use ElectricWave")
+        if self.permeability <= 0.0:
+            raise ValueError(f"permeability must be positive: {self.permeability}")
+        if self.pml is not None:
+            raise NotImplementedError("a pml on the vector Maxwell family is not part of this build")
+        self.field_shape = (self.ncomp, *self.Nx_padded)
+
+    @property
+    def ncomp(self) -> int:
+        return self.ndim
+
+    @property
+    def npairs(self) -> int:
+        """Curl components, one per axis pair: 1 in 2D and 3 in 3D."""
+        return self.ndim * (self.ndim - 1) // 2
+
+    @property
+    def radius(self) -> int:
+        return self.space_order // 2
+
+    @property
+    def interior(self) -> tuple[slice, ...]:
+        return (slice(None), *(slice(0, n) for n in self.Nx))
+
+    def flatten_indices(self, position) -> np.ndarray:
+        """(1 + ndim, num) indices, the component row first, as int32 flat indices
+        (cuwave/wave.py:364-382)."""
+        position = np.asarray(as_numpy(position), dtype=np.int64)
+        if position.ndim != 2 or position.shape[0] != self.node_rows:
+            raise ValueError(f"position needs {self.node_rows} rows (the component, then the "
+                             f"axes), not {position.shape}")
+        lin = position[0] * self.comp_stride
+        if np.any(position[0] < 0) or np.any(position[0] >= self.ncomp):
+            raise ValueError(f"a component row outside 0 .. {self.ncomp - 1}")
+        for d in range(self.ndim):
+            rows = position[1 + d]
+            if np.any(rows < 0) or np.any(rows >= self.Nx[d]):
+                raise ValueError(f"a position on axis {d} lies outside 0 .. {self.Nx[d] - 1}")
+            lin = lin + rows * self.strides[d]
+        return lin.astype(np.int32)
+
+    def parametrization(self, indicator):
+        """The nodal permittivity of the design (a subclass defines it)."""
+        raise NotImplementedError(f"{type(self).__name__} defines no parametrization")
+
+    # ----------------------------------------------------------- materials
+    def _wall_weights(self, axes) -> np.ndarray:
+        """Cell weights of a point family staggered along `axes`: halved on the other
+        walls (cuwave/wave.py:386-394)."""
+        w = np.ones(self.Nx_padded, dtype=self.np_dtype)
+        for d in set(range(self.ndim)) - set(axes):
+            for index in (1, self.Nx[d] - 2):
+                wall = [slice(None)] * self.ndim
+                wall[d] = index
+                w[tuple(wall)] *= 0.5
+        return w
+
+    def _point_average(self, field: np.ndarray, c: int) -> np.ndarray:
+        out = np.zeros(self.Nx_padded, dtype=self.np_dtype)
+        lo = tuple(slice(0, n - 1) if d == c else slice(0, n) for d, n in enumerate(self.Nx))
+        hi = tuple(slice(1, n) if d == c else slice(0, n) for d, n in enumerate(self.Nx))
+        out[lo] = 0.5 * (field[lo] + field[hi])
+        return out
+
+    def _point_harmonic(self, field: np.ndarray, c: int) -> np.ndarray:
+        """The harmonic mean of the two nodes component c sits between
+        (cuwave/wave.py:416-425)."""
+        tiny = np.finfo(self.np_dtype).tiny
+        mean = self._point_average(1.0 / np.maximum(field, tiny), c)
+        return np.where(mean > 0.0, 1.0 / np.maximum(mean, tiny), 0.0).astype(self.np_dtype)
+
+    def point_inverse_inertia(self, eps: np.ndarray) -> np.ndarray:
+        """(ncomp, *Nx_padded): 1 / (W eps) on the component points, zero on a
+        conductor's wall node for the tangential components (cuwave/maxwell.py:247-263)."""
+        tiny = np.finfo(self.np_dtype).tiny
+        minv = np.zeros((self.ncomp, *self.Nx_padded), dtype=self.np_dtype)
+        for c in range(self.ncomp):
+            mass = self._wall_weights((c,)) * self._point_harmonic(eps, c)
+            minv[c] = 1.0 / np.maximum(mass, tiny)
+        for face in faces_with(self, Conductor):
+            wall = [slice(None)] * self.ndim
+            wall[face // 2] = 1 if face % 2 == 0 else self.Nx[face // 2] - 2
+            for c in range(self.ncomp):
+                if c != face // 2:
+                    minv[(c, *wall)] = 0.0
+        return minv
+
+    def indicator_shape(self) -> tuple[int, ...]:
+        return self.Nx_padded
+
+    def build_materials(self, indicator) -> dict:
+        """The nodal permittivity and the point inverse inertia, on the host."""
+        self.check_masks()
+        gamma = np.ascontiguousarray(as_numpy(indicator), dtype=self.np_dtype)
+        if gamma.shape != self.indicator_shape():
+            raise ValueError(f"indicator of shape {gamma.shape}, expected {self.indicator_shape()}")
+        eps = np.ascontiguousarray(self.parametrization(gamma), dtype=self.np_dtype)
+        return {"eps": eps, "minv": self.point_inverse_inertia(eps)}
+
+    def excitation_weights(self, mat: dict, lin) -> np.ndarray:
+        """Source weights dt^2 / (eps V) (cuwave/maxwell.py:561-574)."""
+        volume = self.np_dtype(1.0 / float(np.prod(self.dx)))
+        weight = mat["minv"].ravel()[np.asarray(lin, dtype=np.int64)] * volume
+        return (self.np_dtype(self.dt**2 * self.source_factor()) * weight).astype(self.np_dtype)
+
+    def adjoint_weights(self, sensors) -> np.ndarray:
+        """1 / V: dJ/du is nodal, so it undoes the volume the source weights divide by."""
+        volume = self.np_dtype(1.0 / float(np.prod(self.dx)))
+        return np.full(np.asarray(as_numpy(sensors)).shape[1], volume, dtype=self.np_dtype)
+
+    def step_factors(self) -> list:
+        """Per-axis update factors dt^2 / dx, the curl carrying the other 1 / dx."""
+        return [self.np_dtype(self.dt**2 / d) for d in self.dx]
+
+    def source_factor(self) -> float:
+        return 1.0
+
+    def adjoint_mass_factor(self) -> float:
+        return 1.0 / self.dt**2
+
+    def abi_grid(self, nbatch: int) -> _abi.MaxwellGrid:
+        g = _abi.MaxwellGrid()
+        g.ndim, g.order, g.nbatch = self.ndim, self.space_order, int(nbatch)
+        g.lanes = shot_lanes(nbatch)
+        g.dtype = 0 if self.precision == "float32" else 1
+        for d in range(self.ndim):
+            g.n[d], g.np[d] = self.Nx[d], self.Nx_padded[d]
+        g.conductor = face_mask(self, Conductor)
+        for k, w in enumerate(staggered_weights(self.radius, self.stencil)):
+            g.stag[k] = float(self.np_dtype(w))
+        g.nu = float(self.np_dtype(1.0 / self.permeability))
+        for d in range(self.ndim):
+            g.inv_dx[d] = float(self.np_dtype(1.0 / self.dx[d]))
+            g.step[d] = float(self.step_factors()[d])
+        return g
+
+
+@dataclass
+class DielectricWave(MaxwellWave):
+    """Two-phase dielectric, gamma interpolating the permittivity (cuwave/maxwell.py:676-708)."""
+
+    permittivity1: float = None  # background, gamma = 0
+    permittivity2: float = None  # design material, gamma = 1
+
+    def __post_init__(self) -> None:
+        super().__post_init__()
+        if None in (self.permittivity1, self.permittivity2):
+            raise ValueError("DielectricWave requires permittivity1 and permittivity2")
+        if min(self.permittivity1, self.permittivity2) <= 0.0:
+            raise ValueError(f"permittivity must be positive: {self.permittivity1}, "
+                             f"{self.permittivity2}")
+
+    @property
+    def light_speed(self) -> float:
+        """The fastest speed on the grid, 1 / sqrt(min(eps) mu), which bounds dt."""
+        return 1.0 / math.sqrt(min(self.permittivity1, self.permittivity2) * self.permeability)
+
+    def parametrization(self, indicator):
+        """eps1 + gamma (eps2 - eps1) (cuwave/maxwell.py:65-67)."""
+        return (self.permittivity1
+                + indicator * (self.permittivity2 - self.permittivity1)).astype(self.np_dtype)
+
+    def parametrization_jacobian(self, indicator=None):
+        """The permittivity is affine in gamma; the permeability does not depend on it."""
+        return self.permittivity2 - self.permittivity1, 0.0
+
+
+class MaxwellEngine(ElasticEngine):
+    """The pair of one Maxwell simulation bound to its material, for `nbatch` shots.
+
+    Per-shot arrays end in the lanes axis: a field level is (ncomp, *Nx_padded,
+    lanes), the curl scratch (npairs, *Nx_padded, lanes). The point lists are
+    the elastic family's (`cw_maxwell_points` runs the same kernel).
+    """
+
+    points_entry = "cw_maxwell_points"
+
+    def __init__(self, sim: MaxwellWave, mat: dict, nbatch: int):
+        self.device = _abi.require_device()
+        self.sim, self.nbatch = sim, int(nbatch)
+        self.grid = sim.abi_grid(self.nbatch)
+        self.lanes = self.grid.lanes
+        self.minv = torch.as_tensor(mat["minv"], device=self.device)
+        self.curl = torch.zeros((sim.npairs, *sim.Nx_padded, self.lanes), dtype=sim.dtype,
+                                device=self.device)
+        self.args = _abi.MaxwellStepArgs()
+        self.args.grid = self.grid
+        self.args.minv, self.args.curl = _abi.ptr(self.minv), _abi.ptr(self.curl)
+
+    def step(self, u0, u1, u2, g_mass=None, v1=None, mf=0.0):
+        """u2 of the pair (u2 may be u0); with g_mass the adjoint step's inertia density of
+        u1 against the forward v1 (cw_maxwell_step)."""
+        a = self.args
+        a.u0, a.u1, a.u2 = _abi.ptr(u0), _abi.ptr(u1), _abi.ptr(u2)
+        a.g_mass, a.v1, a.mass_factor = _abi.ptr(g_mass), _abi.ptr(v1), float(mf)
+        _abi.call("cw_maxwell_step", ctypes.byref(a), _abi.stream())
+        return u2
+
+    def gradient_fields(self) -> dict:
+        sim = self.sim
+        return {"mass": torch.zeros((sim.ncomp, *sim.Nx_padded), dtype=sim.dtype,
+                                    device=self.device)}
+
+
+def _point_average_adjoint(density: np.ndarray, c: int) -> np.ndarray:
+    """Transpose of the point mean: half of `density` back onto each node it spans
+    (cuwave/wave.py:405-413)."""
+    out = 0.5 * density
+    to = [slice(None)] * density.ndim
+    fro = [slice(None)] * density.ndim
+    to[c], fro[c] = slice(1, None), slice(0, -1)
+    out[tuple(to)] += 0.5 * density[tuple(fro)]
+    return out
+
+
+def finalize_gradients(sim: MaxwellWave, g_mass: np.ndarray, eps: np.ndarray) -> dict:
+    """The point densities chained through the harmonic means onto the nodal
+    permittivity (cuwave/maxwell.py:601-613, cuwave/wave.py:427-433); zero stiffness,
+    the permeability not being designed."""
+    tiny = np.finfo(sim.np_dtype).tiny
+    safe = np.maximum(eps, tiny)
+    out = np.zeros(sim.Nx_padded, dtype=sim.np_dtype)
+    for c in range(sim.ncomp):
+        density = g_mass[c] * sim._wall_weights((c,))
+        mean = sim._point_harmonic(eps, c)
+        out += _point_average_adjoint(density * mean * mean, c) / (safe * safe)
+    return {"mass": out, "stiff": np.zeros(sim.Nx_padded, dtype=sim.np_dtype)}
+
+
+def simulate_shots(sim: MaxwellWave, sources, indicator, sensors=None):
+    """Every shot of `sources` forward in one batch: the (B, ncomp, *Nx) final fields
+    and, with `sensors`, the (B, N, count) records."""
+    sources = list(sources)
+    _require(sim, sources, sensors)
+    mat = sim.build_materials(indicator)
+    eng = MaxwellEngine(sim, mat, len(sources))
+    src, signal = eng.sources(mat, sources)
+    U = eng.levels(2)
+    rec = record = None
+    if sensors is not None:
+        rec = eng.sensors(sensors)
+        record = torch.zeros((sim.N, rec.count, eng.lanes), dtype=sim.dtype, device=eng.device)
+    for t in range(sim.N):
+        # slot t % 2 takes u^t over u^(t - 2)
+        u = eng.step(U[t % 2], U[1 - t % 2], U[t % 2])
+        src.inject(u, signal, t)
+        if rec is not None:
+            rec.record(u, record, t)
+    field = eng.shots(U[(sim.N - 1) % 2])[(slice(None), *sim.interior)].clone()
+    return (field, eng.shots(record)) if sensors is not None else field
+
+
+def simulate(sim: MaxwellWave, source: Source, indicator, sensors=None):
+    """The final interior field and, with `sensors`, the (N, count) record."""
+    out = simulate_shots(sim, [source], indicator, sensors)
+    if isinstance(out, tuple):
+        return tuple(o[0] for o in out)
+    return out[0]
+
+
+def sensitivity(sim: MaxwellWave, source: Source, indicator, sensors, objective):
+    """Cost and d(cost)/d(mass, stiff) by the adjoint over a stored history
+    (cuwave/sensitivity.py:384-497): N + 2 forward levels, then the adjoint steps
+    accumulating the inertia density, chained onto the nodal permittivity."""
+    _require(sim, [source], sensors)
+    mat = sim.build_materials(indicator)
+    eng = MaxwellEngine(sim, mat, 1)
+    src, signal = eng.sources(mat, [source])
+    rec = eng.sensors(sensors)
+    record = torch.zeros((sim.N, rec.count, eng.lanes), dtype=sim.dtype, device=eng.device)
+    V = eng.levels(sim.N + 2)
+    for t in range(sim.N):
+        u = eng.step(V[t], V[t + 1], V[t + 2])
+        src.inject(u, signal, t)
+        rec.record(u, record, t)
+    um = eng.shots(record)[0]
+    cost, dphi = objective(um)
+    dphi = torch.as_tensor(dphi, dtype=sim.dtype, device=eng.device)
+    adjoint = eng.lane_table(adjoint_signal(sim, dphi[None], sensors))
+    lin = sim.flatten_indices(sensors)
+    adj = eng.sensors(sensors, weight=sim.excitation_weights(mat, lin))
+    grads = eng.gradient_fields()
+    mf = sim.np_dtype(sim.adjoint_mass_factor())
+    P = eng.levels(2)
+    for m in range(sim.N):
+        v1 = V[sim.N + 1 - m]
+        l0, l1 = P[m % 2], P[1 - m % 2]
+        eng.step(l0, l1, l0, grads["mass"], v1, mf)
+        adj.adjoint_inject(l0, adjoint, m, grads["mass"], v1, mf)
+    out = finalize_gradients(sim, grads["mass"].cpu().numpy(), mat["eps"])
+    return cost, {k: torch.as_tensor(v, device=eng.device) for k, v in out.items()}, um, {}

@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 10
+#define CW_ABI_VERSION 11
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -396,6 +396,51 @@ int cw_elastic_prepare(const cw_elastic_grid *grid, const cw_elastic_material *m
 int cw_elastic_step(const cw_elastic_step_args *args, void *stream);
 int cw_elastic_gradient(const cw_elastic_gradient_args *args, void *stream);
 int cw_elastic_points(const cw_elastic_grid *grid, const cw_elastic_point_args *args,
+                      void *stream);
+
+/* ------------------------------------------------------------ Maxwell
+ * The vector Maxwell family: the Yee lattice in second-order curl-curl form
+ * (cuwave/maxwell.py:162-260, cuwave/kernels/maxwell.cu:44-275). Electric
+ * component c sits half a node up axis c, the curl of the pair (k, l) half a
+ * node up both. Per-shot arrays are (component, *np, lanes); the curl
+ * scratch (npairs, *np, lanes), npairs 1 in 2D and 3 in 3D (pair (k, l) at
+ * row 3 - k - l). */
+
+/* the grid and the operator constants (MaxwellWave's compile flags and
+ * `_pair_args`, cuwave/maxwell.py:185-190, 322-340) */
+typedef struct cw_maxwell_grid {
+  int32_t ndim;      /* 2 or 3 */
+  int32_t order;     /* space order: 2, 4, 6 or 8 */
+  int32_t n[3];      /* logical extents, ghosts included */
+  int32_t np[3];     /* padded extents */
+  int32_t nbatch, lanes, dtype;
+  int32_t conductor; /* Conductor faces, bit 2 axis + side; the others are
+                        Magnetic */
+  double stag[4];    /* the full-radius staggered weights, stag[k - 1] */
+  double nu;         /* 1 / permeability */
+  double inv_dx[3];  /* the curl factors 1 / dx */
+  double step[3];    /* the update factors dt^2 / dx */
+} cw_maxwell_grid;
+
+/* one step over a batch (curl_kernel then fd_kernel, cuwave/kernels/
+ * maxwell.cu:209-274): nu W curl(u1) into `curl`, then u2 = -u0 + 2 u1 +
+ * minv force; u2 may be u0. With g_mass set it is the adjoint step: the
+ * inertia density of u1 against the forward v1, the shots summed into the
+ * per-point accumulator (ncomp, *np) (the GRADIENT build, :268-272). */
+typedef struct cw_maxwell_step_args {
+  cw_maxwell_grid grid;
+  const void *minv; /* (ncomp, *np) point inverse inertia */
+  void *curl;
+  const void *u0, *u1;
+  void *u2;
+  void *g_mass;     /* NULL for a forward step */
+  const void *v1;
+  double mass_factor; /* 1 / dt^2 (cuwave/wave.py:597-599) */
+} cw_maxwell_step_args;
+
+int cw_maxwell_step(const cw_maxwell_step_args *args, void *stream);
+/* points of a Maxwell field, as cw_elastic_points */
+int cw_maxwell_points(const cw_maxwell_grid *grid, const cw_elastic_point_args *args,
                       void *stream);
 
 /* ------------------------------------------------------------ U-Net

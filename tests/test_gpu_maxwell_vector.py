@@ -1,0 +1,203 @@
+"""The vector Maxwell kernels against the oracle on the GPU.
+
+- the forward march, several shots in one batch, bit for bit against the
+  oracle's one-shot march: 2D and 3D, orders 2 to 8, conductor, magnetic and
+  mixed walls, float64 and float32;
+- the fdtdx dipole cases through the kernels, at the gaps the reference's
+  tests hold (cuwave tests/maxwell_test.py:968-978);
+- one adjoint step with its inertia density, bit for bit against the
+  oracle's GRADIENT step;
+- `sensitivity` against central finite differences of the misfit
+  (cuwave tests/maxwell_test.py:839-846).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import ROOT
+
+from cuwave_amd import _abi
+from cuwave_amd import maxwell as MX
+from cuwave_amd.boundary import Conductor, Magnetic
+from cuwave_amd.stencils import staggered_weights
+from cuwave_amd.wave import Source
+from oracle import maxwell as M
+
+pytestmark = pytest.mark.gpu
+
+WALLS = {"conductor": None, "magnetic": Magnetic,
+         "mixed": lambda n: ((Conductor, Magnetic),) + ((Magnetic, Conductor),) * (n - 1)}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def library():
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    _abi.load()
+
+
+def vector_sim(Nx, order, walls="conductor", precision="float64", N=30):
+    boundary = WALLS[walls]
+    if callable(boundary):
+        boundary = boundary(len(Nx))
+    dx = tuple(1.0 / (n - 3) for n in Nx)
+    return MX.DielectricWave(Nx, dx, N, 0.25 * min(dx), precision=precision, space_order=order,
+                             permittivity1=1.0, permittivity2=4.0, boundary=boundary)
+
+
+def logical(sim, a):
+    return a[(..., *(slice(0, n) for n in sim.Nx))]
+
+
+def oracle_march(sim, gamma, source, sensors):
+    """The oracle's one-shot march of `source` with the product's weights' arithmetic."""
+    dt = sim.np_dtype
+    eps = logical(sim, sim.parametrization(gamma))
+    grid = sim.abi_grid(1)
+    minv = M.point_inverse_inertia(eps, grid.conductor)
+    pos = np.asarray(source.position)
+    volume = dt(1.0 / float(np.prod(sim.dx)))
+    weight = (dt(sim.dt**2) * (minv[tuple(pos)] * volume)).astype(dt)
+    sens = np.asarray(sensors)
+    return M.simulate(minv, staggered_weights(sim.radius, sim.stencil).astype(dt),
+                      [dt(1.0 / d) for d in sim.dx], sim.step_factors(), dt(1.0), grid.conductor,
+                      sim.N, tuple(pos), weight, np.asarray(source.signal, dtype=dt), tuple(sens))
+
+
+def shots(sim, rng, count):
+    """`count` shots of two sources each on random component points of the interior."""
+    out = []
+    for _ in range(count):
+        pos = [rng.integers(0, sim.ncomp, 2)]
+        for n in sim.Nx:
+            pos.append(rng.integers(2, n - 3, 2))
+        out.append(Source(np.array(pos), rng.standard_normal((sim.N, 2)).astype(sim.np_dtype)))
+    return out
+
+
+def sensor_points(sim, rng, count=5):
+    pos = [rng.integers(0, sim.ncomp, count)] + [rng.integers(1, n - 2, count) for n in sim.Nx]
+    return np.array(pos)
+
+
+@pytest.mark.parametrize("Nx", [(22, 26), (14, 16, 18)])
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
+@pytest.mark.parametrize("walls", sorted(WALLS))
+def test_the_forward_march_is_the_oracles_bit_for_bit(Nx, order, walls):
+    print(f"maxwell forward {Nx} order {order} {walls}", flush=True)
+    sim = vector_sim(Nx, order, walls)
+    rng = np.random.default_rng(order + len(Nx))
+    gamma = rng.random(sim.Nx_padded)
+    sources = shots(sim, rng, 3)
+    sensors = sensor_points(sim, rng)
+    field, record = MX.simulate_shots(sim, sources, gamma, sensors)
+    for b, source in enumerate(sources):
+        last, traces = oracle_march(sim, gamma, source, sensors)
+        assert np.array_equal(field[b].cpu().numpy(), last), f"shot {b}"
+        assert np.array_equal(record[b].cpu().numpy(), traces), f"shot {b}"
+
+
+@pytest.mark.parametrize("Nx", [(22, 26), (14, 16, 18)])
+def test_the_float32_march_is_the_oracles_bit_for_bit(Nx):
+    print(f"maxwell float32 {Nx}", flush=True)
+    sim = vector_sim(Nx, 4, "mixed", precision="float32")
+    rng = np.random.default_rng(7)
+    gamma = rng.random(sim.Nx_padded).astype(np.float32)
+    sources = shots(sim, rng, 5)
+    sensors = sensor_points(sim, rng)
+    field, record = MX.simulate_shots(sim, sources, gamma, sensors)
+    for b, source in enumerate(sources):
+        last, traces = oracle_march(sim, gamma, source, sensors)
+        assert np.array_equal(field[b].cpu().numpy(), last), f"shot {b}"
+        assert np.array_equal(record[b].cpu().numpy(), traces), f"shot {b}"
+
+
+@pytest.mark.parametrize("name, bound", [("vector3D_vacuum", 1e-12), ("vector3D_block", 1e-3)])
+def test_the_kernels_reproduce_fdtdx(name, bound):
+    print(f"maxwell fdtdx {name}", flush=True)
+    ref = np.load(ROOT / "tests" / "golden" / f"fdtdx_{name}.npz")
+    shape = tuple(int(n) + 2 for n in ref["shape"])
+    S = float(ref["courant"])
+    sim = MX.DielectricWave(shape, (1.0,) * 3, len(ref["amplitude"]), S, precision="float64",
+                            permittivity1=1.0, permittivity2=float(ref["permittivity"]),
+                            boundary=Conductor)
+    gamma = np.zeros(sim.Nx_padded)
+    if ref["block_low"].size:
+        low, size = ref["block_low"] + 1, ref["block_size"]
+        gamma[tuple(slice(lo, lo + n) for lo, n in zip(low, size))] = 1.0
+    source = np.array([[2], *([s + 1] for s in ref["source"])])
+    mat = sim.build_materials(gamma)
+    weight = sim.excitation_weights(mat, sim.flatten_indices(source))
+    loads = np.diff(-S * ref["amplitude"], prepend=0.0)
+    probes = ref["probes"].T + 1
+    sensors = np.concatenate([np.full((1, probes.shape[1]), 2), probes])
+    _, traces = MX.simulate(sim, Source(source, (loads / weight)[:, None]), gamma, sensors)
+    theirs = ref["traces"]
+    assert np.linalg.norm(traces.cpu().numpy() - theirs) / np.linalg.norm(theirs) < bound
+
+
+@pytest.mark.parametrize("Nx, order", [((20, 23), 6), ((11, 12, 14), 4)])
+def test_the_adjoint_step_is_the_oracles_gradient_step(Nx, order):
+    print(f"maxwell adjoint step {Nx} order {order}", flush=True)
+    sim = vector_sim(Nx, order, "mixed")
+    rng = np.random.default_rng(11)
+    mat = sim.build_materials(rng.random(sim.Nx_padded))
+    eng = MX.MaxwellEngine(sim, mat, 1)
+    levels = [rng.standard_normal((sim.ncomp, *sim.Nx_padded, 1)) for _ in range(3)]
+    free = np.zeros(levels[0].shape, dtype=bool)
+    free[(slice(None), *(slice(1, n - 1) for n in sim.Nx))] = True
+    l0, l1, v1 = (torch.as_tensor(np.where(free, x, 0.0), device="cuda") for x in levels)
+    g0 = rng.standard_normal((sim.ncomp, *sim.Nx_padded))
+    g = torch.as_tensor(g0, device="cuda")
+    mf = 1.0 / sim.dt**2
+    out = eng.step(l0.clone(), l1, l0.clone(), g, v1, mf)
+    d = sim.np_dtype
+    g_want = logical(sim, g0).copy()
+    want = M.step(logical(sim, l0.cpu().numpy()[..., 0]), logical(sim, l1.cpu().numpy()[..., 0]),
+                  logical(sim, mat["minv"]), staggered_weights(sim.radius, sim.stencil).astype(d),
+                  [d(1.0 / x) for x in sim.dx], sim.step_factors(), 1.0,
+                  sim.abi_grid(1).conductor, v1=logical(sim, v1.cpu().numpy()[..., 0]),
+                  g_mass=g_want, mf=mf)
+    got = logical(sim, out.cpu().numpy()[..., 0])
+    valid = np.zeros(want.shape, dtype=bool)
+    for c in range(sim.ncomp):
+        valid[(c, *(slice(1, n - 1 - (k == c)) for k, n in enumerate(sim.Nx)))] = True
+    assert np.array_equal(np.where(valid, got, 0.0), want)
+    assert np.array_equal(logical(sim, g.cpu().numpy()), g_want)
+
+
+@pytest.mark.parametrize("Nx, order", [((16, 18), 4), ((10, 11, 12), 2)])
+def test_the_gradient_matches_finite_differences(Nx, order):
+    print(f"maxwell gradient {Nx} order {order}", flush=True)
+    sim = vector_sim(Nx, order, "mixed", N=60)
+    rng = np.random.default_rng(5)
+    gamma = 0.3 + 0.4 * rng.random(sim.Nx_padded)
+    t = np.arange(sim.N) * sim.dt
+    f0 = 4.0 / (sim.N * sim.dt)
+    arg = (np.pi * f0 * (t - 1.2 / f0)) ** 2
+    pulse = (1 - 2 * arg) * np.exp(-arg)
+    centre = [n // 2 for n in Nx]
+    source = Source(np.array([[1], *([c] for c in centre)]), pulse[:, None])
+    sensors = np.array([[0, 1, sim.ncomp - 1], *([c - 2, c + 2, c + 1] for c in centre)])
+    observed = MX.simulate(sim, source, 0.5 * np.ones(sim.Nx_padded), sensors)[1].cpu().numpy()
+
+    def objective(um):
+        r = um.cpu().numpy() - observed
+        return 0.5 * float(np.sum(r * r)), torch.as_tensor(r)
+
+    cost, grads, _, _ = MX.sensitivity(sim, source, gamma, sensors, objective)
+    d_mass, _ = sim.parametrization_jacobian()
+    g = (d_mass * grads["mass"]).cpu().numpy()
+    h = 1e-5
+    for node in [tuple(c + 1 for c in centre), tuple(c - 1 for c in centre),
+                 tuple(min(c + 3, n - 2) for c, n in zip(centre, Nx))]:
+        costs = []
+        for s in (1, -1):
+            probe = gamma.copy()
+            probe[node] += s * h
+            um = MX.simulate(sim, source, probe, sensors)[1]
+            costs.append(objective(um)[0])
+        fd = (costs[0] - costs[1]) / (2 * h)
+        assert abs(g[node] - fd) <= 1e-5 * abs(fd) + 1e-10 * abs(cost), (node, g[node], fd)
