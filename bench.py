@@ -770,11 +770,12 @@ def main():
     p.add_argument("--shot-batch", type=int, default=16,
                    help="learned: shots a sweep (24 shots as 16 + 8 leave no lane padding)")
     p.add_argument("--epochs", type=int, default=100)
-    p.add_argument("--workload", choices=("2d", "3d", "elastic", "forward", "learned"),
+    p.add_argument("--workload", choices=("2d", "3d", "elastic", "forward", "learned", "maxwell"),
                    default="2d",
                    help="3d: the 3D acoustic gradient on a 354^3 grid instead (its own line); "
                         "elastic: the 2D elastic gradient on 1058 x 2114, 16 shots; "
-                        "forward: the plain 298 x 298 forward, one shot")
+                        "forward: the plain 298 x 298 forward, one shot; "
+                        "maxwell: the 3D vector Maxwell forward on 162^3, 8 shots")
     args = p.parse_args()
     if args.probe_copy:
         probe_copy()
@@ -794,6 +795,12 @@ def main():
             print(json.dumps({"error": "this build runs on one GPU"}))
             sys.exit(2)
         print(json.dumps(forward_line(args)))
+        return
+    if args.workload == "maxwell":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU"}))
+            sys.exit(2)
+        print(json.dumps(maxwell_line(args)))
         return
     if args.workload == "elastic":
         if args.gpus != 1:
@@ -903,6 +910,122 @@ def elastic_line(args) -> dict:
                    "time_steps": nt, "interior_nodes": res["n_int"],
                    "parallelism": "single GPU, shots batched per launch"},
         "roofline": res["roofline"], "cpu_baseline": cpu, "kernels": res["kernels"],
+    }
+
+
+MX_N, MX_SHOTS, MX_STEPS = 162, 8, 200
+
+
+def workload_maxwell(nt: int = MX_STEPS, shots: int = MX_SHOTS, n: int = MX_N):
+    """The 3D vector Maxwell forward: a conducting box of n^3 nodes (n - 2 interior a
+    side), dx = 1 / (n - 3), order 4, float32; eps 1 with an eps 4 sphere of radius
+    n / 5 at the centre; `shots` z dipoles (a Ricker current of 4 periods across the
+    run) on a ring in the mid plane, 16 probes of every component on a line;
+    dt = 0.4 dx / sqrt(3)."""
+    from cuwave_amd import maxwell, wave
+    from cuwave_amd.signals import ricker
+
+    dx = 1.0 / (n - 3)
+    dt = 0.4 * dx / np.sqrt(3.0)
+    sim = maxwell.DielectricWave((n,) * 3, (dx,) * 3, nt, dt, precision="float32",
+                                 space_order=4, permittivity1=1.0, permittivity2=4.0)
+    a = np.meshgrid(*(np.arange(m) for m in sim.Nx_padded), indexing="ij")
+    sphere = sum((x - n // 2) ** 2 for x in a) <= (n // 5) ** 2
+    indicator = sphere.astype(np.float32)
+    wavelet = ricker(np.arange(nt) * dt, 1.0, 4.0 / (nt * dt))
+    sources = []
+    for b in range(shots):
+        angle = 2 * np.pi * b / shots
+        at = [n // 2 + int(0.35 * n * np.cos(angle)), n // 2 + int(0.35 * n * np.sin(angle))]
+        sources.append(wave.Source(np.array([[2], [n // 2], [at[0]], [at[1]]]), wavelet[:, None]))
+    line = np.round(np.linspace(4, n - 5, 16)).astype(int)
+    sensors = np.concatenate([np.stack([np.full(16, c), np.full(16, n // 3), line,
+                                        np.full(16, n // 2)]) for c in range(3)], axis=1)
+    return sim, indicator, sources, sensors
+
+
+def maxwell_line(args) -> dict:
+    """Gcell-updates/s of the 3D vector Maxwell forward (interior nodes x shots x steps),
+    the steps' launches timed with HIP events on the library's stream."""
+    import torch
+
+    from cuwave_amd import maxwell
+
+    nt = args.nt if args.nt != STEPS else MX_STEPS
+    shots = args.shots if args.shots != SHOTS else MX_SHOTS
+    sim, indicator, sources, sensors = workload_maxwell(nt, shots)
+    n_int = int(np.prod([n - 2 for n in sim.Nx]))
+    mat = sim.build_materials(indicator)
+    eng = maxwell.MaxwellEngine(sim, mat, len(sources))
+    src, signal = eng.sources(mat, sources)
+    rec = eng.sensors(sensors)
+    record = torch.zeros((sim.N, rec.count, eng.lanes), dtype=sim.dtype, device=eng.device)
+    U = eng.levels(2)
+    stream = torch.cuda.current_stream()  # the stream the library launches on
+
+    def run():
+        U.zero_()
+        for t in range(sim.N):
+            u = eng.step(U[t % 2], U[1 - t % 2], U[t % 2])
+            src.inject(u, signal, t)
+            rec.record(u, record, t)
+
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        run()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    step_ms = start.elapsed_time(stop) / (args.steps * sim.N)
+    # a step's algorithmic bytes: the curl reads u1's 3 components and writes 3 pair
+    # curls, the update reads the 3 curls, u0 and u1 and writes u2, per lane; the
+    # point inverse inertia (3 a node) once for every lane
+    per_node = (18 * eng.lanes + 3) * 4
+    achieved = per_node * n_int / (step_ms * 1e-3)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import maxwell as M
+        from cuwave_amd.stencils import staggered_weights
+
+        sample = 4
+        inner = (slice(None), *(slice(0, n) for n in sim.Nx))
+        minv = mat["minv"][inner]
+        pos = np.asarray(sources[0].position)
+        weight = sim.excitation_weights(mat, sim.flatten_indices(pos))
+        times = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            M.simulate(minv, staggered_weights(2, sim.stencil).astype(np.float32),
+                       [np.float32(1.0 / d) for d in sim.dx], sim.step_factors(),
+                       np.float32(1.0), sim.abi_grid(1).conductor, sample, tuple(pos), weight,
+                       np.asarray(sources[0].signal, dtype=np.float32), tuple(sensors))
+            times.append(time.perf_counter() - c0)
+        med = float(np.median(times))
+        cpu = {"value": n_int * sample / med / 1e9, "unit": "Gcell-updates/s", "cores": 1,
+               "kind": "port",
+               "sample": f"1 shot, the full {sim.Nx[0]}^3 grid, the first {sample} steps through "
+                         f"the numpy oracle; median of 3 runs ({med:.1f} s a run)"}
+    return {
+        "metric": "Gcell-updates/s (shot-batched 3D vector Maxwell forward)",
+        "value": n_int * shots * nt * args.steps / elapsed / 1e9, "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "3D vector Maxwell forward: traces of every shot, a conducting "
+                               "box with a dielectric sphere",
+                   "grid": f"{sim.Nx[0]}^3", "space_order": 4, "shots": shots,
+                   "receivers": int(sensors.shape[1]), "time_steps": nt, "interior_nodes": n_int,
+                   "parallelism": "single GPU, shots batched per launch"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None},
+        "cpu_baseline": cpu,
+        "kernels": {"step_ms": step_ms, "bytes_per_node": per_node,
+                    "note": "a step's launches (curl, update, sources, probes) on the stream"},
     }
 
 

@@ -330,6 +330,10 @@ class MaxwellEngine(ElasticEngine):
         _abi.call("cw_maxwell_step", ctypes.byref(a), _abi.stream())
         return u2
 
+    def pair(self, adjoint: bool = False, with_pml: bool = True) -> "MaxwellEngine":
+        """The step's launches as the elastic engine's pair offers them (no pml here)."""
+        return self
+
     def gradient_fields(self) -> dict:
         sim = self.sim
         return {"mass": torch.zeros((sim.ncomp, *sim.Nx_padded), dtype=sim.dtype,

@@ -79,6 +79,13 @@ def adjoint_signal(sim: Simulation, dphi, sensors, scale: float = 1.0, delay: in
 
 
 # --------------------------------------------------------------------- helpers
+def _maxwell(sim: Simulation) -> bool:
+    """Whether `sim` is of the vector Maxwell family, whose drivers live in `maxwell.py`."""
+    from .maxwell import MaxwellWave
+
+    return isinstance(sim, MaxwellWave)
+
+
 def _elastic(sim: Simulation) -> bool:
     """Whether `sim` is of the elastic family, whose drivers live in `elastic.py`."""
     return sim.ncomp > 1 or type(sim).__name__ == "ElasticWave"
@@ -223,6 +230,10 @@ def sensitivity(sim: Simulation, source: Source, indicator, sensors, objective):
     transpose with both gradient densities fused in and the transposed PML, the
     layer then frozen out of the gradient (cuwave/sensitivity.py:384-497).
     """
+    if _maxwell(sim):
+        from . import maxwell
+
+        return maxwell.sensitivity(sim, source, indicator, sensors, objective)
     if _elastic(sim):
         from . import elastic
 
@@ -486,6 +497,9 @@ def reconstruction_sensitivity(sim: Simulation, source: Source, indicator, senso
     `strip` node count, the `drift` of the rebuilt initial state, the steps per
     `segment` and the forward `sweeps`.
     """
+    if _maxwell(sim):
+        raise NotImplementedError("reconstruction_sensitivity is not built for the vector "
+                                  "Maxwell family; use sensitivity")
     if _elastic(sim):
         from . import elastic
 

@@ -215,6 +215,9 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
         raise NotImplementedError("devices= shot dealing is not part of this build (one GPU)")
     require_full_domain(sim)
     sources = list(sources)
+    if sim.ncomp > 1 and type(sim).__name__ != "ElasticWave":
+        raise NotImplementedError("misfit_gradient is not built for the vector Maxwell family; "
+                                  "use sensitivity")
     if type(sim).__name__ == "ElasticWave" and adjoint in (sensitivity, reconstruction_sensitivity):
         from . import elastic
 

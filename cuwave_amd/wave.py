@@ -300,7 +300,14 @@ def simulate_shots(sim: Simulation, sources, indicator, sensors=None, record_eve
     steps when that is given.
     """
     from .engine import Propagator, sensor_points, source_points
+    from .maxwell import MaxwellWave
 
+    if isinstance(sim, MaxwellWave):
+        from . import maxwell
+
+        if record_every is not None:
+            raise NotImplementedError("record_every is built for the pressure family")
+        return maxwell.simulate_shots(sim, sources, indicator, sensors)
     if sim.ncomp > 1 or type(sim).__name__ == "ElasticWave":
         from . import elastic
 
@@ -495,9 +502,10 @@ def stable_timestep(sim: Simulation, indicator, iterations: int = 60,
         mask[(c, *inner)] = True
     if sim.ncomp > 1 or type(sim).__name__ == "ElasticWave":
         from .elastic import ElasticEngine
+        from .maxwell import MaxwellEngine, MaxwellWave
 
         mat = sim.build_materials(indicator)
-        eng = ElasticEngine(sim, mat, 1)
+        eng = (MaxwellEngine if isinstance(sim, MaxwellWave) else ElasticEngine)(sim, mat, 1)
         pair = eng.pair()
         slots = eng.levels(3)
 

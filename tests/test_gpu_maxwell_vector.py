@@ -201,3 +201,39 @@ def test_the_gradient_matches_finite_differences(Nx, order):
             costs.append(objective(um)[0])
         fd = (costs[0] - costs[1]) / (2 * h)
         assert abs(g[node] - fd) <= 1e-5 * abs(fd) + 1e-10 * abs(cost), (node, g[node], fd)
+
+
+def test_the_top_level_drivers_reach_the_family():
+    from cuwave_amd import sensitivity as S
+    from cuwave_amd import wave as W
+
+    print("maxwell top-level drivers", flush=True)
+    sim = vector_sim((12, 13, 14), 4, "mixed", N=25)
+    rng = np.random.default_rng(9)
+    gamma = rng.random(sim.Nx_padded)
+    source = shots(sim, rng, 1)[0]
+    sensors = sensor_points(sim, rng, 3)
+    mine = MX.simulate(sim, source, gamma, sensors)
+    top = W.simulate(sim, source, gamma, sensors)
+    assert all(torch.equal(a, b) for a, b in zip(mine, top))
+
+    def objective(um):
+        return 0.5 * float(torch.sum(um * um)), um
+
+    _, g1, _, _ = MX.sensitivity(sim, source, gamma, sensors, objective)
+    _, g2, _, _ = S.sensitivity(sim, source, gamma, sensors, objective)
+    assert torch.equal(g1["mass"], g2["mass"])
+
+
+def test_the_measured_timestep_sits_at_the_yee_limit():
+    # a uniform vacuum at order 2: c dt <= dx / sqrt(ndim) on the infinite lattice,
+    # which a closed box approaches from above
+    print("maxwell stable timestep", flush=True)
+    from cuwave_amd.wave import stable_timestep
+
+    n = 24
+    sim = MX.DielectricWave((n,) * 3, (1.0 / n,) * 3, 10, 0.1 / n, precision="float64",
+                            permittivity1=1.0, permittivity2=4.0, boundary=Magnetic)
+    dt = stable_timestep(sim, np.zeros(sim.Nx_padded), iterations=200, safety=1.0)
+    limit = (1.0 / n) / np.sqrt(3.0)
+    assert limit <= dt <= 1.1 * limit, (dt, limit)

@@ -268,3 +268,18 @@ def test_indices_carry_the_component_row():
                             sim.strides[0] + 2 * sim.strides[1] + 3]
     with pytest.raises(ValueError, match="component"):
         sim.flatten_indices(np.array([[3], [3], [4], [5]]))
+
+
+def test_the_drivers_not_built_for_the_family_refuse_it():
+    from cuwave_amd import sensitivity as S
+    from cuwave_amd import utils as U
+    from cuwave_amd.wave import Source
+
+    sim = vector_sim((9, 11, 13))
+    source = Source(np.array([[0], [4], [5], [6]]), np.zeros((sim.N, 1)))
+    sensors = np.array([[1], [4], [5], [7]])
+    with pytest.raises(NotImplementedError, match="Maxwell"):
+        S.reconstruction_sensitivity(sim, source, np.zeros(sim.Nx_padded), sensors, None)
+    with pytest.raises(NotImplementedError, match="Maxwell"):
+        U.misfit_gradient(sim, [source], np.zeros(sim.Nx_padded), sensors,
+                          np.zeros((1, sim.N, 1)))
