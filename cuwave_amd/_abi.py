@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 11
+ABI_VERSION = 12
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -256,6 +256,44 @@ class MaxwellStepArgs(ctypes.Structure):
     ]
 
 
+class AnisoGrid(ctypes.Structure):
+    _fields_ = [
+        ("ndim", I32),
+        ("n", I32 * 3),
+        ("np", I32 * 3),
+        ("nbatch", I32),
+        ("lanes", I32),
+        ("dtype", I32),
+        ("step", F64),
+    ]
+
+
+class AnisoStepArgs(ctypes.Structure):
+    _fields_ = [
+        ("grid", AnisoGrid),
+        ("table", VP),
+        ("minv", VP),
+        ("cell", VP),
+        ("u0", VP),
+        ("u1", VP),
+        ("u2", VP),
+    ]
+
+
+class AnisoGradientArgs(ctypes.Structure):
+    _fields_ = [
+        ("grid", AnisoGrid),
+        ("table", VP),
+        ("g_mass", VP),
+        ("g_cell", VP),
+        ("u0", VP),
+        ("u1", VP),
+        ("u2", VP),
+        ("l1", VP),
+        ("mass_factor", F64),
+    ]
+
+
 class ElasticPointArgs(ctypes.Structure):
     _fields_ = [
         ("starts", VP),
@@ -338,6 +376,9 @@ SIGNATURES = {
     "cw_elastic_points": ([ctypes.POINTER(ElasticGrid), ctypes.POINTER(ElasticPointArgs), VP], I32),
     "cw_maxwell_step": ([ctypes.POINTER(MaxwellStepArgs), VP], I32),
     "cw_maxwell_points": ([ctypes.POINTER(MaxwellGrid), ctypes.POINTER(ElasticPointArgs), VP], I32),
+    "cw_aniso_step": ([ctypes.POINTER(AnisoStepArgs), VP], I32),
+    "cw_aniso_gradient": ([ctypes.POINTER(AnisoGradientArgs), VP], I32),
+    "cw_aniso_points": ([ctypes.POINTER(AnisoGrid), ctypes.POINTER(ElasticPointArgs), VP], I32),
     "cw_conv3x3_bf16_fwd": ([ctypes.POINTER(Conv3x3Args), VP], I32),
     "cw_conv3x3_bf16_bwd_data": ([ctypes.POINTER(Conv3x3Args), VP], I32),
     "cw_conv3x3_bf16_bwd_weight": ([ctypes.POINTER(Conv3x3Args), VP], I32),

@@ -2,7 +2,8 @@
 
 Every translation unit (the C ABI of the pressure family in `csrc/scalar.hip`
 and of the elastic family in `csrc/elastic.hip`, of the vector Maxwell
-family in `csrc/maxwell.hip`, the U-Net's matrix-core
+family in `csrc/maxwell.hip`, of the cell-assembled elastic family in
+`csrc/anisotropic.hip`, the U-Net's matrix-core
 kernels in `csrc/unet.hip`, the step kernels of one
 precision and radius in each `csrc/steps_*.hip` and `csrc/elastic_*.hip`)
 compiles in its own process, all at once, and the objects link into one
@@ -39,7 +40,8 @@ FLAGS = [
 
 def sources() -> list[Path]:
     return [CSRC / "scalar.hip", *sorted(CSRC.glob("steps_*.hip")), CSRC / "elastic.hip",
-            *sorted(CSRC.glob("elastic_*.hip")), CSRC / "maxwell.hip", CSRC / "unet.hip"]
+            *sorted(CSRC.glob("elastic_*.hip")), CSRC / "maxwell.hip", CSRC / "anisotropic.hip",
+            CSRC / "unet.hip"]
 
 
 def _run(cmd: list[str], verbose: bool) -> None:

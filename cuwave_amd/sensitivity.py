@@ -27,7 +27,7 @@ import torch
 
 from .boundary import pml_nodes
 from .engine import Propagator, sensor_points, source_points
-from .wave import Simulation, Source, as_numpy
+from .wave import Simulation, Source, as_numpy, family_drivers
 
 # cancellation past which B(w, w) - B(u, u) has eaten too much of the mantissa
 CANCELLATION_LIMIT = {"float32": 1e5, "float64": 1e12}
@@ -79,16 +79,10 @@ def adjoint_signal(sim: Simulation, dphi, sensors, scale: float = 1.0, delay: in
 
 
 # --------------------------------------------------------------------- helpers
-def _maxwell(sim: Simulation) -> bool:
-    """Whether `sim` is of the vector Maxwell family, whose drivers live in `maxwell.py`."""
-    from .maxwell import MaxwellWave
-
-    return isinstance(sim, MaxwellWave)
-
-
 def _elastic(sim: Simulation) -> bool:
-    """Whether `sim` is of the elastic family, whose drivers live in `elastic.py`."""
-    return sim.ncomp > 1 or type(sim).__name__ == "ElasticWave"
+    """Whether `sim` is of a vector family (elastic, cell-assembled elastic, vector
+    Maxwell), which the pressure family's drivers do not step."""
+    return sim.ncomp > 1 or type(sim).__name__ == "ElasticWave" or family_drivers(sim) is not None
 
 
 def _require(sim: Simulation, source: Source, sensors) -> None:
@@ -230,10 +224,9 @@ def sensitivity(sim: Simulation, source: Source, indicator, sensors, objective):
     transpose with both gradient densities fused in and the transposed PML, the
     layer then frozen out of the gradient (cuwave/sensitivity.py:384-497).
     """
-    if _maxwell(sim):
-        from . import maxwell
-
-        return maxwell.sensitivity(sim, source, indicator, sensors, objective)
+    own = family_drivers(sim)
+    if own is not None:
+        return own.sensitivity(sim, source, indicator, sensors, objective)
     if _elastic(sim):
         from . import elastic
 
@@ -497,9 +490,9 @@ def reconstruction_sensitivity(sim: Simulation, source: Source, indicator, senso
     `strip` node count, the `drift` of the rebuilt initial state, the steps per
     `segment` and the forward `sweeps`.
     """
-    if _maxwell(sim):
-        raise NotImplementedError("reconstruction_sensitivity is not built for the vector "
-                                  "Maxwell family; use sensitivity")
+    if family_drivers(sim) is not None:
+        raise NotImplementedError(f"reconstruction_sensitivity is not built for "
+                                  f"{type(sim).__name__}; use sensitivity")
     if _elastic(sim):
         from . import elastic
 

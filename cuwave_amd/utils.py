@@ -215,9 +215,11 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
         raise NotImplementedError("devices= shot dealing is not part of this build (one GPU)")
     require_full_domain(sim)
     sources = list(sources)
-    if sim.ncomp > 1 and type(sim).__name__ != "ElasticWave":
-        raise NotImplementedError("misfit_gradient is not built for the vector Maxwell family; "
-                                  "use sensitivity")
+    from .wave import family_drivers
+
+    if family_drivers(sim) is not None:
+        raise NotImplementedError(f"misfit_gradient is not built for {type(sim).__name__}; "
+                                  f"use sensitivity")
     if type(sim).__name__ == "ElasticWave" and adjoint in (sensitivity, reconstruction_sensitivity):
         from . import elastic
 

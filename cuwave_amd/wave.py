@@ -263,6 +263,23 @@ class Simulation:
         return 1.0 / self.dt**2
 
 
+def family_drivers(sim: Simulation):
+    """The module whose drivers step `sim` when it is of the vector Maxwell or the
+    cell-assembled elastic family (each has its own kernels), else None."""
+    from .anisotropic import AnisotropicElasticWave
+    from .maxwell import MaxwellWave
+
+    if isinstance(sim, MaxwellWave):
+        from . import maxwell
+
+        return maxwell
+    if isinstance(sim, AnisotropicElasticWave):
+        from . import anisotropic
+
+        return anisotropic
+    return None
+
+
 def shot_lanes(nbatch: int) -> int:
     """The innermost extent of a batch's per-shot arrays: a power of two >= nbatch, at most 64."""
     nbatch = int(nbatch)
@@ -300,14 +317,12 @@ def simulate_shots(sim: Simulation, sources, indicator, sensors=None, record_eve
     steps when that is given.
     """
     from .engine import Propagator, sensor_points, source_points
-    from .maxwell import MaxwellWave
 
-    if isinstance(sim, MaxwellWave):
-        from . import maxwell
-
+    own = family_drivers(sim)
+    if own is not None:
         if record_every is not None:
             raise NotImplementedError("record_every is built for the pressure family")
-        return maxwell.simulate_shots(sim, sources, indicator, sensors)
+        return own.simulate_shots(sim, sources, indicator, sensors)
     if sim.ncomp > 1 or type(sim).__name__ == "ElasticWave":
         from . import elastic
 
@@ -500,12 +515,15 @@ def stable_timestep(sim: Simulation, indicator, iterations: int = 60,
         stag = getattr(sim, "radius", None) is not None and sim.ncomp > 1
         inner = tuple(slice(1, n - 1 - (stag and d == c)) for d, n in enumerate(sim.Nx))
         mask[(c, *inner)] = True
-    if sim.ncomp > 1 or type(sim).__name__ == "ElasticWave":
+    if sim.ncomp > 1 or type(sim).__name__ == "ElasticWave" or family_drivers(sim) is not None:
+        from .anisotropic import AnisoEngine, AnisotropicElasticWave
         from .elastic import ElasticEngine
         from .maxwell import MaxwellEngine, MaxwellWave
 
         mat = sim.build_materials(indicator)
-        eng = (MaxwellEngine if isinstance(sim, MaxwellWave) else ElasticEngine)(sim, mat, 1)
+        engine = (MaxwellEngine if isinstance(sim, MaxwellWave) else
+                  AnisoEngine if isinstance(sim, AnisotropicElasticWave) else ElasticEngine)
+        eng = engine(sim, mat, 1)
         pair = eng.pair()
         slots = eng.levels(3)
 

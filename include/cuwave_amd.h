@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 11
+#define CW_ABI_VERSION 12
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -442,6 +442,52 @@ int cw_maxwell_step(const cw_maxwell_step_args *args, void *stream);
 /* points of a Maxwell field, as cw_elastic_points */
 int cw_maxwell_points(const cw_maxwell_grid *grid, const cw_elastic_point_args *args,
                       void *stream);
+
+/* ------------------------------------------------------------ cell-assembled elastic
+ * The anisotropic elastic family: collocated displacement components, the
+ * operator gathered cell by cell from one cell's stencil K = -B^T C B
+ * (cuwave/anisotropic.py:1-160, cuwave/kernels/anisotropic.cu:72-397),
+ * order 2. Per-shot arrays are (ndim, *np, lanes); K is (nloc, nloc),
+ * nloc = 2^ndim ndim, row corner ndim + component, corner c having bit d set
+ * on the plus side of axis d. */
+
+typedef struct cw_aniso_grid {
+  int32_t ndim;   /* 1, 2 or 3 */
+  int32_t n[3];   /* logical extents, ghosts included */
+  int32_t np[3];  /* padded extents */
+  int32_t nbatch, lanes, dtype;
+  double step;    /* the update factor dt^2 (the spacing sits in K) */
+} cw_aniso_grid;
+
+/* one step over a batch (fd_kernel, anisotropic.cu:367-397): the
+ * cell-gathered force on every component, u2 = -u0 + 2 u1 + dt^2 minv force;
+ * u2 may be u0 */
+typedef struct cw_aniso_step_args {
+  cw_aniso_grid grid;
+  const void *table; /* K in the grid's dtype */
+  const void *minv;  /* (*np) lumped inverse inertia, zero on a clamped wall */
+  const void *cell;  /* (*np) the cell material at each cell's low corner */
+  const void *u0, *u1;
+  void *u2;
+} cw_aniso_step_args;
+
+/* the densities of a forward triplet against the adjoint l1
+ * (gradient_kernel, anisotropic_sensitivity.cu:111-156), the shots summed:
+ * g_mass -= mass_factor sum l1 (u2 - 2 u1 + u0) at every node, g_cell -= l1
+ * K u1 over the cell whose low corner the node is */
+typedef struct cw_aniso_gradient_args {
+  cw_aniso_grid grid;
+  const void *table;
+  void *g_mass, *g_cell; /* (*np) each */
+  const void *u0, *u1, *u2, *l1;
+  double mass_factor;    /* rho0 V / dt^2 */
+} cw_aniso_gradient_args;
+
+int cw_aniso_step(const cw_aniso_step_args *args, void *stream);
+int cw_aniso_gradient(const cw_aniso_gradient_args *args, void *stream);
+/* points of a field (kinds 0 inject, 1 record, 2 replay), as cw_elastic_points */
+int cw_aniso_points(const cw_aniso_grid *grid, const cw_elastic_point_args *args,
+                    void *stream);
 
 /* ------------------------------------------------------------ U-Net
  * The learned preconditioner's kernels (no reference counterpart: the
