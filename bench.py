@@ -770,12 +770,14 @@ def main():
     p.add_argument("--shot-batch", type=int, default=16,
                    help="learned: shots a sweep (24 shots as 16 + 8 leave no lane padding)")
     p.add_argument("--epochs", type=int, default=100)
-    p.add_argument("--workload", choices=("2d", "3d", "elastic", "forward", "learned", "maxwell"),
+    p.add_argument("--workload",
+                   choices=("2d", "3d", "elastic", "forward", "learned", "maxwell", "aniso"),
                    default="2d",
                    help="3d: the 3D acoustic gradient on a 354^3 grid instead (its own line); "
                         "elastic: the 2D elastic gradient on 1058 x 2114, 16 shots; "
                         "forward: the plain 298 x 298 forward, one shot; "
-                        "maxwell: the 3D vector Maxwell forward on 162^3, 8 shots")
+                        "maxwell: the 3D vector Maxwell forward on 162^3, 8 shots; "
+                        "aniso: the 2D cell-assembled elastic forward on 1026^2, 16 shots")
     args = p.parse_args()
     if args.probe_copy:
         probe_copy()
@@ -795,6 +797,12 @@ def main():
             print(json.dumps({"error": "this build runs on one GPU"}))
             sys.exit(2)
         print(json.dumps(forward_line(args)))
+        return
+    if args.workload == "aniso":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU"}))
+            sys.exit(2)
+        print(json.dumps(aniso_line(args)))
         return
     if args.workload == "maxwell":
         if args.gpus != 1:
@@ -1026,6 +1034,120 @@ def maxwell_line(args) -> dict:
         "cpu_baseline": cpu,
         "kernels": {"step_ms": step_ms, "bytes_per_node": per_node,
                     "note": "a step's launches (curl, update, sources, probes) on the stream"},
+    }
+
+
+AN_N, AN_SHOTS, AN_STEPS = 1026, 16, 200
+
+
+def workload_aniso(nt: int = AN_STEPS, shots: int = AN_SHOTS, n: int = AN_N):
+    """The 2D cell-assembled elastic forward: n^2 nodes, dx = 1 m, float32, traction
+    faces but a clamped bottom; rho0 2000, vp 3000, vs 1500 with an orthotropic C
+    (C11 2 rho0 vp^2, C22 rho0 vp^2, C12 rho0 (vp^2 - 2 vs^2), C66 rho0 vs^2); gamma
+    1 with a gamma 1.5 disk of radius n / 8; `shots` vertical forces (Ricker, 4
+    periods across the run) along row n / 8, 32 receivers of both components on row
+    2; dt = 0.25 dx / (sqrt(2) c), c = sqrt(2) vp the fastest speed C carries."""
+    from cuwave_amd import anisotropic, wave
+    from cuwave_amd.boundary import Clamped, Traction
+    from cuwave_amd.signals import ricker
+
+    rho, vp, vs = 2000.0, 3000.0, 1500.0
+    C = rho * np.array([[2 * vp**2, vp**2 - 2 * vs**2, 0.0],
+                        [vp**2 - 2 * vs**2, vp**2, 0.0], [0.0, 0.0, vs**2]])
+    dt = 0.25 / (np.sqrt(2.0) * np.sqrt(2.0) * vp)
+    sim = anisotropic.AnisotropicElasticWave(
+        (n, n), (1.0, 1.0), nt, dt, precision="float32", density=rho, wavespeed_p=vp,
+        wavespeed_s=vs, C=C, boundary=((Traction, Clamped), (Traction, Traction)))
+    a0, a1 = np.meshgrid(*(np.arange(m) for m in sim.Nx_padded), indexing="ij")
+    disk = (a0 - n // 2) ** 2 + (a1 - n // 2) ** 2 <= (n // 8) ** 2
+    indicator = np.where(disk, 1.5, 1.0).astype(np.float32)
+    wavelet = ricker(np.arange(nt) * dt, 1.0, 4.0 / (nt * dt))
+    cols = np.round(np.linspace(8, n - 9, shots)).astype(int)
+    sources = [wave.Source(np.array([[0], [n // 8], [c]]), wavelet[:, None]) for c in cols]
+    line = np.round(np.linspace(4, n - 5, 32)).astype(int)
+    sensors = np.concatenate([np.stack([np.full(32, c), np.full(32, 2), line]) for c in range(2)],
+                             axis=1)
+    return sim, indicator, sources, sensors
+
+
+def aniso_line(args) -> dict:
+    """Gcell-updates/s of the 2D cell-assembled elastic forward (interior nodes x shots x
+    steps), the steps' launches timed with HIP events on the library's stream."""
+    import torch
+
+    from cuwave_amd import anisotropic
+
+    nt = args.nt if args.nt != STEPS else AN_STEPS
+    shots = args.shots if args.shots != SHOTS else AN_SHOTS
+    sim, indicator, sources, sensors = workload_aniso(nt, shots)
+    n_int = int(np.prod([n - 2 for n in sim.Nx]))
+    mat = sim.build_materials(indicator)
+    eng = anisotropic.AnisoEngine(sim, mat, len(sources))
+    src, signal = eng.sources(mat, sources)
+    rec = eng.sensors(sensors)
+    record = torch.zeros((sim.N, rec.count, eng.lanes), dtype=sim.dtype, device=eng.device)
+    U = eng.levels(2)
+    stream = torch.cuda.current_stream()  # the stream the library launches on
+
+    def run():
+        U.zero_()
+        for t in range(sim.N):
+            u = eng.step(U[t % 2], U[1 - t % 2], U[t % 2])
+            src.inject(u, signal, t)
+            rec.record(u, record, t)
+
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        run()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    step_ms = start.elapsed_time(stop) / (args.steps * sim.N)
+    # a step's algorithmic bytes: u0 and u1 read and u2 written, 2 components a lane;
+    # the inverse inertia and the cell material once for every lane
+    per_node = (6 * eng.lanes + 2) * 4
+    achieved = per_node * n_int / (step_ms * 1e-3)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import anisotropic as O
+
+        sample = 16
+        inner = tuple(slice(0, n) for n in sim.Nx)
+        pos = np.asarray(sources[0].position)
+        weight = (np.float32(sim.dt**2) * mat["minv"][inner][tuple(pos[1:])]).astype(np.float32)
+        times = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            O.simulate(mat["minv"][inner], mat["cell"][inner], mat["table"],
+                       np.float32(sim.dt**2), sample, tuple(pos), weight,
+                       np.asarray(sources[0].signal, dtype=np.float32), tuple(sensors))
+            times.append(time.perf_counter() - c0)
+        med = float(np.median(times))
+        cpu = {"value": n_int * sample / med / 1e9, "unit": "Gcell-updates/s", "cores": 1,
+               "kind": "port",
+               "sample": f"1 shot, the full {sim.Nx[0]}^2 grid, the first {sample} steps through "
+                         f"the numpy oracle; median of 3 runs ({med:.1f} s a run)"}
+    return {
+        "metric": "Gcell-updates/s (shot-batched 2D cell-assembled elastic forward)",
+        "value": n_int * shots * nt * args.steps / elapsed / 1e9, "unit": "Gcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2D anisotropic elastic forward: traces of every shot, an "
+                               "orthotropic medium with a denser disk",
+                   "grid": f"{sim.Nx[0]}^2", "space_order": 2, "shots": shots,
+                   "receivers": int(sensors.shape[1]), "time_steps": nt, "interior_nodes": n_int,
+                   "parallelism": "single GPU, shots batched per launch"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None},
+        "cpu_baseline": cpu,
+        "kernels": {"step_ms": step_ms, "bytes_per_node": per_node,
+                    "note": "a step's launches (the step, sources, probes) on the stream"},
     }
 
 
