@@ -35,7 +35,12 @@ def test_struct_layouts_match_the_header():
     text = (ROOT / "include" / "cuwave_amd.h").read_text()
     for struct, cls in (("cw_grid", _abi.Grid), ("cw_points", _abi.Points),
                         ("cw_strip", _abi.Strip), ("cw_levels", _abi.Levels),
-                        ("cw_backward_args", _abi.BackwardArgs)):
+                        ("cw_backward_args", _abi.BackwardArgs),
+                        ("cw_maxwell_grid", _abi.MaxwellGrid),
+                        ("cw_maxwell_step_args", _abi.MaxwellStepArgs),
+                        ("cw_aniso_grid", _abi.AnisoGrid),
+                        ("cw_aniso_step_args", _abi.AnisoStepArgs),
+                        ("cw_aniso_gradient_args", _abi.AnisoGradientArgs)):
         body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (struct, struct), text, re.S).group(1)
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
         fields = re.findall(r"(\w+)\s*(?:\[\d+\])*\s*[,;]", body)
@@ -88,6 +93,45 @@ def test_bad_arguments_are_refused_before_any_launch():
     b.grid = sim.abi_grid(1)
     with pytest.raises(RuntimeError):
         _abi.call("cw_scalar_backward", ctypes.byref(b), 0, 1, None)
+
+
+def test_the_vector_families_refuse_bad_arguments_before_any_launch():
+    from cuwave_amd import anisotropic, maxwell
+
+    mx = maxwell.DielectricWave((10, 11, 12), (0.1,) * 3, 5, 0.01, permittivity1=1.0,
+                                permittivity2=2.0)
+    a = _abi.MaxwellStepArgs()
+    a.grid = mx.abi_grid(3)
+    with pytest.raises(RuntimeError, match="curl scratch"):
+        _abi.call("cw_maxwell_step", ctypes.byref(a), None)
+    a.grid.ndim = 1
+    with pytest.raises(RuntimeError, match="no curl"):
+        _abi.call("cw_maxwell_step", ctypes.byref(a), None)
+    a.grid = mx.abi_grid(3)
+    a.grid.lanes = 3
+    with pytest.raises(RuntimeError, match="power of two"):
+        _abi.call("cw_maxwell_step", ctypes.byref(a), None)
+    a.grid = mx.abi_grid(3)
+    a.grid.conductor = 1 << 6
+    with pytest.raises(RuntimeError, match="one bit a face"):
+        _abi.call("cw_maxwell_step", ctypes.byref(a), None)
+    an = anisotropic.AnisotropicElasticWave((10, 12), (0.1, 0.1), 5, 1e-3, density=1.0,
+                                            wavespeed_p=2.0, wavespeed_s=1.0)
+    s = _abi.AnisoStepArgs()
+    s.grid = an.abi_grid(2)
+    with pytest.raises(RuntimeError, match="table"):
+        _abi.call("cw_aniso_step", ctypes.byref(s), None)
+    s.grid.np[1] = 13
+    with pytest.raises(RuntimeError, match="padded"):
+        _abi.call("cw_aniso_step", ctypes.byref(s), None)
+    g = _abi.AnisoGradientArgs()
+    g.grid = an.abi_grid(2)
+    with pytest.raises(RuntimeError, match="accumulators"):
+        _abi.call("cw_aniso_gradient", ctypes.byref(g), None)
+    p = _abi.ElasticPointArgs()
+    p.runs, p.kind = 1, 3
+    with pytest.raises(RuntimeError, match="points need"):
+        _abi.call("cw_aniso_points", ctypes.byref(an.abi_grid(2)), ctypes.byref(p), None)
 
 
 def test_lanes_hold_the_batch_as_a_power_of_two():
