@@ -15,12 +15,14 @@ reference's closures over a batch of shots, the shots innermost:
 - `sensitivity`: the stored-history adjoint of cuwave/sensitivity.py:482-497,
   the adjoint stepped by the step itself (the operator is symmetric) and the
   densities of each forward triplet against it from `cw_aniso_gradient`, then
-  chained onto the nodes (cuwave/anisotropic.py:457-506).
+  chained onto the nodes (cuwave/anisotropic.py:457-506);
+- `reconstruction_sensitivity`: the same with the forward field rebuilt by the
+  reverse march instead of stored.
 
 The host-side parametrization (the one-cell stencil, the lumped inertia, the
 harmonic cell means, the chain rule) runs in numpy in the simulation's
 precision, as the reference computes it. Not built here: a pml, damping, the
-blocked launch tier, `reconstruction_sensitivity` for this family.
+blocked launch tier.
 """
 
 from __future__ import annotations
@@ -392,3 +394,53 @@ def sensitivity(sim: AnisotropicElasticWave, source: Source, indicator, sensors,
         eng.gradient(grads, V[N - 1 - m], V[N - m], V[N + 1 - m], p)
     out = finalize_gradients(sim, grads["mass"].cpu().numpy(), grads["cell"].cpu().numpy(), mat)
     return cost, {k: torch.as_tensor(v, device=eng.device) for k, v in out.items()}, um, {}
+
+
+def reconstruction_sensitivity(sim: AnisotropicElasticWave, source: Source, indicator, sensors,
+                               objective, strip_dtype: str | None = None,
+                               checkpoints: int | None = None):
+    """Cost and gradients as `sensitivity`, the forward field rebuilt backwards instead of
+    stored (cuwave/sensitivity.py:500-750). With neither a pml nor damping every node is
+    reversible, so there is no strip: u^(t-2) = -u^t + 2 u^(t-1) + dt^2 minv force(u^(t-1))
+    + the source of step t, the step itself with u0 and u2 swapped, one level ahead of
+    the triplet the gradient pairs with lambda. `info` carries `strip` (0) and the
+    `drift` of the rebuilt initial state against the last level."""
+    if strip_dtype is not None or checkpoints is not None:
+        raise NotImplementedError("nothing is stored on this family's reverse march: no strip "
+                                  "dtype or checkpoints to choose")
+    _require(sim, [source], sensors)
+    mat = sim.build_materials(indicator)
+    eng = AnisoEngine(sim, mat, 1)
+    src, signal = eng.sources(mat, [source])
+    rec = eng.sensors(sensors)
+    record = torch.zeros((sim.N, rec.count, eng.lanes), dtype=sim.dtype, device=eng.device)
+    U = eng.levels(3)
+    N = sim.N
+    for t in range(N):
+        u = eng.step(U[t % 3], U[(t + 1) % 3], U[(t + 2) % 3])
+        src.inject(u, signal, t)
+        rec.record(u, record, t)
+    um = eng.shots(record)[0]
+    cost, dphi = objective(um)
+    dphi = torch.as_tensor(dphi, dtype=sim.dtype, device=eng.device)
+    adjoint = eng.lane_table(adjoint_signal(sim, dphi[None], sensors))
+    lin = sim.flatten_indices(sensors)
+    adj = eng.sensors(sensors, weight=sim.excitation_weights(mat, lin))
+    grads = eng.gradient_fields()
+    P = eng.levels(2)
+    # u^t sits in slot (t + 2) % 3: the march leaves hi = u^(N-1), mid = u^(N-2)
+    # and lo = u^(N-3)
+    hi, mid, lo = U[(N + 1) % 3], U[N % 3], U[(N + 2) % 3]
+    scale = float(torch.max(torch.abs(hi)))
+    for m in range(N):
+        p = eng.step(P[m % 2], P[1 - m % 2], P[m % 2])  # lambda^n, n = N - 1 - m
+        adj.inject(p, adjoint, m)
+        eng.gradient(grads, lo, mid, hi, p)  # u^(N-3-m), u^(N-2-m), u^(N-1-m)
+        if m < N - 1:
+            # u^(N-4-m) over the top level's slot, from step N - 2 - m
+            src.inject(eng.step(mid, lo, hi), signal, N - 2 - m)
+            hi, mid, lo = mid, lo, hi
+    drift = float(torch.max(torch.abs(torch.stack([lo, mid])))) / scale if scale > 0 else 0.0
+    out = finalize_gradients(sim, grads["mass"].cpu().numpy(), grads["cell"].cpu().numpy(), mat)
+    return (cost, {k: torch.as_tensor(v, device=eng.device) for k, v in out.items()}, um,
+            {"strip": 0, "drift": drift})

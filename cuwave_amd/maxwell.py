@@ -24,7 +24,9 @@ the shots innermost in every per-shot array as in the other families:
 - `sensitivity`: the stored-history adjoint, the adjoint step being the
   forward step itself (no pml breaks its symmetry) with the inertia density
   fused in (cuwave/maxwell.py:278-282, 400-445), chained through the harmonic
-  point means onto the nodal permittivity (:601-613).
+  point means onto the nodal permittivity (:601-613);
+- `reconstruction_sensitivity`: the same adjoint with the forward field
+  rebuilt level by level by the reverse march instead of stored.
 
 The host-side parametrization (the point inertia, the excitation weights,
 the chain rule) runs in numpy in the simulation's precision, as the reference
@@ -427,3 +429,50 @@ def sensitivity(sim: MaxwellWave, source: Source, indicator, sensors, objective)
         adj.adjoint_inject(l0, adjoint, m, grads["mass"], v1, mf)
     out = finalize_gradients(sim, grads["mass"].cpu().numpy(), mat["eps"])
     return cost, {k: torch.as_tensor(v, device=eng.device) for k, v in out.items()}, um, {}
+
+
+def reconstruction_sensitivity(sim: MaxwellWave, source: Source, indicator, sensors, objective,
+                               strip_dtype: str | None = None, checkpoints: int | None = None):
+    """Cost and gradients as `sensitivity`, the forward field rebuilt backwards instead of
+    stored (cuwave/sensitivity.py:500-750). With neither a pml nor damping every point
+    is reversible, so there is no strip: u^(n-2) = -u^n + 2 u^(n-1) + minv force(u^(n-1))
+    + the source of step n, the step itself with u0 and u2 swapped. `info` carries
+    `strip` (0) and the `drift` of the rebuilt initial state against the last level."""
+    if strip_dtype is not None or checkpoints is not None:
+        raise NotImplementedError("nothing is stored on this family's reverse march: no strip "
+                                  "dtype or checkpoints to choose")
+    _require(sim, [source], sensors)
+    mat = sim.build_materials(indicator)
+    eng = MaxwellEngine(sim, mat, 1)
+    src, signal = eng.sources(mat, [source])
+    rec = eng.sensors(sensors)
+    record = torch.zeros((sim.N, rec.count, eng.lanes), dtype=sim.dtype, device=eng.device)
+    U = eng.levels(2)
+    N = sim.N
+    for t in range(N):
+        u = eng.step(U[t % 2], U[1 - t % 2], U[t % 2])
+        src.inject(u, signal, t)
+        rec.record(u, record, t)
+    um = eng.shots(record)[0]
+    cost, dphi = objective(um)
+    dphi = torch.as_tensor(dphi, dtype=sim.dtype, device=eng.device)
+    adjoint = eng.lane_table(adjoint_signal(sim, dphi[None], sensors))
+    lin = sim.flatten_indices(sensors)
+    adj = eng.sensors(sensors, weight=sim.excitation_weights(mat, lin))
+    grads = eng.gradient_fields()
+    mf = sim.np_dtype(sim.adjoint_mass_factor())
+    P = eng.levels(2)
+    cur, prev = U[(N - 1) % 2], U[N % 2]  # u^(N-1), u^(N-2)
+    scale = float(torch.max(torch.abs(cur)))
+    for m in range(N):
+        n = N - 1 - m
+        l0, l1 = P[m % 2], P[1 - m % 2]
+        eng.step(l0, l1, l0, grads["mass"], cur, mf)
+        adj.adjoint_inject(l0, adjoint, m, grads["mass"], cur, mf)
+        # u^(n-2) over u^n's slot, then the pair moves down a level
+        src.inject(eng.step(cur, prev, cur), signal, n)
+        cur, prev = prev, cur
+    drift = float(torch.max(torch.abs(U))) / scale if scale > 0 else 0.0
+    out = finalize_gradients(sim, grads["mass"].cpu().numpy(), mat["eps"])
+    return (cost, {k: torch.as_tensor(v, device=eng.device) for k, v in out.items()}, um,
+            {"strip": 0, "drift": drift})

@@ -490,9 +490,10 @@ def reconstruction_sensitivity(sim: Simulation, source: Source, indicator, senso
     `strip` node count, the `drift` of the rebuilt initial state, the steps per
     `segment` and the forward `sweeps`.
     """
-    if family_drivers(sim) is not None:
-        raise NotImplementedError(f"reconstruction_sensitivity is not built for "
-                                  f"{type(sim).__name__}; use sensitivity")
+    own = family_drivers(sim)
+    if own is not None:
+        return own.reconstruction_sensitivity(sim, source, indicator, sensors, objective,
+                                              strip_dtype, checkpoints)
     if _elastic(sim):
         from . import elastic
 

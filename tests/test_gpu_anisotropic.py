@@ -6,6 +6,8 @@
 - the gradient kernel's two densities bit for bit against the oracle's;
 - `sensitivity` against central finite differences of the misfit
   (cuwave tests/anisotropic_test.py:315-363);
+- `reconstruction_sensitivity` (the field rebuilt by the reverse march)
+  against the stored-history gradient;
 - the top-level drivers reaching the family.
 """
 
@@ -186,3 +188,29 @@ def test_the_top_level_drivers_reach_the_family():
     _, g1, _, _ = AN.sensitivity(sim, source, gamma, sensors, objective)
     _, g2, _, _ = S.sensitivity(sim, source, gamma, sensors, objective)
     assert torch.equal(g1["stiff"], g2["stiff"]) and torch.equal(g1["mass"], g2["mass"])
+
+
+@pytest.mark.parametrize("Nx, faces, precision, bound",
+                         [((18, 20), "mixed", "float64", 1e-9), ((9, 10, 11), "clamped", "float64", 1e-9),
+                          ((20, 22), "traction", "float32", 2e-3)])
+def test_the_reverse_march_reproduces_the_stored_gradient(Nx, faces, precision, bound):
+    # no pml: the leapfrog runs backwards exactly but for rounding
+    print(f"anisotropic reconstruction {Nx} {faces} {precision}", flush=True)
+    from cuwave_amd import sensitivity as S
+
+    sim = aniso_sim(Nx, faces, precision=precision, N=60)
+    rng = np.random.default_rng(13)
+    gamma = (0.5 + rng.random(sim.Nx_padded)).astype(sim.np_dtype)
+    source = shots(sim, rng, 1)[0]
+    sensors = sensor_points(sim, rng, 4)
+
+    def objective(um):
+        return 0.5 * float(torch.sum(um * um)), um
+
+    c1, g1, u1, _ = AN.sensitivity(sim, source, gamma, sensors, objective)
+    c2, g2, u2, info = S.reconstruction_sensitivity(sim, source, gamma, sensors, objective)
+    assert c1 == c2 and torch.equal(u1, u2)
+    for k in ("mass", "stiff"):
+        gap = float(torch.linalg.vector_norm(g2[k] - g1[k]) / torch.linalg.vector_norm(g1[k]))
+        assert gap < bound, (k, gap)
+    assert info["strip"] == 0 and info["drift"] < (1e-10 if precision == "float64" else 1e-3)

@@ -8,7 +8,8 @@
 - one adjoint step with its inertia density, bit for bit against the
   oracle's GRADIENT step;
 - `sensitivity` against central finite differences of the misfit
-  (cuwave tests/maxwell_test.py:839-846).
+  (cuwave tests/maxwell_test.py:839-846), `reconstruction_sensitivity` (the
+  field rebuilt by the reverse march) against it.
 """
 
 from __future__ import annotations
@@ -237,3 +238,30 @@ def test_the_measured_timestep_sits_at_the_yee_limit():
     dt = stable_timestep(sim, np.zeros(sim.Nx_padded), iterations=200, safety=1.0)
     limit = (1.0 / n) / np.sqrt(3.0)
     assert limit <= dt <= 1.1 * limit, (dt, limit)
+
+
+@pytest.mark.parametrize("Nx, order, precision, bound",
+                         [((16, 18), 4, "float64", 1e-9), ((10, 11, 12), 6, "float64", 1e-9),
+                          ((14, 15, 16), 2, "float32", 2e-3)])
+def test_the_reverse_march_reproduces_the_stored_gradient(Nx, order, precision, bound):
+    # no pml: the leapfrog runs backwards exactly but for rounding
+    # (cuwave tests/maxwell_test.py:844-846)
+    print(f"maxwell reconstruction {Nx} order {order} {precision}", flush=True)
+    from cuwave_amd import sensitivity as S
+
+    sim = vector_sim(Nx, order, "mixed", precision=precision, N=60)
+    rng = np.random.default_rng(12)
+    gamma = rng.random(sim.Nx_padded).astype(sim.np_dtype)
+    source = shots(sim, rng, 1)[0]
+    sensors = sensor_points(sim, rng, 4)
+
+    def objective(um):
+        return 0.5 * float(torch.sum(um * um)), um
+
+    c1, g1, u1, _ = MX.sensitivity(sim, source, gamma, sensors, objective)
+    c2, g2, u2, info = S.reconstruction_sensitivity(sim, source, gamma, sensors, objective)
+    assert c1 == c2 and torch.equal(u1, u2)
+    gap = float(torch.linalg.vector_norm(g2["mass"] - g1["mass"]) /
+                torch.linalg.vector_norm(g1["mass"]))
+    assert gap < bound, gap
+    assert info["strip"] == 0 and info["drift"] < (1e-10 if precision == "float64" else 1e-3)
