@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 12
+ABI_VERSION = 13
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -246,6 +246,8 @@ class MaxwellStepArgs(ctypes.Structure):
     _fields_ = [
         ("grid", MaxwellGrid),
         ("minv", VP),
+        ("nu_pair", VP),
+        ("g_nu", VP),
         ("curl", VP),
         ("u0", VP),
         ("u1", VP),

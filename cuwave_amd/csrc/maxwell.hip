@@ -86,8 +86,19 @@ int step_lanes(const cw_maxwell_step_args &a, hipStream_t stream) {
   const int64_t threads = (int64_t)gc.interior * (a.grid.lanes / V);
   const int blocks = (int)((threads + cwm::kThreads - 1) / cwm::kThreads);
   T *h = (T *)a.curl;
-  hipLaunchKernelGGL((cwm::mx_curl<T, NDIM, R, V>), dim3(blocks), dim3(cwm::kThreads), 0, stream,
-                     gc, (const T *)a.u1, h);
+  const T *nu_pair = (const T *)a.nu_pair;
+  if (a.g_nu)
+    hipLaunchKernelGGL((cwm::mx_curl<T, NDIM, R, V, true, true>), dim3(blocks),
+                       dim3(cwm::kThreads), 0, stream, gc, (const T *)a.u1, h, nu_pair,
+                       (T *)a.g_nu, (const T *)a.v1);
+  else if (nu_pair)
+    hipLaunchKernelGGL((cwm::mx_curl<T, NDIM, R, V, true, false>), dim3(blocks),
+                       dim3(cwm::kThreads), 0, stream, gc, (const T *)a.u1, h, nu_pair,
+                       (T *)nullptr, (const T *)nullptr);
+  else
+    hipLaunchKernelGGL((cwm::mx_curl<T, NDIM, R, V, false, false>), dim3(blocks),
+                       dim3(cwm::kThreads), 0, stream, gc, (const T *)a.u1, h, (const T *)nullptr,
+                       (T *)nullptr, (const T *)nullptr);
   if (a.g_mass)
     hipLaunchKernelGGL((cwm::mx_update<T, NDIM, R, V, true>), dim3(blocks), dim3(cwm::kThreads),
                        0, stream, gu, (const T *)a.minv, (const T *)h, (const T *)a.u0,
@@ -158,6 +169,8 @@ int cw_maxwell_step(const cw_maxwell_step_args *args, void *stream) {
     return fail("u2 and the curl scratch must not alias u1 or each other");
   if (args->g_mass && !args->v1)
     return fail("the adjoint step needs the forward field v1");
+  if (args->g_nu && (!args->g_mass || !args->nu_pair))
+    return fail("the permeability density needs the adjoint step and nu_pair");
   return dispatch_step(*args, (hipStream_t)stream);
 }
 

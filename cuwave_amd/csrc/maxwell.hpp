@@ -124,10 +124,13 @@ __device__ __forceinline__ Q curl_of(const MxGeom<T> &g, const T *__restrict__ u
 
 // curl_kernel (maxwell.cu:209-239): every pair point of the node off the
 // staggered ghosts of its two axes, nu times its cell weight (halved on the
-// walls of the axis it does not span) times the curl
-template <typename T, int NDIM, int R, int V>
+// walls of the axis it does not span) times the curl; MAG reads that product
+// per pair point from nu_pair (a designed permeability), and GRAD then takes
+// the adjoint curl against the forward one off g_nu, summed over the lanes
+template <typename T, int NDIM, int R, int V, bool MAG, bool GRAD>
 __global__ void __launch_bounds__(kThreads)
-mx_curl(MxGeom<T> g, const T *__restrict__ u1, T *__restrict__ h) {
+mx_curl(MxGeom<T> g, const T *__restrict__ u1, T *__restrict__ h,
+        const T *__restrict__ nu_pair, T *__restrict__ g_nu, const T *__restrict__ v1) {
   using Q = LanesOf<T, V>;
   int A[3];
   int64_t idx;
@@ -147,7 +150,16 @@ mx_curl(MxGeom<T> g, const T *__restrict__ u1, T *__restrict__ h) {
         if (d != k && d != l && (A[d] == 1 || A[d] == g.n[d] - 2))
           w *= (T)0.5;
       const Q b = curl_of<NDIM, R, Q>(g, u1, e, k, l, A);
-      lanes_put(h, pair_row<NDIM>(k, l) * g.nodes * g.lanes + e, (g.nu * w) * b);
+      const int64_t p = pair_row<NDIM>(k, l) * g.nodes + idx;
+      if constexpr (GRAD) {
+        const T s = cw::shot_sum(b * curl_of<NDIM, R, Q>(g, v1, e, k, l, A), g.lanes);
+        if (lane0 == 0)
+          g_nu[p] -= s;
+      }
+      if constexpr (MAG)
+        lanes_put(h, p * g.lanes + lane0, nu_pair[p] * b);
+      else
+        lanes_put(h, p * g.lanes + lane0, (g.nu * w) * b);
     }
 }
 

@@ -30,13 +30,16 @@ the shots innermost in every per-shot array as in the other families:
 
 The host-side parametrization (the point inertia, the excitation weights,
 the chain rule) runs in numpy in the simulation's precision, as the reference
-computes it. Not built here: a pml, damping, a designed permeability
-(`MagnetodielectricWave`), the fused and blocked launch tiers.
+computes it. A subclass with `magnetic = True` designs the permeability too
+(the reference's `USE_MAGNETIC` build): the curl then reads nu W per pair
+point and the adjoint also accumulates the permeability density. Not built
+here: a pml, damping, the fused and blocked launch tiers.
 """
 
 from __future__ import annotations
 
 import ctypes
+import itertools
 import math
 from dataclasses import dataclass
 
@@ -126,10 +129,11 @@ class MaxwellWave(Simulation):
     field on its wall node at zero through a zeroed inertia.
     """
 
-    permeability: float = 1.0  # uniform mu: the permeability is not designed here
+    permeability: float = 1.0  # background mu, designed only where `magnetic` is set
 
     default_boundary = Conductor
     face_conditions = (Conductor, Magnetic)
+    magnetic = False  # whether the permeability carries the design too
 
     def __post_init__(self) -> None:
         super().__post_init__()
@@ -176,8 +180,13 @@ class MaxwellWave(Simulation):
         return lin.astype(np.int32)
 
     def parametrization(self, indicator):
-        """The nodal permittivity of the design (a subclass defines it)."""
+        """(nodal permittivity, nodal 1 / mu or None) of the design (a subclass defines it)."""
         raise NotImplementedError(f"{type(self).__name__} defines no parametrization")
+
+    @property
+    def pair_axes(self) -> list[tuple[int, int]]:
+        """The axes of each curl row: (1, 2), (0, 2), (0, 1) in 3D, (0, 1) in 2D."""
+        return [(0, 1)] if self.ndim == 2 else [(1, 2), (0, 2), (0, 1)]
 
     # ----------------------------------------------------------- materials
     def _wall_weights(self, axes) -> np.ndarray:
@@ -205,6 +214,18 @@ class MaxwellWave(Simulation):
         mean = self._point_average(1.0 / np.maximum(field, tiny), c)
         return np.where(mean > 0.0, 1.0 / np.maximum(mean, tiny), 0.0).astype(self.np_dtype)
 
+    def _pair_average(self, field: np.ndarray, axes) -> np.ndarray:
+        """The harmonic mean of the four nodes a pair point straddles (cuwave/wave.py:436-457)."""
+        out = np.zeros(self.Nx_padded, dtype=self.np_dtype)
+        safe = np.maximum(field, np.finfo(self.np_dtype).tiny)
+        inner = tuple(slice(0, n - 1) if d in axes else slice(0, n) for d, n in enumerate(self.Nx))
+        for bits in itertools.product((0, 1), repeat=2):
+            shifted = tuple(slice(bits[axes.index(d)], n - 1 + bits[axes.index(d)]) if d in axes
+                            else slice(0, n) for d, n in enumerate(self.Nx))
+            out[inner] += 1.0 / safe[shifted]
+        out[inner] = self.np_dtype(4.0) / out[inner]
+        return out
+
     def point_inverse_inertia(self, eps: np.ndarray) -> np.ndarray:
         """(ncomp, *Nx_padded): 1 / (W eps) on the component points, zero on a
         conductor's wall node for the tangential components (cuwave/maxwell.py:247-263)."""
@@ -230,8 +251,16 @@ class MaxwellWave(Simulation):
         gamma = np.ascontiguousarray(as_numpy(indicator), dtype=self.np_dtype)
         if gamma.shape != self.indicator_shape():
             raise ValueError(f"indicator of shape {gamma.shape}, expected {self.indicator_shape()}")
-        eps = np.ascontiguousarray(self.parametrization(gamma), dtype=self.np_dtype)
-        return {"eps": eps, "minv": self.point_inverse_inertia(eps)}
+        eps, nu = self.parametrization(gamma)
+        eps = np.ascontiguousarray(eps, dtype=self.np_dtype)
+        mat = {"eps": eps, "minv": self.point_inverse_inertia(eps)}
+        if self.magnetic:
+            # nu W of every pair point, the weight folded in (cuwave/maxwell.py:264-270)
+            nu = np.ascontiguousarray(nu, dtype=self.np_dtype)
+            mat["nu"] = nu
+            mat["nu_pair"] = np.stack([self._wall_weights(axes) * self._pair_average(nu, axes)
+                                       for axes in self.pair_axes]).astype(self.np_dtype)
+        return mat
 
     def excitation_weights(self, mat: dict, lin) -> np.ndarray:
         """Source weights dt^2 / (eps V) (cuwave/maxwell.py:561-574)."""
@@ -292,9 +321,10 @@ class DielectricWave(MaxwellWave):
         return 1.0 / math.sqrt(min(self.permittivity1, self.permittivity2) * self.permeability)
 
     def parametrization(self, indicator):
-        """eps1 + gamma (eps2 - eps1) (cuwave/maxwell.py:65-67)."""
-        return (self.permittivity1
-                + indicator * (self.permittivity2 - self.permittivity1)).astype(self.np_dtype)
+        """eps1 + gamma (eps2 - eps1) (cuwave/maxwell.py:65-67, 700-704); the permeability
+        left uniform."""
+        eps = self.permittivity1 + indicator * (self.permittivity2 - self.permittivity1)
+        return eps.astype(self.np_dtype), None
 
     def parametrization_jacobian(self, indicator=None):
         """The permittivity is affine in gamma; the permeability does not depend on it."""
@@ -319,16 +349,20 @@ class MaxwellEngine(ElasticEngine):
         self.minv = torch.as_tensor(mat["minv"], device=self.device)
         self.curl = torch.zeros((sim.npairs, *sim.Nx_padded, self.lanes), dtype=sim.dtype,
                                 device=self.device)
+        self.nu_pair = (torch.as_tensor(mat["nu_pair"], device=self.device) if sim.magnetic
+                        else None)
         self.args = _abi.MaxwellStepArgs()
         self.args.grid = self.grid
         self.args.minv, self.args.curl = _abi.ptr(self.minv), _abi.ptr(self.curl)
+        self.args.nu_pair = _abi.ptr(self.nu_pair)
 
-    def step(self, u0, u1, u2, g_mass=None, v1=None, mf=0.0):
+    def step(self, u0, u1, u2, g_mass=None, v1=None, mf=0.0, g_nu=None):
         """u2 of the pair (u2 may be u0); with g_mass the adjoint step's inertia density of
-        u1 against the forward v1 (cw_maxwell_step)."""
+        u1 against the forward v1, with g_nu too the permeability density (cw_maxwell_step)."""
         a = self.args
         a.u0, a.u1, a.u2 = _abi.ptr(u0), _abi.ptr(u1), _abi.ptr(u2)
         a.g_mass, a.v1, a.mass_factor = _abi.ptr(g_mass), _abi.ptr(v1), float(mf)
+        a.g_nu = _abi.ptr(g_nu)
         _abi.call("cw_maxwell_step", ctypes.byref(a), _abi.stream())
         return u2
 
@@ -338,8 +372,16 @@ class MaxwellEngine(ElasticEngine):
 
     def gradient_fields(self) -> dict:
         sim = self.sim
-        return {"mass": torch.zeros((sim.ncomp, *sim.Nx_padded), dtype=sim.dtype,
-                                    device=self.device)}
+        g = {"mass": torch.zeros((sim.ncomp, *sim.Nx_padded), dtype=sim.dtype,
+                                 device=self.device)}
+        if sim.magnetic:
+            g["nu"] = torch.zeros((sim.npairs, *sim.Nx_padded), dtype=sim.dtype,
+                                  device=self.device)
+        return g
+
+    def adjoint_step(self, l0, l1, grads, v1, mf):
+        """The adjoint step over l0 with every density this family accumulates."""
+        return self.step(l0, l1, l0, grads["mass"], v1, mf, grads.get("nu"))
 
 
 def _point_average_adjoint(density: np.ndarray, c: int) -> np.ndarray:
@@ -353,10 +395,29 @@ def _point_average_adjoint(density: np.ndarray, c: int) -> np.ndarray:
     return out
 
 
-def finalize_gradients(sim: MaxwellWave, g_mass: np.ndarray, eps: np.ndarray) -> dict:
+def _pair_average_adjoint(sim: MaxwellWave, density, field, axes) -> np.ndarray:
+    """Transpose of the pair mean: d(harmonic mean)/d(node) = (mean / node)^2 / 4
+    (cuwave/wave.py:460-475)."""
+    safe = np.maximum(field, np.finfo(sim.np_dtype).tiny)
+    mean = sim._pair_average(field, axes)
+    scattered = density * mean * mean * 0.25
+    out = np.zeros(sim.Nx_padded, dtype=sim.np_dtype)
+    for bits in itertools.product((0, 1), repeat=2):
+        to = [slice(None)] * sim.ndim
+        fro = [slice(None)] * sim.ndim
+        for d, b in zip(axes, bits):
+            to[d], fro[d] = slice(b, None), slice(0, -b if b else None)
+        out[tuple(to)] += scattered[tuple(fro)] / safe[tuple(to)] ** 2
+    return out
+
+
+def finalize_gradients(sim: MaxwellWave, grads: dict, mat: dict) -> dict:
     """The point densities chained through the harmonic means onto the nodal
-    permittivity (cuwave/maxwell.py:601-613, cuwave/wave.py:427-433); zero stiffness,
-    the permeability not being designed."""
+    permittivity, and with a designed permeability the pair densities through the
+    pair means onto the nodal 1 / mu (cuwave/maxwell.py:601-613, cuwave/wave.py:
+    427-475); zero stiffness otherwise."""
+    eps = mat["eps"]
+    g_mass = grads["mass"].cpu().numpy()
     tiny = np.finfo(sim.np_dtype).tiny
     safe = np.maximum(eps, tiny)
     out = np.zeros(sim.Nx_padded, dtype=sim.np_dtype)
@@ -364,7 +425,12 @@ def finalize_gradients(sim: MaxwellWave, g_mass: np.ndarray, eps: np.ndarray) ->
         density = g_mass[c] * sim._wall_weights((c,))
         mean = sim._point_harmonic(eps, c)
         out += _point_average_adjoint(density * mean * mean, c) / (safe * safe)
-    return {"mass": out, "stiff": np.zeros(sim.Nx_padded, dtype=sim.np_dtype)}
+    stiff = np.zeros(sim.Nx_padded, dtype=sim.np_dtype)
+    if sim.magnetic:
+        g_nu = grads["nu"].cpu().numpy()
+        for p, axes in enumerate(sim.pair_axes):
+            stiff += _pair_average_adjoint(sim, g_nu[p] * sim._wall_weights(axes), mat["nu"], axes)
+    return {"mass": out, "stiff": stiff}
 
 
 def simulate_shots(sim: MaxwellWave, sources, indicator, sensors=None):
@@ -425,9 +491,9 @@ def sensitivity(sim: MaxwellWave, source: Source, indicator, sensors, objective)
     for m in range(sim.N):
         v1 = V[sim.N + 1 - m]
         l0, l1 = P[m % 2], P[1 - m % 2]
-        eng.step(l0, l1, l0, grads["mass"], v1, mf)
+        eng.adjoint_step(l0, l1, grads, v1, mf)
         adj.adjoint_inject(l0, adjoint, m, grads["mass"], v1, mf)
-    out = finalize_gradients(sim, grads["mass"].cpu().numpy(), mat["eps"])
+    out = finalize_gradients(sim, grads, mat)
     return cost, {k: torch.as_tensor(v, device=eng.device) for k, v in out.items()}, um, {}
 
 
@@ -467,12 +533,12 @@ def reconstruction_sensitivity(sim: MaxwellWave, source: Source, indicator, sens
     for m in range(N):
         n = N - 1 - m
         l0, l1 = P[m % 2], P[1 - m % 2]
-        eng.step(l0, l1, l0, grads["mass"], cur, mf)
+        eng.adjoint_step(l0, l1, grads, cur, mf)
         adj.adjoint_inject(l0, adjoint, m, grads["mass"], cur, mf)
         # u^(n-2) over u^n's slot, then the pair moves down a level
         src.inject(eng.step(cur, prev, cur), signal, n)
         cur, prev = prev, cur
     drift = float(torch.max(torch.abs(U))) / scale if scale > 0 else 0.0
-    out = finalize_gradients(sim, grads["mass"].cpu().numpy(), mat["eps"])
+    out = finalize_gradients(sim, grads, mat)
     return (cost, {k: torch.as_tensor(v, device=eng.device) for k, v in out.items()}, um,
             {"strip": 0, "drift": drift})

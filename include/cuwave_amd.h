@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 12
+#define CW_ABI_VERSION 13
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -430,6 +430,13 @@ typedef struct cw_maxwell_grid {
 typedef struct cw_maxwell_step_args {
   cw_maxwell_grid grid;
   const void *minv; /* (ncomp, *np) point inverse inertia */
+  /* (npairs, *np) nu W of each pair point for a designed permeability (the
+   * USE_MAGNETIC build, cuwave/maxwell.py:264-270), or NULL for grid.nu and
+   * the wall weights */
+  const void *nu_pair;
+  /* (npairs, *np) with g_mass and nu_pair set: the permeability density, the
+   * adjoint curl against the forward one (maxwell.cu:229-231), or NULL */
+  void *g_nu;
   void *curl;
   const void *u0, *u1;
   void *u2;

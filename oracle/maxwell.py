@@ -9,7 +9,9 @@ arrays at a time, one shot, on the logical grid (no padding):
   242-274), `step(..., v1=, g_mass=)` its GRADIENT build;
 - `point_inverse_inertia` is MaxwellWave.build_materials with the
   permittivity's harmonic point means and the wall weights
-  (cuwave/maxwell.py:247-272, cuwave/wave.py:386-425);
+  (cuwave/maxwell.py:247-272, cuwave/wave.py:386-425), and
+  `pair_inverse_permeability` its pair-point nu W where the permeability is
+  designed (the USE_MAGNETIC build);
 - `simulate` is the forward march of cuwave/wave.py:1071-1163 with the
   point sources and probes of cuwave/kernels/common.cuh:456-490.
 
@@ -73,22 +75,31 @@ def interior(shape) -> np.ndarray:
     return m
 
 
-def curl(u: np.ndarray, w: np.ndarray, inv_dx, nu, conductor: int) -> np.ndarray:
-    """nu W b of every pair point (maxwell.cu:209-239): u (ndim, *Nx)."""
+def pair_curl(u: np.ndarray, k: int, l: int, w, inv_dx, conductor: int) -> np.ndarray:
+    """The (k, l) curl b at its pair points (maxwell.cu:49-73)."""
+    dt = u.dtype.type
+    b = np.zeros(u.shape[1:], dtype=u.dtype)
+    ol, ok = conductor >> (2 * l), conductor >> (2 * k)
+    for j in range(1, len(w) + 1):
+        dk = shifted(u[k], l, j, ol, False) - shifted(u[k], l, 1 - j, ol, False)
+        dl = shifted(u[l], k, j, ok, False) - shifted(u[l], k, 1 - j, ok, False)
+        b = b - (dt(inv_dx[l]) * dt(w[j - 1])) * dk
+        b = b + (dt(inv_dx[k]) * dt(w[j - 1])) * dl
+    return b
+
+
+def curl(u: np.ndarray, w: np.ndarray, inv_dx, nu, conductor: int, nu_pair=None, v1=None,
+         g_nu=None) -> np.ndarray:
+    """nu W b of every pair point (maxwell.cu:209-239): u (ndim, *Nx). With `nu_pair`
+    (npairs, *Nx) the product nu W is read per pair point, and with `g_nu` the adjoint
+    curl b against the forward one of v1 is taken off it in place."""
     ndim = u.shape[0]
     shape = u.shape[1:]
     dt = u.dtype.type
-    R = len(w)
     h = np.zeros((1 if ndim == 2 else 3, *shape), dtype=u.dtype)
     for k in range(ndim - 1):
         for l in range(k + 1, ndim):
-            b = np.zeros(shape, dtype=u.dtype)
-            ol, ok = conductor >> (2 * l), conductor >> (2 * k)
-            for j in range(1, R + 1):
-                dk = shifted(u[k], l, j, ol, False) - shifted(u[k], l, 1 - j, ol, False)
-                dl = shifted(u[l], k, j, ok, False) - shifted(u[l], k, 1 - j, ok, False)
-                b = b - (dt(inv_dx[l]) * dt(w[j - 1])) * dk
-                b = b + (dt(inv_dx[k]) * dt(w[j - 1])) * dl
+            b = pair_curl(u, k, l, w, inv_dx, conductor)
             weight = np.ones(shape, dtype=u.dtype)
             for d in range(ndim):
                 if d not in (k, l):
@@ -96,7 +107,12 @@ def curl(u: np.ndarray, w: np.ndarray, inv_dx, nu, conductor: int) -> np.ndarray
                     weight = np.where((a == 1) | (a == shape[d] - 2), weight * dt(0.5), weight)
             valid = interior(shape) & (axis_index(shape, k) <= shape[k] - 3) & (
                 axis_index(shape, l) <= shape[l] - 3)
-            h[pair_row(ndim, k, l)] = np.where(valid, (dt(nu) * weight) * b, 0)
+            p = pair_row(ndim, k, l)
+            if g_nu is not None:
+                bv = pair_curl(v1, k, l, w, inv_dx, conductor)
+                g_nu[p] = np.where(valid, g_nu[p] - b * bv, g_nu[p])
+            scaled = nu_pair[p] * b if nu_pair is not None else (dt(nu) * weight) * b
+            h[p] = np.where(valid, scaled, 0)
     return h
 
 
@@ -121,13 +137,14 @@ def force(h: np.ndarray, c: int, w: np.ndarray, f, conductor: int, ndim: int) ->
     return out
 
 
-def step(u0, u1, minv, w, inv_dx, step_f, nu, conductor, v1=None, g_mass=None, mf=0.0):
-    """u2 of the pair; with v1 and g_mass the inertia density taken off g_mass in place
-    (maxwell.cu:242-274)."""
+def step(u0, u1, minv, w, inv_dx, step_f, nu, conductor, v1=None, g_mass=None, mf=0.0,
+         nu_pair=None, g_nu=None):
+    """u2 of the pair; with v1 and g_mass the inertia density taken off g_mass in place,
+    with g_nu too the permeability density (maxwell.cu:209-274)."""
     ndim = u1.shape[0]
     shape = u1.shape[1:]
     dt = u1.dtype.type
-    h = curl(u1, w, inv_dx, nu, conductor)
+    h = curl(u1, w, inv_dx, nu, conductor, nu_pair, v1, g_nu)
     u2 = np.zeros_like(u1)
     for c in range(ndim):
         valid = interior(shape) & (axis_index(shape, c) <= shape[c] - 3)
@@ -188,7 +205,32 @@ def point_inverse_inertia(eps: np.ndarray, conductor: int) -> np.ndarray:
     return minv
 
 
-def simulate(minv, w, inv_dx, step_f, nu, conductor, nsteps, src, weight, signal, probes):
+def pair_average(field: np.ndarray, axes) -> np.ndarray:
+    """The harmonic mean of the four nodes a pair point straddles (cuwave/wave.py:436-457)."""
+    dt = field.dtype
+    out = np.zeros_like(field)
+    safe = np.maximum(field, np.finfo(dt).tiny)
+    inner = tuple(slice(0, n - 1) if d in axes else slice(0, n) for d, n in enumerate(field.shape))
+    for b0 in (0, 1):
+        for b1 in (0, 1):
+            bits = dict(zip(axes, (b0, b1)))
+            shifted_ = tuple(slice(bits[d], n - 1 + bits[d]) if d in axes else slice(0, n)
+                             for d, n in enumerate(field.shape))
+            out[inner] += 1.0 / safe[shifted_]
+    out[inner] = dt.type(4.0) / out[inner]
+    return out
+
+
+def pair_inverse_permeability(nu: np.ndarray) -> np.ndarray:
+    """nu W of every pair point: the wall weights of the axes the pair does not span
+    times the pair mean of nu (cuwave/maxwell.py:264-270)."""
+    axes = [(0, 1)] if nu.ndim == 2 else [(1, 2), (0, 2), (0, 1)]
+    return np.stack([wall_weights(nu.shape, a).astype(nu.dtype) * pair_average(nu, a)
+                     for a in axes])
+
+
+def simulate(minv, w, inv_dx, step_f, nu, conductor, nsteps, src, weight, signal, probes,
+             nu_pair=None):
     """The forward march of one shot: u^t = step(u^(t-2), u^(t-1)), then the sources
     (field += weight signal, entry by entry), then the probes. `src`, `probes` are
     (component, *node) index tuples of arrays; returns (the last level, (N, count))."""
@@ -198,7 +240,7 @@ def simulate(minv, w, inv_dx, step_f, nu, conductor, nsteps, src, weight, signal
     u1 = np.zeros(shape, dtype=dt)
     traces = np.zeros((nsteps, len(probes[0])), dtype=dt)
     for t in range(nsteps):
-        u2 = step(u0, u1, minv, w, inv_dx, step_f, nu, conductor)
+        u2 = step(u0, u1, minv, w, inv_dx, step_f, nu, conductor, nu_pair=nu_pair)
         for e in range(len(src[0])):
             at = tuple(int(s[e]) for s in src)
             u2[at] = u2[at] + weight[e] * signal[t, e]

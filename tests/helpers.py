@@ -85,3 +85,28 @@ def rel_l2(a, b) -> float:
 
 def host(a) -> np.ndarray:
     return a.detach().cpu().numpy() if hasattr(a, "detach") else np.asarray(a)
+
+
+def magnetodielectric_class():
+    """A DielectricWave that designs the permeability too, to reach the designed-permeability
+    path (cuwave tests/maxwell_test.py:102-119): both materials interpolate."""
+    from dataclasses import dataclass
+
+    from cuwave_amd.maxwell import DielectricWave
+
+    @dataclass
+    class MagnetodielectricWave(DielectricWave):
+        magnetic = True
+        permeability2: float = 2.0
+
+        def parametrization(self, indicator):
+            eps = self.permittivity1 + indicator * (self.permittivity2 - self.permittivity1)
+            mu = self.permeability + indicator * (self.permeability2 - self.permeability)
+            return eps.astype(self.np_dtype), (1.0 / mu).astype(self.np_dtype)
+
+        def parametrization_jacobian(self, indicator):
+            contrast = self.permeability2 - self.permeability
+            mu = self.permeability + indicator * contrast
+            return self.permittivity2 - self.permittivity1, -contrast / mu**2
+
+    return MagnetodielectricWave

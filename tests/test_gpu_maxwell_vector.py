@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import ROOT
+from helpers import ROOT, magnetodielectric_class
 
 from cuwave_amd import _abi
 from cuwave_amd import maxwell as MX
@@ -39,13 +39,14 @@ def library():
     _abi.load()
 
 
-def vector_sim(Nx, order, walls="conductor", precision="float64", N=30):
+def vector_sim(Nx, order, walls="conductor", precision="float64", N=30, magnetic=False):
     boundary = WALLS[walls]
     if callable(boundary):
         boundary = boundary(len(Nx))
     dx = tuple(1.0 / (n - 3) for n in Nx)
-    return MX.DielectricWave(Nx, dx, N, 0.25 * min(dx), precision=precision, space_order=order,
-                             permittivity1=1.0, permittivity2=4.0, boundary=boundary)
+    cls = magnetodielectric_class() if magnetic else MX.DielectricWave
+    return cls(Nx, dx, N, 0.25 * min(dx), precision=precision, space_order=order,
+               permittivity1=1.0, permittivity2=4.0, boundary=boundary)
 
 
 def logical(sim, a):
@@ -55,7 +56,9 @@ def logical(sim, a):
 def oracle_march(sim, gamma, source, sensors):
     """The oracle's one-shot march of `source` with the product's weights' arithmetic."""
     dt = sim.np_dtype
-    eps = logical(sim, sim.parametrization(gamma))
+    eps, nu = sim.parametrization(gamma)
+    eps = logical(sim, eps)
+    nu_pair = M.pair_inverse_permeability(logical(sim, nu)) if sim.magnetic else None
     grid = sim.abi_grid(1)
     minv = M.point_inverse_inertia(eps, grid.conductor)
     pos = np.asarray(source.position)
@@ -64,7 +67,8 @@ def oracle_march(sim, gamma, source, sensors):
     sens = np.asarray(sensors)
     return M.simulate(minv, staggered_weights(sim.radius, sim.stencil).astype(dt),
                       [dt(1.0 / d) for d in sim.dx], sim.step_factors(), dt(1.0), grid.conductor,
-                      sim.N, tuple(pos), weight, np.asarray(source.signal, dtype=dt), tuple(sens))
+                      sim.N, tuple(pos), weight, np.asarray(source.signal, dtype=dt), tuple(sens),
+                      nu_pair=nu_pair)
 
 
 def shots(sim, rng, count):
@@ -265,3 +269,95 @@ def test_the_reverse_march_reproduces_the_stored_gradient(Nx, order, precision, 
                 torch.linalg.vector_norm(g1["mass"]))
     assert gap < bound, gap
     assert info["strip"] == 0 and info["drift"] < (1e-10 if precision == "float64" else 1e-3)
+
+
+# ------------------------------------------------------ designed permeability
+@pytest.mark.parametrize("Nx, order, walls", [((22, 26), 4, "mixed"), ((14, 16, 18), 2, "magnetic")])
+def test_a_designed_permeability_march_is_the_oracles_bit_for_bit(Nx, order, walls):
+    print(f"maxwell magnetic forward {Nx} order {order} {walls}", flush=True)
+    sim = vector_sim(Nx, order, walls, magnetic=True)
+    rng = np.random.default_rng(21)
+    gamma = rng.random(sim.Nx_padded)
+    sources = shots(sim, rng, 2)
+    sensors = sensor_points(sim, rng)
+    field, record = MX.simulate_shots(sim, sources, gamma, sensors)
+    for b, source in enumerate(sources):
+        last, traces = oracle_march(sim, gamma, source, sensors)
+        assert np.array_equal(field[b].cpu().numpy(), last), f"shot {b}"
+        assert np.array_equal(record[b].cpu().numpy(), traces), f"shot {b}"
+
+
+@pytest.mark.parametrize("Nx, order", [((20, 23), 4), ((11, 12, 14), 2)])
+def test_the_permeability_density_is_the_oracles(Nx, order):
+    print(f"maxwell magnetic adjoint step {Nx} order {order}", flush=True)
+    sim = vector_sim(Nx, order, "mixed", magnetic=True)
+    rng = np.random.default_rng(22)
+    gamma = rng.random(sim.Nx_padded)
+    mat = sim.build_materials(gamma)
+    eng = MX.MaxwellEngine(sim, mat, 1)
+    free = np.zeros((sim.ncomp, *sim.Nx_padded, 1), dtype=bool)
+    free[(slice(None), *(slice(1, n - 1) for n in sim.Nx))] = True
+    l0, l1, v1 = (torch.as_tensor(np.where(free, rng.standard_normal(free.shape), 0.0),
+                                  device="cuda") for _ in range(3))
+    grads = eng.gradient_fields()
+    g0 = rng.standard_normal(grads["nu"].shape)
+    grads["nu"].copy_(torch.as_tensor(g0))
+    mf = 1.0 / sim.dt**2
+    out = eng.adjoint_step(l0.clone(), l1, grads, v1, mf)
+    d = sim.np_dtype
+    g_mass = np.zeros((sim.ncomp, *sim.Nx))
+    g_nu = logical(sim, g0).copy()
+    want = M.step(logical(sim, l0.cpu().numpy()[..., 0]), logical(sim, l1.cpu().numpy()[..., 0]),
+                  logical(sim, mat["minv"]), staggered_weights(sim.radius, sim.stencil).astype(d),
+                  [d(1.0 / x) for x in sim.dx], sim.step_factors(), 1.0,
+                  sim.abi_grid(1).conductor, v1=logical(sim, v1.cpu().numpy()[..., 0]),
+                  g_mass=g_mass, mf=mf, nu_pair=logical(sim, mat["nu_pair"]), g_nu=g_nu)
+    valid = np.zeros(want.shape, dtype=bool)
+    for c in range(sim.ncomp):
+        valid[(c, *(slice(1, n - 1 - (k == c)) for k, n in enumerate(sim.Nx)))] = True
+    assert np.array_equal(np.where(valid, logical(sim, out.cpu().numpy()[..., 0]), 0.0), want)
+    assert np.array_equal(logical(sim, grads["mass"].cpu().numpy()), g_mass)
+    assert np.array_equal(logical(sim, grads["nu"].cpu().numpy()), g_nu)
+
+
+@pytest.mark.parametrize("Nx, order", [((16, 18), 4), ((10, 11, 12), 2)])
+def test_a_designed_permeability_matches_finite_differences(Nx, order):
+    # cuwave tests/maxwell_test.py:847-851
+    print(f"maxwell magnetic FD {Nx} order {order}", flush=True)
+    from cuwave_amd import sensitivity as S
+
+    sim = vector_sim(Nx, order, "mixed", N=60, magnetic=True)
+    rng = np.random.default_rng(23)
+    gamma = 0.3 + 0.4 * rng.random(sim.Nx_padded)
+    t = np.arange(sim.N) * sim.dt
+    f0 = 4.0 / (sim.N * sim.dt)
+    arg = (np.pi * f0 * (t - 1.2 / f0)) ** 2
+    pulse = (1 - 2 * arg) * np.exp(-arg)
+    centre = [n // 2 for n in Nx]
+    source = Source(np.array([[1], *([c] for c in centre)]), pulse[:, None])
+    sensors = np.array([[0, 1, sim.ncomp - 1], *([c - 2, c + 2, c + 1] for c in centre)])
+    observed = MX.simulate(sim, source, 0.5 * np.ones(sim.Nx_padded), sensors)[1].cpu().numpy()
+
+    def objective(um):
+        r = um.cpu().numpy() - observed
+        return 0.5 * float(np.sum(r * r)), torch.as_tensor(r)
+
+    cost, grads, _, _ = MX.sensitivity(sim, source, gamma, sensors, objective)
+    d_mass, d_stiff = sim.parametrization_jacobian(gamma)
+    g = d_mass * grads["mass"].cpu().numpy() + d_stiff * grads["stiff"].cpu().numpy()
+    assert np.abs(grads["stiff"].cpu().numpy()).max() > 0
+    h = 1e-5
+    for node in [tuple(c + 1 for c in centre), tuple(c - 1 for c in centre),
+                 tuple(min(c + 3, n - 2) for c, n in zip(centre, Nx))]:
+        costs = []
+        for sgn in (1, -1):
+            probe = gamma.copy()
+            probe[node] += sgn * h
+            costs.append(objective(MX.simulate(sim, source, probe, sensors)[1])[0])
+        fd = (costs[0] - costs[1]) / (2 * h)
+        assert abs(g[node] - fd) <= 1e-5 * abs(fd) + 1e-10 * abs(cost), (node, g[node], fd)
+    _, g2, _, info = S.reconstruction_sensitivity(sim, source, gamma, sensors, objective)
+    for k in ("mass", "stiff"):
+        gap = float(torch.linalg.vector_norm(g2[k] - grads[k]) /
+                    torch.linalg.vector_norm(grads[k]))
+        assert gap < 1e-9, (k, gap)

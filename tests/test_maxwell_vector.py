@@ -172,7 +172,8 @@ def test_a_discrete_gradient_is_a_null_mode(shape, order):
 def test_a_planted_curl_factor_breaks_the_null_mode(monkeypatch):
     # the curl's two axis factors swapped (the axes' spacings differ)
     real = M.curl
-    monkeypatch.setattr(M, "curl", lambda u, w, inv_dx, nu, c: real(u, w, inv_dx[::-1], nu, c))
+    monkeypatch.setattr(M, "curl",
+                        lambda u, w, inv_dx, nu, c, *rest: real(u, w, inv_dx[::-1], nu, c, *rest))
     shape, order = (30, 34), 4
     rng = np.random.default_rng(1)
     apply, _, dx = operator(shape, order, 0, rng)
@@ -244,7 +245,7 @@ def test_the_point_inertia_is_the_oracles(boundary):
     rng = np.random.default_rng(3)
     gamma = rng.random(sim.Nx_padded)
     mat = sim.build_materials(gamma)
-    eps = sim.parametrization(gamma)[tuple(slice(0, n) for n in sim.Nx)]
+    eps = sim.parametrization(gamma)[0][tuple(slice(0, n) for n in sim.Nx)]
     want = M.point_inverse_inertia(eps, sim.abi_grid(1).conductor)
     got = mat["minv"][(slice(None), *(slice(0, n) for n in sim.Nx))]
     assert np.array_equal(got, want)
@@ -284,3 +285,44 @@ def test_the_drivers_not_built_for_the_family_refuse_it():
     with pytest.raises(NotImplementedError, match="not built for DielectricWave"):
         U.misfit_gradient(sim, [source], np.zeros(sim.Nx_padded), sensors,
                           np.zeros((1, sim.N, 1)))
+
+
+def test_the_designed_permeability_is_the_oracles():
+    from helpers import magnetodielectric_class
+
+    cls = magnetodielectric_class()
+    for Nx in ((10, 12), (9, 11, 13)):
+        dx = tuple(1.0 / (n - 3) for n in Nx)
+        sim = cls(Nx, dx, 10, 0.1 * min(dx), precision="float64", permittivity1=1.0,
+                  permittivity2=4.0, boundary=Magnetic)
+        gamma = np.random.default_rng(5).random(sim.Nx_padded)
+        mat = sim.build_materials(gamma)
+        nu = sim.parametrization(gamma)[1][tuple(slice(0, n) for n in Nx)]
+        got = mat["nu_pair"][(slice(None), *(slice(0, n) for n in Nx))]
+        assert np.array_equal(got, M.pair_inverse_permeability(nu))
+
+
+def test_a_designed_permeability_keeps_the_operator_symmetric():
+    rng = np.random.default_rng(9)
+    shape = (12, 13, 14)
+    conductor = conductor_mask(shape, WALLS["mixed"](3))
+    eps = 1.0 + rng.random(shape)
+    nu_pair = M.pair_inverse_permeability(0.5 + rng.random(shape))
+    minv = M.point_inverse_inertia(eps, conductor)
+    mass = np.where(minv > 0, 1.0 / np.maximum(minv, 1e-300), 0.0)
+    w = staggered_weights(2, "optimized")
+    dx = tuple(1.0 / (n - 3) for n in shape)
+    free = np.zeros((3, *shape), dtype=bool)
+    for c in range(3):
+        free[(c, *(slice(1, n - 1 - (d == c)) for d, n in enumerate(shape)))] = True
+    free &= minv > 0
+
+    def apply(x):
+        out = M.step(np.zeros_like(x), x, minv, w, [1 / d for d in dx], [1e-4 / d for d in dx],
+                     1.0, conductor, nu_pair=nu_pair)
+        return np.where(free, (2.0 * x - out) * mass, 0.0)
+
+    x = np.where(free, rng.standard_normal(free.shape), 0.0)
+    y = np.where(free, rng.standard_normal(free.shape), 0.0)
+    ax, ay = apply(x), apply(y)
+    assert abs(np.sum(y * ax) - np.sum(x * ay)) / (np.linalg.norm(ax) * np.linalg.norm(y)) < 1e-12
