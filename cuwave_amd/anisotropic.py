@@ -37,7 +37,7 @@ import torch
 from . import _abi
 from .boundary import Clamped, Traction, faces_with
 from .elastic import PAIRS, ElasticEngine, _require, adjoint_signal
-from .wave import Simulation, Source, as_numpy, shot_lanes
+from .wave import MAX_BATCH, Simulation, Source, as_numpy, shot_lanes
 
 
 def corner_bits(ndim: int) -> list[tuple[int, ...]]:
@@ -396,22 +396,16 @@ def sensitivity(sim: AnisotropicElasticWave, source: Source, indicator, sensors,
     return cost, {k: torch.as_tensor(v, device=eng.device) for k, v in out.items()}, um, {}
 
 
-def reconstruction_sensitivity(sim: AnisotropicElasticWave, source: Source, indicator, sensors,
-                               objective, strip_dtype: str | None = None,
-                               checkpoints: int | None = None):
-    """Cost and gradients as `sensitivity`, the forward field rebuilt backwards instead of
-    stored (cuwave/sensitivity.py:500-750). With neither a pml nor damping every node is
-    reversible, so there is no strip: u^(t-2) = -u^t + 2 u^(t-1) + dt^2 minv force(u^(t-1))
-    + the source of step t, the step itself with u0 and u2 swapped, one level ahead of
-    the triplet the gradient pairs with lambda. `info` carries `strip` (0) and the
-    `drift` of the rebuilt initial state against the last level."""
-    if strip_dtype is not None or checkpoints is not None:
-        raise NotImplementedError("nothing is stored on this family's reverse march: no strip "
-                                  "dtype or checkpoints to choose")
-    _require(sim, [source], sensors)
-    mat = sim.build_materials(indicator)
-    eng = AnisoEngine(sim, mat, 1)
-    src, signal = eng.sources(mat, [source])
+def _reverse_sweep(sim: AnisotropicElasticWave, mat: dict, sources, sensors, objectives):
+    """Every shot of `sources` forward in one batch, then one adjoint sweep with the
+    forward field rebuilt by the reverse march: with neither a pml nor damping every
+    node is reversible, u^(t-2) = -u^t + 2 u^(t-1) + dt^2 minv force(u^(t-1)) + the
+    source of step t (the step with u0 and u2 swapped), one level ahead of the triplet
+    the gradient pairs with lambda, so no strip is kept. Every shot's densities are
+    summed over the lanes in the kernels. `objectives(records)` maps the (B, N, count)
+    records to (costs, (B, N, count) dJ/du). Returns (costs, grads, records, drift)."""
+    eng = AnisoEngine(sim, mat, len(sources))
+    src, signal = eng.sources(mat, sources)
     rec = eng.sensors(sensors)
     record = torch.zeros((sim.N, rec.count, eng.lanes), dtype=sim.dtype, device=eng.device)
     U = eng.levels(3)
@@ -420,10 +414,9 @@ def reconstruction_sensitivity(sim: AnisotropicElasticWave, source: Source, indi
         u = eng.step(U[t % 3], U[(t + 1) % 3], U[(t + 2) % 3])
         src.inject(u, signal, t)
         rec.record(u, record, t)
-    um = eng.shots(record)[0]
-    cost, dphi = objective(um)
-    dphi = torch.as_tensor(dphi, dtype=sim.dtype, device=eng.device)
-    adjoint = eng.lane_table(adjoint_signal(sim, dphi[None], sensors))
+    records = eng.shots(record)
+    costs, dphi = objectives(records)
+    adjoint = eng.lane_table(adjoint_signal(sim, dphi, sensors))
     lin = sim.flatten_indices(sensors)
     adj = eng.sensors(sensors, weight=sim.excitation_weights(mat, lin))
     grads = eng.gradient_fields()
@@ -441,6 +434,62 @@ def reconstruction_sensitivity(sim: AnisotropicElasticWave, source: Source, indi
             src.inject(eng.step(mid, lo, hi), signal, N - 2 - m)
             hi, mid, lo = mid, lo, hi
     drift = float(torch.max(torch.abs(torch.stack([lo, mid])))) / scale if scale > 0 else 0.0
+    return costs, grads, records, drift
+
+
+def reconstruction_sensitivity(sim: AnisotropicElasticWave, source: Source, indicator, sensors,
+                               objective, strip_dtype: str | None = None,
+                               checkpoints: int | None = None):
+    """Cost and gradients as `sensitivity`, the forward field rebuilt backwards instead of
+    stored (cuwave/sensitivity.py:500-750) by the reverse march, which needs no strip here.
+    `info` carries `strip` (0) and the `drift` of the rebuilt initial state."""
+    if strip_dtype is not None or checkpoints is not None:
+        raise NotImplementedError("nothing is stored on this family's reverse march: no strip "
+                                  "dtype or checkpoints to choose")
+    _require(sim, [source], sensors)
+    mat = sim.build_materials(indicator)
+
+    def objectives(records):
+        cost, dphi = objective(records[0])
+        return [cost], torch.as_tensor(dphi, dtype=sim.dtype, device=records.device)[None]
+
+    costs, grads, records, drift = _reverse_sweep(sim, mat, [source], sensors, objectives)
     out = finalize_gradients(sim, grads["mass"].cpu().numpy(), grads["cell"].cpu().numpy(), mat)
-    return (cost, {k: torch.as_tensor(v, device=eng.device) for k, v in out.items()}, um,
+    device = records.device
+    return (costs[0], {k: torch.as_tensor(v, device=device) for k, v in out.items()}, records[0],
             {"strip": 0, "drift": drift})
+
+
+def misfit_gradient(sim: AnisotropicElasticWave, sources, indicator, sensors, observed,
+                    objective, batch: int | None = None, checkpoints: int | None = None):
+    """The misfit summed over the shots and its derivative with respect to gamma
+    (cuwave/utils.py:410-459), `batch` shots a sweep (all by default, at most 64): one
+    forward sweep records every shot, one reverse sweep sums every shot's densities in
+    the kernels, one chain rule finishes the sum."""
+    if checkpoints is not None:
+        raise NotImplementedError("nothing is stored on this family's reverse march: no "
+                                  "checkpoints to choose")
+    sources = list(sources)
+    nodes = sensors.nodes
+    _require(sim, sources, nodes)
+    step = min(batch or MAX_BATCH, MAX_BATCH)
+    mat = sim.build_materials(indicator)
+    cost, total, device = 0.0, None, None
+    for start in range(0, len(sources), step):
+        chunk = sources[start:start + step]
+
+        def objectives(records, start=start, chunk=chunk):
+            costs, dphis = [], []
+            for b in range(len(chunk)):
+                c, dphi = sensors.objective(objective(observed[start + b]))(records[b])
+                costs.append(c)
+                dphis.append(torch.as_tensor(dphi, dtype=sim.dtype, device=records.device))
+            return costs, torch.stack(dphis)
+
+        costs, grads, records, _ = _reverse_sweep(sim, mat, chunk, nodes, objectives)
+        cost += sum(costs)
+        device = records.device
+        total = grads if total is None else {k: total[k] + grads[k] for k in total}
+    out = finalize_gradients(sim, total["mass"].cpu().numpy(), total["cell"].cpu().numpy(), mat)
+    jm, js = sim.parametrization_jacobian(indicator)
+    return cost, torch.as_tensor(jm * out["mass"] + js * out["stiff"], device=device)

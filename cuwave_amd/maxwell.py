@@ -51,7 +51,7 @@ from .boundary import Conductor, Magnetic, face_mask, faces_with
 from .elastic import ElasticEngine, _require, adjoint_signal
 from .scalar import PressureWave
 from .stencils import staggered_weights
-from .wave import Simulation, Source, as_numpy, shot_lanes
+from .wave import MAX_BATCH, Simulation, Source, as_numpy, shot_lanes
 
 
 @dataclass
@@ -497,20 +497,17 @@ def sensitivity(sim: MaxwellWave, source: Source, indicator, sensors, objective)
     return cost, {k: torch.as_tensor(v, device=eng.device) for k, v in out.items()}, um, {}
 
 
-def reconstruction_sensitivity(sim: MaxwellWave, source: Source, indicator, sensors, objective,
-                               strip_dtype: str | None = None, checkpoints: int | None = None):
-    """Cost and gradients as `sensitivity`, the forward field rebuilt backwards instead of
-    stored (cuwave/sensitivity.py:500-750). With neither a pml nor damping every point
-    is reversible, so there is no strip: u^(n-2) = -u^n + 2 u^(n-1) + minv force(u^(n-1))
-    + the source of step n, the step itself with u0 and u2 swapped. `info` carries
-    `strip` (0) and the `drift` of the rebuilt initial state against the last level."""
-    if strip_dtype is not None or checkpoints is not None:
-        raise NotImplementedError("nothing is stored on this family's reverse march: no strip "
-                                  "dtype or checkpoints to choose")
-    _require(sim, [source], sensors)
-    mat = sim.build_materials(indicator)
-    eng = MaxwellEngine(sim, mat, 1)
-    src, signal = eng.sources(mat, [source])
+def _reverse_sweep(sim: MaxwellWave, mat: dict, sources, sensors, objectives):
+    """Every shot of `sources` forward in one batch, then one adjoint sweep with the
+    forward field rebuilt level by level by the reverse march: with neither a pml nor
+    damping every point is reversible, u^(n-2) = -u^n + 2 u^(n-1) + minv force(u^(n-1))
+    + the source of step n (the step with u0 and u2 swapped), so no strip is kept.
+    Every shot's densities are summed over the lanes in the kernels.
+    `objectives(records)` maps the (B, N, count) records to (costs, (B, N, count) dJ/du).
+    Returns (costs, grads, records, drift), `drift` the rebuilt initial state against
+    the last level."""
+    eng = MaxwellEngine(sim, mat, len(sources))
+    src, signal = eng.sources(mat, sources)
     rec = eng.sensors(sensors)
     record = torch.zeros((sim.N, rec.count, eng.lanes), dtype=sim.dtype, device=eng.device)
     U = eng.levels(2)
@@ -519,10 +516,9 @@ def reconstruction_sensitivity(sim: MaxwellWave, source: Source, indicator, sens
         u = eng.step(U[t % 2], U[1 - t % 2], U[t % 2])
         src.inject(u, signal, t)
         rec.record(u, record, t)
-    um = eng.shots(record)[0]
-    cost, dphi = objective(um)
-    dphi = torch.as_tensor(dphi, dtype=sim.dtype, device=eng.device)
-    adjoint = eng.lane_table(adjoint_signal(sim, dphi[None], sensors))
+    records = eng.shots(record)
+    costs, dphi = objectives(records)
+    adjoint = eng.lane_table(adjoint_signal(sim, dphi, sensors))
     lin = sim.flatten_indices(sensors)
     adj = eng.sensors(sensors, weight=sim.excitation_weights(mat, lin))
     grads = eng.gradient_fields()
@@ -539,6 +535,61 @@ def reconstruction_sensitivity(sim: MaxwellWave, source: Source, indicator, sens
         src.inject(eng.step(cur, prev, cur), signal, n)
         cur, prev = prev, cur
     drift = float(torch.max(torch.abs(U))) / scale if scale > 0 else 0.0
+    return costs, grads, records, drift
+
+
+def reconstruction_sensitivity(sim: MaxwellWave, source: Source, indicator, sensors, objective,
+                               strip_dtype: str | None = None, checkpoints: int | None = None):
+    """Cost and gradients as `sensitivity`, the forward field rebuilt backwards instead of
+    stored (cuwave/sensitivity.py:500-750) by the reverse march, which needs no strip here.
+    `info` carries `strip` (0) and the `drift` of the rebuilt initial state."""
+    if strip_dtype is not None or checkpoints is not None:
+        raise NotImplementedError("nothing is stored on this family's reverse march: no strip "
+                                  "dtype or checkpoints to choose")
+    _require(sim, [source], sensors)
+    mat = sim.build_materials(indicator)
+
+    def objectives(records):
+        cost, dphi = objective(records[0])
+        return [cost], torch.as_tensor(dphi, dtype=sim.dtype, device=records.device)[None]
+
+    costs, grads, records, drift = _reverse_sweep(sim, mat, [source], sensors, objectives)
     out = finalize_gradients(sim, grads, mat)
-    return (cost, {k: torch.as_tensor(v, device=eng.device) for k, v in out.items()}, um,
+    device = records.device
+    return (costs[0], {k: torch.as_tensor(v, device=device) for k, v in out.items()}, records[0],
             {"strip": 0, "drift": drift})
+
+
+def misfit_gradient(sim: MaxwellWave, sources, indicator, sensors, observed, objective,
+                    batch: int | None = None, checkpoints: int | None = None):
+    """The misfit summed over the shots and its derivative with respect to the indicator
+    (cuwave/utils.py:410-459), `batch` shots a sweep (all by default, at most 64): one
+    forward sweep records every shot, one reverse sweep sums every shot's densities in
+    the kernels, one chain rule finishes the sum."""
+    if checkpoints is not None:
+        raise NotImplementedError("nothing is stored on this family's reverse march: no "
+                                  "checkpoints to choose")
+    sources = list(sources)
+    nodes = sensors.nodes
+    _require(sim, sources, nodes)
+    step = min(batch or MAX_BATCH, MAX_BATCH)
+    mat = sim.build_materials(indicator)
+    cost, total, device = 0.0, None, None
+    for start in range(0, len(sources), step):
+        chunk = sources[start:start + step]
+
+        def objectives(records, start=start, chunk=chunk):
+            costs, dphis = [], []
+            for b in range(len(chunk)):
+                c, dphi = sensors.objective(objective(observed[start + b]))(records[b])
+                costs.append(c)
+                dphis.append(torch.as_tensor(dphi, dtype=sim.dtype, device=records.device))
+            return costs, torch.stack(dphis)
+
+        costs, grads, records, _ = _reverse_sweep(sim, mat, chunk, nodes, objectives)
+        cost += sum(costs)
+        device = records.device
+        total = grads if total is None else {k: total[k] + grads[k] for k in total}
+    out = finalize_gradients(sim, total, mat)
+    jm, js = sim.parametrization_jacobian(as_numpy(indicator))
+    return cost, torch.as_tensor(jm * out["mass"] + js * out["stiff"], device=device)

@@ -217,9 +217,15 @@ def misfit_gradient(sim: Simulation, sources: Sequence[Source], indicator, senso
     sources = list(sources)
     from .wave import family_drivers
 
-    if family_drivers(sim) is not None:
-        raise NotImplementedError(f"misfit_gradient is not built for {type(sim).__name__}; "
-                                  f"use sensitivity")
+    own = family_drivers(sim)
+    if own is not None:
+        if adjoint not in (sensitivity, reconstruction_sensitivity):
+            raise NotImplementedError(f"{type(sim).__name__} sums its shots through the reverse "
+                                      f"march only")
+        if illum is not None or below is not None:
+            raise NotImplementedError("illum= and below= are built for the scalar equations")
+        return own.misfit_gradient(sim, sources, indicator, sensors, observed, objective,
+                                   batch, checkpoints)
     if type(sim).__name__ == "ElasticWave" and adjoint in (sensitivity, reconstruction_sensitivity):
         from . import elastic
 

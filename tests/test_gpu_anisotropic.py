@@ -25,6 +25,7 @@ from oracle import anisotropic as O
 
 pytestmark = pytest.mark.gpu
 
+FAMILY = "anisotropic"
 FACES = {"traction": None, "clamped": Clamped,
          "mixed": lambda n: ((Traction, Clamped),) + ((Clamped, Traction),) * (n - 1)}
 ORTHO = np.array([[30.0, 6.0, 0.0], [6.0, 12.0, 0.0], [0.0, 0.0, 4.0]])
@@ -44,6 +45,11 @@ def aniso_sim(Nx, faces="traction", precision="float64", C=None, plane="strain",
     return AN.AnisotropicElasticWave(Nx, dx, N, 1e-3, precision=precision, density=2.0,
                                      wavespeed_p=3.0, wavespeed_s=1.5, plane=plane, C=C,
                                      boundary=boundary)
+
+
+def misfit_case():
+    sim = aniso_sim((18, 20), "mixed", N=40, C=ORTHO)
+    return sim, 0.5 + np.random.default_rng(30).random(sim.Nx_padded)
 
 
 def logical(sim, a):
@@ -214,3 +220,31 @@ def test_the_reverse_march_reproduces_the_stored_gradient(Nx, faces, precision, 
         gap = float(torch.linalg.vector_norm(g2[k] - g1[k]) / torch.linalg.vector_norm(g1[k]))
         assert gap < bound, (k, gap)
     assert info["strip"] == 0 and info["drift"] < (1e-10 if precision == "float64" else 1e-3)
+
+
+@pytest.mark.parametrize("batch", [None, 2])
+def test_the_misfit_gradient_sums_the_shots(batch):
+    # one sweep of three shots against the three single-shot gradients, chained
+    from helpers import Receivers
+
+    from cuwave_amd import sensitivity as S
+    from cuwave_amd import utils as U
+
+    print(f"{FAMILY} misfit gradient batch={batch}", flush=True)
+    sim, gamma = misfit_case()
+    rng = np.random.default_rng(31)
+    sources = shots(sim, rng, 3)
+    sensors = sensor_points(sim, rng, 4)
+    observed = [0.5 * rng.standard_normal((sim.N, sensors.shape[1])) for _ in sources]
+    cost, gradient = U.misfit_gradient(sim, sources, gamma, Receivers(sensors), observed,
+                                       batch=batch)
+    want_cost, want = 0.0, 0.0
+    jm, js = sim.parametrization_jacobian(gamma)
+    for source, obs in zip(sources, observed):
+        c, g, _, _ = S.reconstruction_sensitivity(sim, source, gamma, sensors,
+                                                  S.l2_misfit(obs))
+        want_cost += c
+        want = want + jm * g["mass"].cpu().numpy() + js * g["stiff"].cpu().numpy()
+    assert abs(cost - want_cost) <= 1e-12 * want_cost
+    got = gradient.cpu().numpy()
+    assert np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want)

@@ -29,6 +29,7 @@ from oracle import maxwell as M
 
 pytestmark = pytest.mark.gpu
 
+FAMILY = "maxwell"
 WALLS = {"conductor": None, "magnetic": Magnetic,
          "mixed": lambda n: ((Conductor, Magnetic),) + ((Magnetic, Conductor),) * (n - 1)}
 
@@ -47,6 +48,11 @@ def vector_sim(Nx, order, walls="conductor", precision="float64", N=30, magnetic
     cls = magnetodielectric_class() if magnetic else MX.DielectricWave
     return cls(Nx, dx, N, 0.25 * min(dx), precision=precision, space_order=order,
                permittivity1=1.0, permittivity2=4.0, boundary=boundary)
+
+
+def misfit_case():
+    sim = vector_sim((16, 17, 18), 4, "mixed", N=40, magnetic=True)
+    return sim, np.random.default_rng(30).random(sim.Nx_padded)
 
 
 def logical(sim, a):
@@ -361,3 +367,31 @@ def test_a_designed_permeability_matches_finite_differences(Nx, order):
         gap = float(torch.linalg.vector_norm(g2[k] - grads[k]) /
                     torch.linalg.vector_norm(grads[k]))
         assert gap < 1e-9, (k, gap)
+
+
+@pytest.mark.parametrize("batch", [None, 2])
+def test_the_misfit_gradient_sums_the_shots(batch):
+    # one sweep of three shots against the three single-shot gradients, chained
+    from helpers import Receivers
+
+    from cuwave_amd import sensitivity as S
+    from cuwave_amd import utils as U
+
+    print(f"{FAMILY} misfit gradient batch={batch}", flush=True)
+    sim, gamma = misfit_case()
+    rng = np.random.default_rng(31)
+    sources = shots(sim, rng, 3)
+    sensors = sensor_points(sim, rng, 4)
+    observed = [0.5 * rng.standard_normal((sim.N, sensors.shape[1])) for _ in sources]
+    cost, gradient = U.misfit_gradient(sim, sources, gamma, Receivers(sensors), observed,
+                                       batch=batch)
+    want_cost, want = 0.0, 0.0
+    jm, js = sim.parametrization_jacobian(gamma)
+    for source, obs in zip(sources, observed):
+        c, g, _, _ = S.reconstruction_sensitivity(sim, source, gamma, sensors,
+                                                  S.l2_misfit(obs))
+        want_cost += c
+        want = want + jm * g["mass"].cpu().numpy() + js * g["stiff"].cpu().numpy()
+    assert abs(cost - want_cost) <= 1e-12 * want_cost
+    got = gradient.cpu().numpy()
+    assert np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want)

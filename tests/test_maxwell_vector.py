@@ -271,7 +271,7 @@ def test_indices_carry_the_component_row():
         sim.flatten_indices(np.array([[3], [3], [4], [5]]))
 
 
-def test_the_drivers_not_built_for_the_family_refuse_it():
+def test_the_drivers_refuse_what_the_family_does_not_take():
     from cuwave_amd import sensitivity as S
     from cuwave_amd import utils as U
     from cuwave_amd.wave import Source
@@ -282,9 +282,12 @@ def test_the_drivers_not_built_for_the_family_refuse_it():
     with pytest.raises(NotImplementedError, match="no strip"):
         S.reconstruction_sensitivity(sim, source, np.zeros(sim.Nx_padded), sensors, None,
                                      checkpoints=2)
-    with pytest.raises(NotImplementedError, match="not built for DielectricWave"):
+    with pytest.raises(NotImplementedError, match="checkpoints"):
         U.misfit_gradient(sim, [source], np.zeros(sim.Nx_padded), sensors,
-                          np.zeros((1, sim.N, 1)))
+                          np.zeros((1, sim.N, 1)), checkpoints=2)
+    with pytest.raises(NotImplementedError, match="scalar"):
+        U.misfit_gradient(sim, [source], np.zeros(sim.Nx_padded), sensors,
+                          np.zeros((1, sim.N, 1)), below=1.0)
 
 
 def test_the_designed_permeability_is_the_oracles():
