@@ -36,7 +36,7 @@ import torch
 
 from . import _abi
 from .boundary import Clamped, Traction, faces_with
-from .elastic import PAIRS, ElasticEngine, _require, adjoint_signal
+from .elastic import PAIRS, ShotFields, _require, adjoint_signal
 from .wave import MAX_BATCH, Simulation, Source, as_numpy, shot_lanes
 
 
@@ -264,7 +264,7 @@ class AnisotropicElasticWave(Simulation):
         return g
 
 
-class AnisoEngine(ElasticEngine):
+class AnisoEngine(ShotFields):
     """The step of one cell-assembled simulation bound to its material, for `nbatch`
     shots: fields (ncomp, *Nx_padded, lanes). The point lists are the elastic
     family's (`cw_aniso_points` runs the same kernel)."""
@@ -272,10 +272,7 @@ class AnisoEngine(ElasticEngine):
     points_entry = "cw_aniso_points"
 
     def __init__(self, sim: AnisotropicElasticWave, mat: dict, nbatch: int):
-        self.device = _abi.require_device()
-        self.sim, self.nbatch = sim, int(nbatch)
-        self.grid = sim.abi_grid(self.nbatch)
-        self.lanes = self.grid.lanes
+        super().__init__(sim, nbatch)
         self.keep = {k: torch.as_tensor(mat[k], device=self.device)
                      for k in ("table", "minv", "cell")}
         a = self.args = _abi.AnisoStepArgs()

@@ -313,8 +313,8 @@ class _Points:
         a.kind, a.row = kind, int(row)
         a.field, a.table = _abi.ptr(field), _abi.ptr(table)
         a.g_mass, a.u1, a.mass_factor = _abi.ptr(g_mass), _abi.ptr(u1), float(mf)
-        entry = getattr(self.eng, "points_entry", "cw_elastic_points")
-        _abi.call(entry, ctypes.byref(self.eng.grid), ctypes.byref(a), _abi.stream())
+        _abi.call(self.eng.points_entry, ctypes.byref(self.eng.grid), ctypes.byref(a),
+                  _abi.stream())
 
     def inject(self, field, table, row):
         self._call(0, field, table, row)
@@ -370,62 +370,23 @@ class _Pair:
         return u2
 
 
-class ElasticEngine:
-    """The kernels of one elastic simulation bound to its material, for `nbatch` shots.
+class ShotFields:
+    """What the engines of the vector families share: the device, the batch of shots
+    and its lanes, the ABI grid, the time levels and the point lists. Per-shot arrays
+    end in the lanes axis: a field level is (ncomp, *Nx_padded, lanes). A family names
+    the entry point its point lists run through (`points_entry`)."""
 
-    Per-shot arrays end in the lanes axis: a field level is (ncomp, *Nx_padded,
-    lanes), the stress scratch (nvoigt, *Nx_padded, lanes).
-    """
+    points_entry = "cw_elastic_points"
 
-    def __init__(self, sim: ElasticWave, mat: dict, nbatch: int):
+    def __init__(self, sim, nbatch: int):
         self.device = _abi.require_device()
         self.sim, self.nbatch = sim, int(nbatch)
         self.grid = sim.abi_grid(self.nbatch)
         self.lanes = self.grid.lanes
-        tables = closure_tables(sim.space_order, sim.stencil)
-        rows = np.concatenate((tables["node"].ravel(), tables["half"].ravel()))
-        self.rows = torch.as_tensor(rows.astype(sim.np_dtype), device=self.device)
-        self.gamma = mat["gamma"]
-        self.prep = torch.zeros((sim.nvoigt, *sim.Nx_padded), dtype=sim.dtype, device=self.device)
-        self.mat = _abi.ElasticMaterial(_abi.ptr(self.gamma), _abi.ptr(self.prep),
-                                        _abi.ptr(self.rows))
-        _abi.call("cw_elastic_prepare", ctypes.byref(self.grid), ctypes.byref(self.mat),
-                  _abi.stream())
-        self.sigma = torch.zeros((sim.nvoigt, *sim.Nx_padded, self.lanes), dtype=sim.dtype,
-                                 device=self.device)
-        self.slabs = []  # (axis, lo, hi, profile, plane)
-        if sim.pml is not None:
-            total = int(np.prod(sim.Nx_padded))
-            for d, ((lo, hi), prof) in enumerate(zip(pml_widths(sim), pml_profiles(sim))):
-                if prof is not None:
-                    self.slabs.append((d, lo, hi, torch.as_tensor(prof, device=self.device),
-                                       total // sim.Nx_padded[d]))
 
     def levels(self, slots: int) -> torch.Tensor:
         return torch.zeros((slots, *self.sim.field_shape, self.lanes), dtype=self.sim.dtype,
                            device=self.device)
-
-    def pair(self, adjoint: bool = False, with_pml: bool = True) -> _Pair:
-        return _Pair(self, adjoint, with_pml)
-
-    def gradient_fields(self) -> dict:
-        sim = self.sim
-        g = {"mass": torch.zeros((sim.ncomp, *sim.Nx_padded), dtype=sim.dtype, device=self.device),
-             "normal": torch.zeros(sim.Nx_padded, dtype=sim.dtype, device=self.device)}
-        if sim.ndim > 1:
-            g["shear"] = torch.zeros((sim.npairs, *sim.Nx_padded), dtype=sim.dtype,
-                                     device=self.device)
-        return g
-
-    def gradient(self, grads, u0, u1, u2, l1, mf) -> None:
-        """Both densities of the triplet against the adjoint l1 (cw_elastic_gradient)."""
-        a = _abi.ElasticGradientArgs()
-        a.grid, a.rows = self.grid, _abi.ptr(self.rows)
-        a.g_mass, a.g_normal = _abi.ptr(grads["mass"]), _abi.ptr(grads["normal"])
-        a.g_shear = _abi.ptr(grads.get("shear"))
-        a.u0, a.u1, a.u2, a.l1 = _abi.ptr(u0), _abi.ptr(u1), _abi.ptr(u2), _abi.ptr(l1)
-        a.mass_factor = float(mf)
-        _abi.call("cw_elastic_gradient", ctypes.byref(a), _abi.stream())
 
     def shots(self, t: torch.Tensor) -> torch.Tensor:
         """The (nbatch, ...) view of a per-shot array whose last axis is the lanes axis."""
@@ -466,6 +427,53 @@ class ElasticEngine:
         return _Points(self, lin, -1, np.arange(count, dtype=np.int32),
                        (count * self.lanes, self.lanes, 1), weight)
 
+
+class ElasticEngine(ShotFields):
+    """The kernels of one elastic simulation bound to its material, for `nbatch` shots:
+    the stress scratch is (nvoigt, *Nx_padded, lanes)."""
+
+    def __init__(self, sim: ElasticWave, mat: dict, nbatch: int):
+        super().__init__(sim, nbatch)
+        tables = closure_tables(sim.space_order, sim.stencil)
+        rows = np.concatenate((tables["node"].ravel(), tables["half"].ravel()))
+        self.rows = torch.as_tensor(rows.astype(sim.np_dtype), device=self.device)
+        self.gamma = mat["gamma"]
+        self.prep = torch.zeros((sim.nvoigt, *sim.Nx_padded), dtype=sim.dtype, device=self.device)
+        self.mat = _abi.ElasticMaterial(_abi.ptr(self.gamma), _abi.ptr(self.prep),
+                                        _abi.ptr(self.rows))
+        _abi.call("cw_elastic_prepare", ctypes.byref(self.grid), ctypes.byref(self.mat),
+                  _abi.stream())
+        self.sigma = torch.zeros((sim.nvoigt, *sim.Nx_padded, self.lanes), dtype=sim.dtype,
+                                 device=self.device)
+        self.slabs = []  # (axis, lo, hi, profile, plane)
+        if sim.pml is not None:
+            total = int(np.prod(sim.Nx_padded))
+            for d, ((lo, hi), prof) in enumerate(zip(pml_widths(sim), pml_profiles(sim))):
+                if prof is not None:
+                    self.slabs.append((d, lo, hi, torch.as_tensor(prof, device=self.device),
+                                       total // sim.Nx_padded[d]))
+
+    def pair(self, adjoint: bool = False, with_pml: bool = True) -> _Pair:
+        return _Pair(self, adjoint, with_pml)
+
+    def gradient_fields(self) -> dict:
+        sim = self.sim
+        g = {"mass": torch.zeros((sim.ncomp, *sim.Nx_padded), dtype=sim.dtype, device=self.device),
+             "normal": torch.zeros(sim.Nx_padded, dtype=sim.dtype, device=self.device)}
+        if sim.ndim > 1:
+            g["shear"] = torch.zeros((sim.npairs, *sim.Nx_padded), dtype=sim.dtype,
+                                     device=self.device)
+        return g
+
+    def gradient(self, grads, u0, u1, u2, l1, mf) -> None:
+        """Both densities of the triplet against the adjoint l1 (cw_elastic_gradient)."""
+        a = _abi.ElasticGradientArgs()
+        a.grid, a.rows = self.grid, _abi.ptr(self.rows)
+        a.g_mass, a.g_normal = _abi.ptr(grads["mass"]), _abi.ptr(grads["normal"])
+        a.g_shear = _abi.ptr(grads.get("shear"))
+        a.u0, a.u1, a.u2, a.l1 = _abi.ptr(u0), _abi.ptr(u1), _abi.ptr(u2), _abi.ptr(l1)
+        a.mass_factor = float(mf)
+        _abi.call("cw_elastic_gradient", ctypes.byref(a), _abi.stream())
 
 # ------------------------------------------------------------------ gradients
 def _point_average_adjoint(sim: ElasticWave, density: np.ndarray, c: int) -> np.ndarray:

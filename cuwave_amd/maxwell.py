@@ -48,7 +48,7 @@ import torch
 
 from . import _abi
 from .boundary import Conductor, Magnetic, face_mask, faces_with
-from .elastic import ElasticEngine, _require, adjoint_signal
+from .elastic import ShotFields, _require, adjoint_signal
 from .scalar import PressureWave
 from .stencils import staggered_weights
 from .wave import MAX_BATCH, Simulation, Source, as_numpy, shot_lanes
@@ -331,7 +331,7 @@ class DielectricWave(MaxwellWave):
         return self.permittivity2 - self.permittivity1, 0.0
 
 
-class MaxwellEngine(ElasticEngine):
+class MaxwellEngine(ShotFields):
     """The pair of one Maxwell simulation bound to its material, for `nbatch` shots.
 
     Per-shot arrays end in the lanes axis: a field level is (ncomp, *Nx_padded,
@@ -342,10 +342,7 @@ class MaxwellEngine(ElasticEngine):
     points_entry = "cw_maxwell_points"
 
     def __init__(self, sim: MaxwellWave, mat: dict, nbatch: int):
-        self.device = _abi.require_device()
-        self.sim, self.nbatch = sim, int(nbatch)
-        self.grid = sim.abi_grid(self.nbatch)
-        self.lanes = self.grid.lanes
+        super().__init__(sim, nbatch)
         self.minv = torch.as_tensor(mat["minv"], device=self.device)
         self.curl = torch.zeros((sim.npairs, *sim.Nx_padded, self.lanes), dtype=sim.dtype,
                                 device=self.device)
