@@ -924,6 +924,38 @@ def elastic_line(args) -> dict:
 MX_N, MX_SHOTS, MX_STEPS = 162, 8, 200
 
 
+class _Traces:
+    """Receivers on unknowns for `misfit_gradient`: the record is the traces."""
+
+    def __init__(self, nodes):
+        self.nodes = nodes
+
+    def objective(self, objective):
+        return objective
+
+
+def gradient_rate(sim, indicator, sources, sensors) -> dict:
+    """The misfit gradient of every shot in one batch against zero data (one forward
+    and one reverse sweep and the host chain rule), timed whole: W = 2 B n_int N."""
+    import torch
+
+    from cuwave_amd import utils
+
+    observed = [np.zeros((sim.N, sensors.shape[1]), dtype=sim.np_dtype)] * len(sources)
+    n_int = int(np.prod([n - 2 for n in sim.Nx]))
+    utils.misfit_gradient(sim, sources, indicator, _Traces(sensors), observed)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    utils.misfit_gradient(sim, sources, indicator, _Traces(sensors), observed)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    return {"value": 2 * n_int * len(sources) * sim.N / seconds / 1e9,
+            "unit": "Gcell-updates/s", "seconds": seconds,
+            "note": "misfit_gradient of every shot in one batch, the field rebuilt by the "
+                    "reverse march; forward and adjoint updates counted, host chain rule "
+                    "included"}
+
+
 def workload_maxwell(nt: int = MX_STEPS, shots: int = MX_SHOTS, n: int = MX_N):
     """The 3D vector Maxwell forward: a conducting box of n^3 nodes (n - 2 interior a
     side), dx = 1 / (n - 3), order 4, float32; eps 1 with an eps 4 sphere of radius
@@ -1034,6 +1066,7 @@ def maxwell_line(args) -> dict:
         "cpu_baseline": cpu,
         "kernels": {"step_ms": step_ms, "bytes_per_node": per_node,
                     "note": "a step's launches (curl, update, sources, probes) on the stream"},
+        "gradient": gradient_rate(sim, indicator, sources, sensors),
     }
 
 
@@ -1148,6 +1181,7 @@ def aniso_line(args) -> dict:
         "cpu_baseline": cpu,
         "kernels": {"step_ms": step_ms, "bytes_per_node": per_node,
                     "note": "a step's launches (the step, sources, probes) on the stream"},
+        "gradient": gradient_rate(sim, indicator, sources, sensors),
     }
 
 
