@@ -820,6 +820,7 @@ def main():
         if args.gpus != 1:
             print(json.dumps({"error": "this build runs on one GPU"}))
             sys.exit(2)
+        args.nt = 1500 if args.nt == STEPS else args.nt  # the 3D case's own step count
         res = gpu_run_3d(args)
         if not args.no_cpu:
             cores = len(os.sched_getaffinity(0))
