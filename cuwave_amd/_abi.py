@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 13
+ABI_VERSION = 14
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -255,6 +255,8 @@ class MaxwellStepArgs(ctypes.Structure):
         ("g_mass", VP),
         ("v1", VP),
         ("mass_factor", F64),
+        ("damping", VP),
+        ("dt", F64),
     ]
 
 

@@ -103,11 +103,12 @@ int step_lanes(const cw_maxwell_step_args &a, hipStream_t stream) {
     hipLaunchKernelGGL((cwm::mx_update<T, NDIM, R, V, true>), dim3(blocks), dim3(cwm::kThreads),
                        0, stream, gu, (const T *)a.minv, (const T *)h, (const T *)a.u0,
                        (const T *)a.u1, (T *)a.u2, (T *)a.g_mass, (const T *)a.v1,
-                       (T)a.mass_factor);
+                       (T)a.mass_factor, (const T *)a.damping, (T)a.dt);
   else
     hipLaunchKernelGGL((cwm::mx_update<T, NDIM, R, V, false>), dim3(blocks), dim3(cwm::kThreads),
                        0, stream, gu, (const T *)a.minv, (const T *)h, (const T *)a.u0,
-                       (const T *)a.u1, (T *)a.u2, (T *)nullptr, (const T *)nullptr, (T)0);
+                       (const T *)a.u1, (T *)a.u2, (T *)nullptr, (const T *)nullptr, (T)0,
+                       (const T *)a.damping, (T)a.dt);
   return hip_status("maxwell step");
 }
 

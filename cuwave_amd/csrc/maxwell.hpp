@@ -198,7 +198,7 @@ template <typename T, int NDIM, int R, int V, bool GRAD>
 __global__ void __launch_bounds__(kThreads)
 mx_update(MxGeom<T> g, const T *__restrict__ minv, const T *__restrict__ h, const T *u0,
           const T *__restrict__ u1, T *u2, T *__restrict__ g_mass, const T *__restrict__ v1,
-          T mf) {
+          T mf, const T *__restrict__ damping, T dt) {
   using Q = LanesOf<T, V>;
   int A[3];
   int64_t idx;
@@ -215,7 +215,14 @@ mx_update(MxGeom<T> g, const T *__restrict__ minv, const T *__restrict__ h, cons
     const int64_t at = n * g.lanes + lane0;
     const T mi = minv[n];
     const Q w0 = lanes_at<Q>(u0, at), w1 = lanes_at<Q>(u1, at);
-    const Q next = -w0 + (T)2 * w1 + mi * force;
+    Q next;
+    if (damping) {
+      // the damped update (maxwell.cu:257-264), damping nodal
+      const T beta = (T)0.5 * mi * damping[idx] * dt;
+      next = ((T)2 * w1 - w0 * ((T)1 - beta) + mi * force) / ((T)1 + beta);
+    } else {
+      next = -w0 + (T)2 * w1 + mi * force;
+    }
     if constexpr (GRAD) {
       const Q v = lanes_at<Q>(v1, at);
       const T sum = cw::shot_sum(mf * v * (next - (T)2 * w1 + w0), g.lanes);

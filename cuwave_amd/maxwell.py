@@ -30,7 +30,10 @@ the shots innermost in every per-shot array as in the other families:
 
 The host-side parametrization (the point inertia, the excitation weights,
 the chain rule) runs in numpy in the simulation's precision, as the reference
-computes it. A subclass with `magnetic = True` designs the permeability too
+computes it. A nodal `damping` field (a sponge) damps the update as the
+reference's USE_DAMPING build does; the stored-history `sensitivity` takes it,
+the reverse march does not. A subclass with `magnetic = True` designs the
+permeability too
 (the reference's `USE_MAGNETIC` build): the curl then reads nu W per pair
 point and the adjoint also accumulates the permeability density. Not built
 here: a pml, damping, the fused and blocked launch tiers.
@@ -134,6 +137,7 @@ class MaxwellWave(Simulation):
     default_boundary = Conductor
     face_conditions = (Conductor, Magnetic)
     magnetic = False  # whether the permeability carries the design too
+    accepts_damping = True  # a nodal conductivity-like field, as a sponge builds it
 
     def __post_init__(self) -> None:
         super().__post_init__()
@@ -242,6 +246,13 @@ class MaxwellWave(Simulation):
                     minv[(c, *wall)] = 0.0
         return minv
 
+    def inverse_inertia(self, indicator) -> np.ndarray:
+        """Nodal 1 / (eps W), what a sponge scales its damping by (cuwave/maxwell.py:239-245)."""
+        eps = np.asarray(self.parametrization(np.asarray(indicator, dtype=self.np_dtype))[0],
+                         dtype=self.np_dtype)
+        mass = eps * self.cell_weights()
+        return (1.0 / np.maximum(mass, np.finfo(self.np_dtype).tiny)).astype(self.np_dtype)
+
     def indicator_shape(self) -> tuple[int, ...]:
         return self.Nx_padded
 
@@ -254,6 +265,8 @@ class MaxwellWave(Simulation):
         eps, nu = self.parametrization(gamma)
         eps = np.ascontiguousarray(eps, dtype=self.np_dtype)
         mat = {"eps": eps, "minv": self.point_inverse_inertia(eps)}
+        if self.damping is not None:
+            mat["damping"] = self.damping
         if self.magnetic:
             # nu W of every pair point, the weight folded in (cuwave/maxwell.py:264-270)
             nu = np.ascontiguousarray(nu, dtype=self.np_dtype)
@@ -263,9 +276,15 @@ class MaxwellWave(Simulation):
         return mat
 
     def excitation_weights(self, mat: dict, lin) -> np.ndarray:
-        """Source weights dt^2 / (eps V) (cuwave/maxwell.py:561-574)."""
+        """Source weights dt^2 / (eps V), over 1 + beta where damped (cuwave/maxwell.py:
+        561-574)."""
         volume = self.np_dtype(1.0 / float(np.prod(self.dx)))
-        weight = mat["minv"].ravel()[np.asarray(lin, dtype=np.int64)] * volume
+        lin = np.asarray(lin, dtype=np.int64)
+        minv = mat["minv"].ravel()[lin]
+        weight = minv * volume
+        if self.damping is not None:
+            beta = 0.5 * minv * mat["damping"].ravel()[lin % self.comp_stride] * self.dt
+            weight = weight / (1.0 + beta)
         return (self.np_dtype(self.dt**2 * self.source_factor()) * weight).astype(self.np_dtype)
 
     def adjoint_weights(self, sensors) -> np.ndarray:
@@ -348,10 +367,13 @@ class MaxwellEngine(ShotFields):
                                 device=self.device)
         self.nu_pair = (torch.as_tensor(mat["nu_pair"], device=self.device) if sim.magnetic
                         else None)
+        self.damping = (torch.as_tensor(mat["damping"], device=self.device)
+                        if sim.damping is not None else None)
         self.args = _abi.MaxwellStepArgs()
         self.args.grid = self.grid
         self.args.minv, self.args.curl = _abi.ptr(self.minv), _abi.ptr(self.curl)
         self.args.nu_pair = _abi.ptr(self.nu_pair)
+        self.args.damping, self.args.dt = _abi.ptr(self.damping), float(sim.dt)
 
     def step(self, u0, u1, u2, g_mass=None, v1=None, mf=0.0, g_nu=None):
         """u2 of the pair (u2 may be u0); with g_mass the adjoint step's inertia density of
@@ -503,6 +525,9 @@ def _reverse_sweep(sim: MaxwellWave, mat: dict, sources, sensors, objectives):
     `objectives(records)` maps the (B, N, count) records to (costs, (B, N, count) dJ/du).
     Returns (costs, grads, records, drift), `drift` the rebuilt initial state against
     the last level."""
+    if sim.damping is not None:
+        raise NotImplementedError("a damped field does not run backwards: use sensitivity, "
+                                  "which stores the history")
     eng = MaxwellEngine(sim, mat, len(sources))
     src, signal = eng.sources(mat, sources)
     rec = eng.sensors(sensors)

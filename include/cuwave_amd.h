@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 13
+#define CW_ABI_VERSION 14
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -443,6 +443,11 @@ typedef struct cw_maxwell_step_args {
   void *g_mass;     /* NULL for a forward step */
   const void *v1;
   double mass_factor; /* 1 / dt^2 (cuwave/wave.py:597-599) */
+  /* a nodal damping field d, or NULL: the update becomes (2 u1 - u0 (1 - beta)
+   * + minv force) / (1 + beta), beta = minv d dt / 2 (the USE_DAMPING build,
+   * maxwell.cu:257-264); the adjoint step is the same damped step */
+  const void *damping;
+  double dt;
 } cw_maxwell_step_args;
 
 int cw_maxwell_step(const cw_maxwell_step_args *args, void *stream);

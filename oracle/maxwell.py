@@ -5,8 +5,9 @@ restates the reference's curl-curl pair on the Yee lattice in numpy, whole
 arrays at a time, one shot, on the logical grid (no padding):
 
 - `curl` is curl_kernel (cuwave/kernels/maxwell.cu:49-73, 209-239);
-- `force` and `step` are fd_kernel without damping (maxwell.cu:171-204,
-  242-274), `step(..., v1=, g_mass=)` its GRADIENT build;
+- `force` and `step` are fd_kernel (maxwell.cu:171-204, 242-274), with a
+  nodal damping field its USE_DAMPING build, `step(..., v1=, g_mass=)` its
+  GRADIENT build;
 - `point_inverse_inertia` is MaxwellWave.build_materials with the
   permittivity's harmonic point means and the wall weights
   (cuwave/maxwell.py:247-272, cuwave/wave.py:386-425), and
@@ -138,9 +139,11 @@ def force(h: np.ndarray, c: int, w: np.ndarray, f, conductor: int, ndim: int) ->
 
 
 def step(u0, u1, minv, w, inv_dx, step_f, nu, conductor, v1=None, g_mass=None, mf=0.0,
-         nu_pair=None, g_nu=None):
+         nu_pair=None, g_nu=None, damping=None, dt_=0.0):
     """u2 of the pair; with v1 and g_mass the inertia density taken off g_mass in place,
-    with g_nu too the permeability density (maxwell.cu:209-274)."""
+    with g_nu too the permeability density; with a nodal `damping` the damped update
+    (2 u1 - u0 (1 - beta) + minv force) / (1 + beta), beta = minv d dt / 2
+    (maxwell.cu:209-274)."""
     ndim = u1.shape[0]
     shape = u1.shape[1:]
     dt = u1.dtype.type
@@ -148,7 +151,12 @@ def step(u0, u1, minv, w, inv_dx, step_f, nu, conductor, v1=None, g_mass=None, m
     u2 = np.zeros_like(u1)
     for c in range(ndim):
         valid = interior(shape) & (axis_index(shape, c) <= shape[c] - 3)
-        nxt = -u0[c] + dt(2) * u1[c] + minv[c] * force(h, c, w, step_f, conductor, ndim)
+        F = force(h, c, w, step_f, conductor, ndim)
+        if damping is None:
+            nxt = -u0[c] + dt(2) * u1[c] + minv[c] * F
+        else:
+            beta = dt(0.5) * minv[c] * damping * dt(dt_)
+            nxt = (dt(2) * u1[c] - u0[c] * (dt(1) - beta) + minv[c] * F) / (dt(1) + beta)
         if g_mass is not None:
             dens = (dt(mf) * v1[c]) * (nxt - dt(2) * u1[c] + u0[c])
             g_mass[c] = np.where(valid, g_mass[c] - dens, g_mass[c])
@@ -230,7 +238,7 @@ def pair_inverse_permeability(nu: np.ndarray) -> np.ndarray:
 
 
 def simulate(minv, w, inv_dx, step_f, nu, conductor, nsteps, src, weight, signal, probes,
-             nu_pair=None):
+             nu_pair=None, damping=None, dt_=0.0):
     """The forward march of one shot: u^t = step(u^(t-2), u^(t-1)), then the sources
     (field += weight signal, entry by entry), then the probes. `src`, `probes` are
     (component, *node) index tuples of arrays; returns (the last level, (N, count))."""
@@ -240,7 +248,8 @@ def simulate(minv, w, inv_dx, step_f, nu, conductor, nsteps, src, weight, signal
     u1 = np.zeros(shape, dtype=dt)
     traces = np.zeros((nsteps, len(probes[0])), dtype=dt)
     for t in range(nsteps):
-        u2 = step(u0, u1, minv, w, inv_dx, step_f, nu, conductor, nu_pair=nu_pair)
+        u2 = step(u0, u1, minv, w, inv_dx, step_f, nu, conductor, nu_pair=nu_pair,
+                  damping=damping, dt_=dt_)
         for e in range(len(src[0])):
             at = tuple(int(s[e]) for s in src)
             u2[at] = u2[at] + weight[e] * signal[t, e]

@@ -395,3 +395,72 @@ def test_the_misfit_gradient_sums_the_shots(batch):
     assert abs(cost - want_cost) <= 1e-12 * want_cost
     got = gradient.cpu().numpy()
     assert np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want)
+
+
+# ---------------------------------------------------------------- damping
+def damped(sim, gamma, width=4, beta=0.2):
+    import dataclasses
+
+    from cuwave_amd import boundary
+
+    return dataclasses.replace(sim, damping=boundary.sponge(sim, gamma, width, beta))
+
+
+@pytest.mark.parametrize("Nx, order, precision", [((22, 26), 4, "float64"),
+                                                  ((14, 16, 18), 2, "float64"),
+                                                  ((22, 26), 6, "float32")])
+def test_a_damped_march_is_the_oracles_bit_for_bit(Nx, order, precision):
+    print(f"maxwell damped forward {Nx} order {order} {precision}", flush=True)
+    rng = np.random.default_rng(41)
+    sim = vector_sim(Nx, order, "mixed", precision=precision)
+    gamma = rng.random(sim.Nx_padded).astype(sim.np_dtype)
+    sim = damped(sim, gamma)
+    sources = shots(sim, rng, 2)
+    sensors = sensor_points(sim, rng)
+    field, record = MX.simulate_shots(sim, sources, gamma, sensors)
+    mat = sim.build_materials(gamma)
+    for b, source in enumerate(sources):
+        pos = np.asarray(source.position)
+        weight = sim.excitation_weights(mat, sim.flatten_indices(pos))
+        d = sim.np_dtype
+        last, traces = M.simulate(
+            logical(sim, mat["minv"]), staggered_weights(sim.radius, sim.stencil).astype(d),
+            [d(1.0 / x) for x in sim.dx], sim.step_factors(), d(1.0), sim.abi_grid(1).conductor,
+            sim.N, tuple(pos), weight, np.asarray(source.signal, dtype=d), tuple(sensors),
+            damping=logical(sim, sim.damping), dt_=sim.dt)
+        assert np.array_equal(field[b].cpu().numpy(), last), f"shot {b}"
+        assert np.array_equal(record[b].cpu().numpy(), traces), f"shot {b}"
+
+
+@pytest.mark.parametrize("Nx, order", [((16, 18), 4), ((10, 11, 12), 2)])
+def test_a_damped_gradient_matches_finite_differences(Nx, order):
+    print(f"maxwell damped FD {Nx} order {order}", flush=True)
+    sim = vector_sim(Nx, order, "mixed", N=60)
+    rng = np.random.default_rng(42)
+    gamma = 0.3 + 0.4 * rng.random(sim.Nx_padded)
+    sim = damped(sim, 0.5 * np.ones(sim.Nx_padded), width=3, beta=0.3)
+    t = np.arange(sim.N) * sim.dt
+    f0 = 4.0 / (sim.N * sim.dt)
+    arg = (np.pi * f0 * (t - 1.2 / f0)) ** 2
+    pulse = (1 - 2 * arg) * np.exp(-arg)
+    centre = [n // 2 for n in Nx]
+    source = Source(np.array([[1], *([c] for c in centre)]), pulse[:, None])
+    sensors = np.array([[0, 1, sim.ncomp - 1], *([c - 2, c + 2, c + 1] for c in centre)])
+    observed = MX.simulate(sim, source, 0.5 * np.ones(sim.Nx_padded), sensors)[1].cpu().numpy()
+
+    def objective(um):
+        r = um.cpu().numpy() - observed
+        return 0.5 * float(np.sum(r * r)), torch.as_tensor(r)
+
+    cost, grads, _, _ = MX.sensitivity(sim, source, gamma, sensors, objective)
+    d_mass, _ = sim.parametrization_jacobian()
+    g = d_mass * grads["mass"].cpu().numpy()
+    h = 1e-5
+    for node in [tuple(c + 1 for c in centre), tuple(c - 1 for c in centre), (2,) * len(Nx)]:
+        costs = []
+        for sgn in (1, -1):
+            probe = gamma.copy()
+            probe[node] += sgn * h
+            costs.append(objective(MX.simulate(sim, source, probe, sensors)[1])[0])
+        fd = (costs[0] - costs[1]) / (2 * h)
+        assert abs(g[node] - fd) <= 1e-5 * abs(fd) + 1e-10 * abs(cost), (node, g[node], fd)

@@ -329,3 +329,40 @@ def test_a_designed_permeability_keeps_the_operator_symmetric():
     y = np.where(free, rng.standard_normal(free.shape), 0.0)
     ax, ay = apply(x), apply(y)
     assert abs(np.sum(y * ax) - np.sum(x * ay)) / (np.linalg.norm(ax) * np.linalg.norm(y)) < 1e-12
+
+
+def test_the_damped_update_is_the_damped_particle():
+    # magnetic walls keep a uniform field curl-free, so each point follows
+    # x2 = (2 x1 - (1 - beta) x0) / (1 + beta), beta = minv d dt / 2
+    # (cuwave tests/maxwell_test.py:340-345, 560-622)
+    rng = np.random.default_rng(11)
+    shape = (12, 13, 14)
+    minv = M.point_inverse_inertia(1.0 + rng.random(shape), 0)
+    damping = rng.random(shape)
+    dt = 0.01
+    a, b = rng.standard_normal(3), rng.standard_normal(3)
+    u0 = np.broadcast_to(a[:, None, None, None], (3, *shape)).copy()
+    u1 = np.broadcast_to(b[:, None, None, None], (3, *shape)).copy()
+    u2 = M.step(u0, u1, minv, staggered_weights(2), [10.0] * 3, [1e-3] * 3, 1.0, 0,
+                damping=damping, dt_=dt)
+    for c in range(3):
+        valid = (slice(1, -1),) * 3
+        valid = tuple(slice(1, n - 2) if d == c else slice(1, n - 1) for d, n in enumerate(shape))
+        beta = 0.5 * minv[c][valid] * damping[valid] * dt
+        want = (2 * b[c] - (1 - beta) * a[c]) / (1 + beta)
+        assert np.allclose(u2[c][valid], want, rtol=1e-14, atol=1e-14)
+
+
+def test_the_reverse_march_refuses_a_damped_field():
+    import dataclasses
+
+    from cuwave_amd import boundary
+    from cuwave_amd import sensitivity as S
+    from cuwave_amd.wave import Source
+
+    sim = vector_sim((12, 13))
+    sim = dataclasses.replace(sim, damping=boundary.sponge(sim, np.zeros(sim.Nx_padded), 3, 0.1))
+    source = Source(np.array([[0], [5], [6]]), np.zeros((sim.N, 1)))
+    with pytest.raises(NotImplementedError, match="damped"):
+        S.reconstruction_sensitivity(sim, source, np.zeros(sim.Nx_padded),
+                                     np.array([[1], [5], [7]]), lambda u: (0.0, u))
