@@ -18,7 +18,7 @@ from pathlib import Path
 import torch
 
 LIBRARY = Path(__file__).resolve().parent / "libcuwave_amd.so"
-ABI_VERSION = 14
+ABI_VERSION = 15
 DEVICE = "cuda"  # where every tensor handed to the library must live
 
 I32, I64, F64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -281,6 +281,8 @@ class AnisoStepArgs(ctypes.Structure):
         ("u0", VP),
         ("u1", VP),
         ("u2", VP),
+        ("damping", VP),
+        ("dt", F64),
     ]
 
 

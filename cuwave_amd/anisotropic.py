@@ -21,7 +21,9 @@ reference's closures over a batch of shots, the shots innermost:
 
 The host-side parametrization (the one-cell stencil, the lumped inertia, the
 harmonic cell means, the chain rule) runs in numpy in the simulation's
-precision, as the reference computes it. Not built here: a pml, damping, the
+precision, as the reference computes it. A nodal `damping` field (a sponge)
+damps the update as the reference's USE_DAMPING build does; the stored-history
+`sensitivity` takes it, the reverse march does not. Not built here: a pml, the
 blocked launch tier.
 """
 
@@ -115,6 +117,7 @@ class AnisotropicElasticWave(Simulation):
 
     default_boundary = Traction
     face_conditions = (Traction, Clamped)
+    accepts_damping = True  # a nodal damping field, as a sponge builds it
 
     def __post_init__(self) -> None:
         super().__post_init__()
@@ -195,11 +198,18 @@ class AnisotropicElasticWave(Simulation):
                 w[tuple(face)] *= 0.5
         return w
 
-    def inverse_inertia(self, gamma: np.ndarray) -> np.ndarray:
-        """The lumped 1 / (gamma rho0 V W), zero on a clamped wall (cuwave/anisotropic.py:260-280)."""
+    def inverse_inertia(self, gamma) -> np.ndarray:
+        """The lumped 1 / (gamma rho0 V W) (cuwave/anisotropic.py:260-264), what a sponge
+        scales its damping by."""
         volume = float(np.prod(self.dx))
-        mass = gamma * self.np_dtype(self.density * volume) * self.cell_weights()
-        minv = (1.0 / np.maximum(mass, np.finfo(self.np_dtype).tiny)).astype(self.np_dtype)
+        mass = (np.asarray(gamma, dtype=self.np_dtype) * self.np_dtype(self.density * volume)
+                * self.cell_weights())
+        return (1.0 / np.maximum(mass, np.finfo(self.np_dtype).tiny)).astype(self.np_dtype)
+
+    def held_inverse_inertia(self, gamma: np.ndarray) -> np.ndarray:
+        """The inverse inertia the step reads: zero on a clamped wall node, which holds it
+        (cuwave/anisotropic.py:266-272)."""
+        minv = self.inverse_inertia(gamma)
         for face in faces_with(self, Clamped):
             wall = [slice(None)] * self.ndim
             wall[face // 2] = 1 if face % 2 == 0 else self.Nx[face // 2] - 2
@@ -226,13 +236,20 @@ class AnisotropicElasticWave(Simulation):
         gamma = np.ascontiguousarray(as_numpy(indicator), dtype=self.np_dtype)
         if gamma.shape != self.indicator_shape():
             raise ValueError(f"indicator of shape {gamma.shape}, expected {self.indicator_shape()}")
-        return {"gamma": gamma, "minv": self.inverse_inertia(gamma),
-                "cell": self.cell_average(gamma), "table": self.stencil_table()}
+        mat = {"gamma": gamma, "minv": self.held_inverse_inertia(gamma),
+               "cell": self.cell_average(gamma), "table": self.stencil_table()}
+        if self.damping is not None:
+            mat["damping"] = self.damping
+        return mat
 
     def excitation_weights(self, mat: dict, lin) -> np.ndarray:
-        """dt^2 / inertia at the node of each point (cuwave/anisotropic.py:290-301)."""
+        """dt^2 / inertia at the node of each point, over 1 + beta where damped
+        (cuwave/anisotropic.py:290-301)."""
         node = np.asarray(lin, dtype=np.int64) % self.comp_stride
         weight = mat["minv"].ravel()[node]
+        if self.damping is not None:
+            beta = 0.5 * weight * mat["damping"].ravel()[node] * self.dt
+            weight = weight / (1.0 + beta)
         return (self.np_dtype(self.dt**2 * self.source_factor()) * weight).astype(self.np_dtype)
 
     def adjoint_weights(self, sensors) -> np.ndarray:
@@ -274,10 +291,11 @@ class AnisoEngine(ShotFields):
     def __init__(self, sim: AnisotropicElasticWave, mat: dict, nbatch: int):
         super().__init__(sim, nbatch)
         self.keep = {k: torch.as_tensor(mat[k], device=self.device)
-                     for k in ("table", "minv", "cell")}
+                     for k in ("table", "minv", "cell", "damping") if k in mat}
         a = self.args = _abi.AnisoStepArgs()
         a.grid = self.grid
         a.table, a.minv, a.cell = (_abi.ptr(self.keep[k]) for k in ("table", "minv", "cell"))
+        a.damping, a.dt = _abi.ptr(self.keep.get("damping")), float(sim.dt)
 
     def step(self, u0, u1, u2):
         """u2 of the step (u2 may be u0) (cw_aniso_step)."""
@@ -401,6 +419,9 @@ def _reverse_sweep(sim: AnisotropicElasticWave, mat: dict, sources, sensors, obj
     the gradient pairs with lambda, so no strip is kept. Every shot's densities are
     summed over the lanes in the kernels. `objectives(records)` maps the (B, N, count)
     records to (costs, (B, N, count) dJ/du). Returns (costs, grads, records, drift)."""
+    if sim.damping is not None:
+        raise NotImplementedError("a damped field does not run backwards: use sensitivity, "
+                                  "which stores the history")
     eng = AnisoEngine(sim, mat, len(sources))
     src, signal = eng.sources(mat, sources)
     rec = eng.sensors(sensors)

@@ -80,7 +80,7 @@ template <typename T, int NDIM, int V> int step_lanes(const cw_aniso_step_args &
   hipLaunchKernelGGL((cwa::an_step<T, NDIM, V>), dim3(blocks_for(a.grid, V)),
                      dim3(cwa::kThreads), 0, s, make_geom<T>(a.grid), (const T *)a.table,
                      (const T *)a.minv, (const T *)a.cell, (const T *)a.u0, (const T *)a.u1,
-                     (T *)a.u2);
+                     (T *)a.u2, (const T *)a.damping, (T)a.dt);
   return hip_status("anisotropic step");
 }
 

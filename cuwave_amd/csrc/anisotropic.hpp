@@ -69,7 +69,8 @@ __device__ __forceinline__ bool an_node(const AnGeom<T> &g, int *A, int64_t *idx
 template <typename T, int NDIM, int V>
 __global__ void __launch_bounds__(kThreads)
 an_step(AnGeom<T> g, const T *__restrict__ K, const T *__restrict__ minv,
-        const T *__restrict__ cell, const T *u0, const T *__restrict__ u1, T *u2) {
+        const T *__restrict__ cell, const T *u0, const T *__restrict__ u1, T *u2,
+        const T *__restrict__ damping, T dt) {
   using Q = LanesOf<T, V>;
   constexpr int NL = nloc<NDIM>();
   int A[3];
@@ -113,10 +114,14 @@ an_step(AnGeom<T> g, const T *__restrict__ K, const T *__restrict__ minv,
     }
   }
   const T mi = minv[idx];
+  // with a damping field the damped update (anisotropic.cu:380-389), nodal beta
+  const T beta = damping ? (T)0.5 * mi * damping[idx] * dt : (T)0;
 #pragma unroll
   for (int i = 0; i < NDIM; ++i) {
     const int64_t at = (i * g.nodes + idx) * g.lanes + lane0;
-    const Q next = -lanes_at<Q>(u0, at) + (T)2 * lanes_at<Q>(u1, at) + (g.f * mi) * force[i];
+    const Q w0 = lanes_at<Q>(u0, at), w1 = lanes_at<Q>(u1, at);
+    const Q next = damping ? ((T)2 * w1 - w0 * ((T)1 - beta) + g.f * mi * force[i]) / ((T)1 + beta)
+                           : -w0 + (T)2 * w1 + (g.f * mi) * force[i];
     lanes_put(u2, at, next);
   }
 }

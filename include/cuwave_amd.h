@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-#define CW_ABI_VERSION 14
+#define CW_ABI_VERSION 15
 
 /* the grid and the operator of one pressure simulation (cuwave/wave.py:479-631,
  * cuwave/scalar.py:140-160): the per-axis step factors and the full-radius
@@ -481,6 +481,11 @@ typedef struct cw_aniso_step_args {
   const void *cell;  /* (*np) the cell material at each cell's low corner */
   const void *u0, *u1;
   void *u2;
+  /* a nodal damping field d, or NULL: u2 = (2 u1 - u0 (1 - beta) + dt^2 minv
+   * force) / (1 + beta), beta = minv d dt / 2 (the USE_DAMPING build,
+   * anisotropic.cu:380-389) */
+  const void *damping;
+  double dt;
 } cw_aniso_step_args;
 
 /* the densities of a forward triplet against the adjoint l1

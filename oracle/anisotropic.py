@@ -75,12 +75,18 @@ def force(u: np.ndarray, cell: np.ndarray, K: np.ndarray) -> np.ndarray:
     return out
 
 
-def step(u0, u1, minv, cell, K, f):
-    """u2 = -u0 + 2 u1 + (f minv) force on the interior nodes, zero elsewhere."""
+def step(u0, u1, minv, cell, K, f, damping=None, dt_=0.0):
+    """u2 = -u0 + 2 u1 + (f minv) force on the interior nodes, zero elsewhere; with a
+    nodal `damping` the damped update (2 u1 - u0 (1 - beta) + f minv force) / (1 + beta),
+    beta = minv d dt / 2 (anisotropic.cu:380-389)."""
     F = force(u1, cell, K)
     dt = u1.dtype.type
     inside = interior(u1.shape[1:])
-    return np.where(inside, -u0 + dt(2) * u1 + (dt(f) * minv) * F, 0)
+    if damping is None:
+        return np.where(inside, -u0 + dt(2) * u1 + (dt(f) * minv) * F, 0)
+    beta = dt(0.5) * minv * damping * dt(dt_)
+    return np.where(inside, (dt(2) * u1 - u0 * (dt(1) - beta) + (dt(f) * minv) * F) /
+                    (dt(1) + beta), 0)
 
 
 def gradient(g_mass, g_cell, u0, u1, u2, l1, K, mf):
@@ -147,7 +153,7 @@ def cell_average(gamma: np.ndarray) -> np.ndarray:
     return out
 
 
-def simulate(minv, cell, K, f, nsteps, src, weight, signal, probes):
+def simulate(minv, cell, K, f, nsteps, src, weight, signal, probes, damping=None, dt_=0.0):
     """The forward march of one shot; `src`, `probes` are (component, *node) index
     tuples. Returns (the last level, (N, count))."""
     shape = (minv.ndim, *minv.shape)
@@ -156,7 +162,7 @@ def simulate(minv, cell, K, f, nsteps, src, weight, signal, probes):
     u1 = np.zeros(shape, dtype=dt)
     traces = np.zeros((nsteps, len(probes[0])), dtype=dt)
     for t in range(nsteps):
-        u2 = step(u0, u1, minv, cell, K, f)
+        u2 = step(u0, u1, minv, cell, K, f, damping, dt_)
         for e in range(len(src[0])):
             at = tuple(int(s[e]) for s in src)
             u2[at] = u2[at] + weight[e] * signal[t, e]

@@ -182,3 +182,37 @@ def test_the_family_refuses_what_it_cannot_step():
     with pytest.raises(ValueError, match="below"):
         AN.AnisotropicElasticWave((10, 12), (0.1, 0.1), 5, 1e-3, density=1.0, wavespeed_p=1.0,
                                   wavespeed_s=1.0)
+
+
+def test_the_damped_update_is_the_damped_particle():
+    # a rigid translation carries no force, so with damping each node follows
+    # x2 = (2 x1 - (1 - beta) x0) / (1 + beta), beta = minv d dt / 2
+    sim = aniso_sim((12, 13, 14))
+    rng = np.random.default_rng(12)
+    minv, cell, K = materials(sim, 0.5 + rng.random(sim.Nx_padded))
+    damping = rng.random(sim.Nx)
+    dt = 1e-3
+    a, b = rng.standard_normal(3), rng.standard_normal(3)
+    u0 = np.broadcast_to(a[:, None, None, None], (3, *sim.Nx)).copy()
+    u1 = np.broadcast_to(b[:, None, None, None], (3, *sim.Nx)).copy()
+    u2 = O.step(u0, u1, minv, cell, K, 1e-6, damping, dt)
+    inner = (slice(1, -1),) * 3
+    beta = 0.5 * minv[inner] * damping[inner] * dt
+    for c in range(3):
+        want = (2 * b[c] - (1 - beta) * a[c]) / (1 + beta)
+        assert np.allclose(u2[c][inner], want, rtol=1e-12, atol=1e-12)
+
+
+def test_the_reverse_march_refuses_a_damped_field():
+    import dataclasses
+
+    from cuwave_amd import boundary
+    from cuwave_amd import sensitivity as S
+    from cuwave_amd.wave import Source
+
+    sim = aniso_sim((12, 13))
+    sim = dataclasses.replace(sim, damping=boundary.sponge(sim, np.ones(sim.Nx_padded), 3, 0.1))
+    source = Source(np.array([[0], [5], [6]]), np.zeros((sim.N, 1)))
+    with pytest.raises(NotImplementedError, match="damped"):
+        S.reconstruction_sensitivity(sim, source, np.ones(sim.Nx_padded),
+                                     np.array([[1], [5], [7]]), lambda u: (0.0, u))

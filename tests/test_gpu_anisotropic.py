@@ -248,3 +248,67 @@ def test_the_misfit_gradient_sums_the_shots(batch):
     assert abs(cost - want_cost) <= 1e-12 * want_cost
     got = gradient.cpu().numpy()
     assert np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want)
+
+
+# ---------------------------------------------------------------- damping
+def damped(sim, gamma, width=4, beta=0.2):
+    import dataclasses
+
+    from cuwave_amd import boundary
+
+    return dataclasses.replace(sim, damping=boundary.sponge(sim, gamma, width, beta))
+
+
+@pytest.mark.parametrize("Nx, faces, precision", [((22, 26), "mixed", "float64"),
+                                                  ((12, 13, 14), "traction", "float64"),
+                                                  ((22, 26), "clamped", "float32")])
+def test_a_damped_march_is_the_oracles_bit_for_bit(Nx, faces, precision):
+    print(f"anisotropic damped forward {Nx} {faces} {precision}", flush=True)
+    rng = np.random.default_rng(43)
+    sim = aniso_sim(Nx, faces, precision=precision)
+    gamma = (0.5 + rng.random(sim.Nx_padded)).astype(sim.np_dtype)
+    sim = damped(sim, gamma)
+    sources = shots(sim, rng, 2)
+    sensors = sensor_points(sim, rng)
+    field, record = AN.simulate_shots(sim, sources, gamma, sensors)
+    mat = sim.build_materials(gamma)
+    for b, source in enumerate(sources):
+        pos = np.asarray(source.position)
+        weight = sim.excitation_weights(mat, sim.flatten_indices(pos))
+        last, traces = O.simulate(logical(sim, mat["minv"]), logical(sim, mat["cell"]),
+                                  mat["table"], sim.np_dtype(sim.dt**2), sim.N, tuple(pos), weight,
+                                  np.asarray(source.signal, dtype=sim.np_dtype), tuple(sensors),
+                                  logical(sim, sim.damping), sim.dt)
+        assert np.array_equal(field[b].cpu().numpy(), last), f"shot {b}"
+        assert np.array_equal(record[b].cpu().numpy(), traces), f"shot {b}"
+
+
+def test_a_damped_gradient_matches_finite_differences():
+    print("anisotropic damped FD", flush=True)
+    Nx = (18, 20)
+    sim = damped(aniso_sim(Nx, "mixed", N=80), np.ones((18, 32)), width=3, beta=0.3)
+    rng = np.random.default_rng(44)
+    gamma = 0.7 + 0.6 * rng.random(sim.Nx_padded)
+    t = np.arange(sim.N) * sim.dt
+    f0 = 4.0 / (sim.N * sim.dt)
+    arg = (np.pi * f0 * (t - 1.2 / f0)) ** 2
+    pulse = (1 - 2 * arg) * np.exp(-arg)
+    source = Source(np.array([[1], [9], [10]]), pulse[:, None])
+    sensors = np.array([[0, 1, 1], [7, 11, 10], [8, 12, 11]])
+    observed = AN.simulate(sim, source, np.ones(sim.Nx_padded), sensors)[1].cpu().numpy()
+
+    def objective(um):
+        r = um.cpu().numpy() - observed
+        return 0.5 * float(np.sum(r * r)), torch.as_tensor(r)
+
+    cost, grads, _, _ = AN.sensitivity(sim, source, gamma, sensors, objective)
+    g = (grads["mass"] + grads["stiff"]).cpu().numpy()
+    h = 1e-6
+    for node in [(10, 11), (8, 9), (2, 2)]:
+        costs = []
+        for s in (1, -1):
+            probe = gamma.copy()
+            probe[node] += s * h
+            costs.append(objective(AN.simulate(sim, source, probe, sensors)[1])[0])
+        fd = (costs[0] - costs[1]) / (2 * h)
+        assert abs(g[node] - fd) <= 1e-5 * abs(fd) + 1e-10 * abs(cost), (node, g[node], fd)
