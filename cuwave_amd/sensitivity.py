@@ -272,6 +272,20 @@ def source_sensitivity(sim: Simulation, source: Source, indicator, sensors, obje
     Returns (cost, gradient, traces, {}), the gradient (N, source points) as
     `source.signal` is laid out.
     """
+    own = family_drivers(sim)
+    if own is not None:
+        # the vector Maxwell and cell-assembled adjoints are the forward step itself
+        # (no pml; damped or not): the load marched forward from the sensors and read
+        # at the sources, reversed in time
+        _require(sim, source, sensors)
+        _, um = own.simulate(sim, source, indicator, sensors)
+        cost, dphi = objective(um)
+        dphi = torch.as_tensor(dphi, dtype=sim.dtype, device=um.device)
+        w_in = torch.as_tensor(np.asarray(sim.adjoint_weights(sensors)), device=um.device)
+        load = Source(sensors, torch.flip(dphi, (0,)) / w_in)
+        _, lam = own.simulate(sim, load, indicator, source.position)
+        w = torch.as_tensor(np.asarray(sim.adjoint_weights(source.position)), device=lam.device)
+        return cost, lam.flip(0) * w, um, {}
     if _elastic(sim):
         raise NotImplementedError("source_sensitivity is built for the scalar equations")
     _require(sim, source, sensors)

@@ -312,3 +312,30 @@ def test_a_damped_gradient_matches_finite_differences():
             costs.append(objective(AN.simulate(sim, source, probe, sensors)[1])[0])
         fd = (costs[0] - costs[1]) / (2 * h)
         assert abs(g[node] - fd) <= 1e-5 * abs(fd) + 1e-10 * abs(cost), (node, g[node], fd)
+
+
+def test_the_source_gradient_matches_finite_differences():
+    # cuwave tests/sensitivity_test.py source_sensitivity: the adjoint read at the source
+    print(f"{FAMILY} source sensitivity", flush=True)
+    from cuwave_amd import sensitivity as S
+    from cuwave_amd.wave import family_drivers
+
+    sim, gamma = misfit_case()
+    rng = np.random.default_rng(51)
+    source = shots(sim, rng, 1)[0]
+    sensors = sensor_points(sim, rng, 4)
+    observed = 0.5 * rng.standard_normal((sim.N, sensors.shape[1]))
+    objective = S.l2_misfit(observed)
+    cost, grad, _, _ = S.source_sensitivity(sim, source, gamma, sensors, objective)
+    g = grad.cpu().numpy()
+    h = 1e-4
+    for t, k in [(3, 0), (sim.N // 2, 1), (sim.N - 5, 0)]:
+        costs = []
+        for sgn in (1, -1):
+            sig = np.array(source.signal, dtype=np.float64)
+            sig[t, k] += sgn * h
+            um = family_drivers(sim).simulate(sim, Source(source.position, sig), gamma,
+                                              sensors)[1]
+            costs.append(objective(um)[0])
+        fd = (costs[0] - costs[1]) / (2 * h)
+        assert abs(g[t, k] - fd) <= 1e-6 * abs(fd) + 1e-9 * abs(cost), (t, k, g[t, k], fd)
