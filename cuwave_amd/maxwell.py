@@ -560,6 +560,33 @@ def _reverse_sweep(sim: MaxwellWave, mat: dict, sources, sensors, objectives):
     return costs, grads, records, drift
 
 
+def _stored_sweep(sim: MaxwellWave, mat: dict, sources, sensors, objectives):
+    """As `_reverse_sweep`, the forward field stored for all N + 2 levels instead of
+    rebuilt: what a damped field, which does not run backwards, needs."""
+    eng = MaxwellEngine(sim, mat, len(sources))
+    src, signal = eng.sources(mat, sources)
+    rec = eng.sensors(sensors)
+    record = torch.zeros((sim.N, rec.count, eng.lanes), dtype=sim.dtype, device=eng.device)
+    N = sim.N
+    V = eng.levels(N + 2)
+    for t in range(N):
+        u = eng.step(V[t], V[t + 1], V[t + 2])
+        src.inject(u, signal, t)
+        rec.record(u, record, t)
+    records = eng.shots(record)
+    costs, dphi = objectives(records)
+    adjoint = eng.lane_table(adjoint_signal(sim, dphi, sensors))
+    adj = eng.sensors(sensors, weight=sim.excitation_weights(mat, sim.flatten_indices(sensors)))
+    grads = eng.gradient_fields()
+    mf = sim.np_dtype(sim.adjoint_mass_factor())
+    P = eng.levels(2)
+    for m in range(N):
+        l0, l1 = P[m % 2], P[1 - m % 2]
+        eng.adjoint_step(l0, l1, grads, V[N + 1 - m], mf)
+        adj.adjoint_inject(l0, adjoint, m, grads["mass"], V[N + 1 - m], mf)
+    return costs, grads, records, 0.0
+
+
 def reconstruction_sensitivity(sim: MaxwellWave, source: Source, indicator, sensors, objective,
                                strip_dtype: str | None = None, checkpoints: int | None = None):
     """Cost and gradients as `sensitivity`, the forward field rebuilt backwards instead of
@@ -587,7 +614,8 @@ def misfit_gradient(sim: MaxwellWave, sources, indicator, sensors, observed, obj
     """The misfit summed over the shots and its derivative with respect to the indicator
     (cuwave/utils.py:410-459), `batch` shots a sweep (all by default, at most 64): one
     forward sweep records every shot, one reverse sweep sums every shot's densities in
-    the kernels, one chain rule finishes the sum."""
+    the kernels, one chain rule finishes the sum. A damped field, which does not run
+    backwards, keeps the forward history instead."""
     if checkpoints is not None:
         raise NotImplementedError("nothing is stored on this family's reverse march: no "
                                   "checkpoints to choose")
@@ -608,7 +636,8 @@ def misfit_gradient(sim: MaxwellWave, sources, indicator, sensors, observed, obj
                 dphis.append(torch.as_tensor(dphi, dtype=sim.dtype, device=records.device))
             return costs, torch.stack(dphis)
 
-        costs, grads, records, _ = _reverse_sweep(sim, mat, chunk, nodes, objectives)
+        sweep = _stored_sweep if sim.damping is not None else _reverse_sweep
+        costs, grads, records, _ = sweep(sim, mat, chunk, nodes, objectives)
         cost += sum(costs)
         device = records.device
         total = grads if total is None else {k: total[k] + grads[k] for k in total}

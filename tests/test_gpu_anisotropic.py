@@ -339,3 +339,29 @@ def test_the_source_gradient_matches_finite_differences():
             costs.append(objective(um)[0])
         fd = (costs[0] - costs[1]) / (2 * h)
         assert abs(g[t, k] - fd) <= 1e-6 * abs(fd) + 1e-9 * abs(cost), (t, k, g[t, k], fd)
+
+
+def test_a_damped_misfit_gradient_sums_the_stored_gradients():
+    # a damped field keeps its history: the batch against the single-shot sensitivity
+    from helpers import Receivers
+
+    from cuwave_amd import sensitivity as S
+    from cuwave_amd import utils as U
+
+    print(f"{FAMILY} damped misfit gradient", flush=True)
+    sim, gamma = misfit_case()
+    sim = damped(sim, gamma, width=3, beta=0.2)
+    rng = np.random.default_rng(61)
+    sources = shots(sim, rng, 3)
+    sensors = sensor_points(sim, rng, 4)
+    observed = [0.5 * rng.standard_normal((sim.N, sensors.shape[1])) for _ in sources]
+    cost, gradient = U.misfit_gradient(sim, sources, gamma, Receivers(sensors), observed)
+    want_cost, want = 0.0, 0.0
+    jm, js = sim.parametrization_jacobian(gamma)
+    for source, obs in zip(sources, observed):
+        c, g, _, _ = S.sensitivity(sim, source, gamma, sensors, S.l2_misfit(obs))
+        want_cost += c
+        want = want + jm * g["mass"].cpu().numpy() + js * g["stiff"].cpu().numpy()
+    assert abs(cost - want_cost) <= 1e-12 * want_cost
+    got = gradient.cpu().numpy()
+    assert np.linalg.norm(got - want) <= 1e-10 * np.linalg.norm(want)
